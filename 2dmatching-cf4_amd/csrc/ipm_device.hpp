@@ -1,0 +1,380 @@
+// Device-side building blocks of the gfx950 particle hot path.
+//
+// Each routine is a restatement of the reference routine cited beside it,
+// written so that every product and sum is rounded in the order the Fortran
+// source evaluates it (the library is compiled with -ffp-contract=off, so no
+// FMA contraction moves a result).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "ipm_hip.h"
+
+namespace ipm {
+
+// R/src/base/Constants.F90:3-15
+constexpr double kPI = 3.141592653589793238;
+constexpr double kKB = 1.3807e-23;
+constexpr double kEXP = 2.71828;
+constexpr double kJTOEV = 1.6022e-19;
+constexpr double kEPS0 = 8.8542e-12;
+constexpr double kMINREAL = 1.0e-15;
+
+constexpr int kMatMetal = 1;       // R/src/pic/materials/Materials.F90:10-12
+constexpr int kMatDiel = 2;
+constexpr int kEdgeV2D = 1;        // R/src/base/Geometry.F90:15-19
+constexpr int kEdgeD2V = 3;
+
+// Z of a deleted slot.  No push from a position inside the grid can reach
+// it, so slots pushed to Z <= 0 and awaiting absorption stay distinguishable.
+constexpr double kDead = -1.0e300;
+
+struct DevTable {
+    int model, nreaction, nsigma, pad;
+    double emin, einterval, emax, ratio;
+    double gas_mass, gas_temperature, beta_max, pad2;
+    const int *rtype;
+    const int *resultant;
+    const double *threshold;
+    const double *prob;     // Probility(j,i) -> prob[(i-1)*nreaction + (j-1)]
+};
+
+// Device-resident description of the whole problem: one pointer is the only
+// kernel argument, so a captured graph keeps working after buffers swap.
+struct DevState {
+    int nz, nr, ns, ng;
+    double dx, dt, vfactor, inv_vfactor;
+    double *f[IPM_NS_MAX][8];          // Z R Vz Vr Vt Az Ar WQ
+    long long cap[IPM_NS_MAX];
+    double charge[IPM_NS_MAX], mass[IPM_NS_MAX], efactor[IPM_NS_MAX];
+    double *raw[IPM_NS_MAX];           // raw per-species deposit (nz*nr)
+    const double *Ez, *Er;
+    const int *cell_material, *hedge, *vedge, *mat_type;
+    int ndiel, nmetal;
+    double *metal_charge;              // nmetal
+    double *sigma;                     // nz*nr
+    unsigned long long *mcc_counts;    // optional: (s*ng+g)*128 + j
+    int count_events;
+    int pad;
+    // queue of particles selected for a collision (null-collision pass)
+    double *q_f[8];
+    long long *q_slot;
+    int *q_species;
+    long long q_cap;
+    DevTable tab[IPM_NS_MAX * IPM_NG_MAX];
+};
+
+// Mutable per-step counters (device memory).
+struct DevCounters {
+    unsigned long long n[IPM_NS_MAX];      // slots in use at step start
+    unsigned long long n_new[IPM_NS_MAX];  // products appended during the pass
+    unsigned long long n_dead[IPM_NS_MAX]; // dead slots among n
+    unsigned long long seed;
+    unsigned int step;
+    unsigned int blocks_done;
+    int overflow;                          // 1 capacity, 2 bad species, 4 queue
+    int pad;
+    unsigned long long q_count;            // entries in the collision queue
+};
+
+// ---------------------------------------------------------------- RNG
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t lo0 = 0xD2511F53u * c[0];
+        uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
+        uint32_t lo1 = 0xCD9E8D57u * c[2];
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
+        uint32_t n0 = hi1 ^ c[1] ^ k0;
+        uint32_t n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+}
+
+struct Rng {
+    uint32_t slot, step, tag, k;
+    uint32_t key0, key1;
+    __device__ __forceinline__ double next()
+    {
+        uint32_t c[4] = {slot, step, k, tag};
+        ++k;
+        philox4x32_10(c, key0, key1);
+        uint64_t x = ((((uint64_t)c[1]) << 32) | c[0]) >> 12;   // 52 bits
+        return ((double)x + 0.5) * 0x1p-52;                     // (0,1)
+    }
+};
+
+// ---------------------------------------------------------------- push
+struct Part {
+    double Z, R, Vz, Vr, Vt, Az, Ar, WQ;
+};
+
+// MoveESIm for one particle, R/src/pic/particle/ParticleBundle.f90:685-745.
+__device__ __forceinline__ void push_one(const DevState &st, double EFactor, Part &p)
+{
+    const int nz = st.nz, nr = st.nr;
+    const size_t ldz = (size_t)nz + 1;   // Ez(0:nz, 1:nr)
+    const size_t ldr = (size_t)nz;       // Er(1:nz, 0:nr)
+    const double *__restrict__ Ez = st.Ez;
+    const double *__restrict__ Er = st.Er;
+
+    const double z = p.Z, r = p.R;
+    int N1 = (int)ceil(z);
+    double Z1 = (double)N1 - z, Z2 = 1.0 - Z1;
+    int N1H = (int)(z + 0.5);
+    double Z1H = (double)N1H + 0.5 - z, Z2H = 1.0 - Z1H;
+    int N2 = (int)ceil(r);
+    double R1 = (double)N2 - r, R2 = 1.0 - R1;
+    int N2H = (int)(r + 0.5);
+    double R1H = (double)N2H + 0.5 - r, R2H = 1.0 - R1H;
+
+    // Index guards keep every read inside the arrays for any input; for the
+    // positions absorption leaves behind (0 < Z < Nz-1, 0 <= R < Nr-1) they
+    // are no-ops except R == 0 exactly, whose N2 row carries weight R1 = 0.
+    int i1 = min(max(N1, 1), nz - 1);
+    int i1h = min(max(N1H, 0), nz - 1);
+    int j2 = min(max(N2, 1), nr - 1);
+    int j2b = min(max(N2, 0), nr - 1) + 1;
+    int j2h = min(max(N2H, 0), nr - 1);
+
+    const double *er0 = Er + (size_t)(i1 - 1) + (size_t)j2h * ldr;
+    const double *ez0 = Ez + (size_t)i1h + (size_t)(j2 - 1) * ldz;
+    const double *ez1 = Ez + (size_t)i1h + (size_t)(j2b - 1) * ldz;
+    double E1 = er0[0] * Z1 * R1H + er0[1] * Z2 * R1H + er0[ldr] * Z1 * R2H + er0[ldr + 1] * Z2 * R2H;
+    double E2 = ez0[0] * Z1H * R1 + ez0[1] * Z2H * R1 + ez1[0] * Z1H * R2 + ez1[1] * Z2H * R2;
+    E1 = E1 * EFactor;
+    E2 = E2 * EFactor;
+
+    double vz = p.Vz + E2;
+    double zz = z + E2;
+    double az = (p.Az + E2) * 0.5;
+    double vr = p.Vr + E1;
+    double rr = r + E1;
+    double ar = (p.Ar + E1) * 0.5;
+
+    vz = vz + az;
+    zz = zz + vz;
+
+    double Vx = vr + ar;
+    double Vy = p.Vt;
+    double X = rr + Vx;
+    double Y = Vy;
+    double P_R = sqrt(X * X + Y * Y);
+    double vt;
+    if (P_R < kMINREAL) {
+        vr = sqrt(Vx * Vx + Vy * Vy);
+        vt = kMINREAL;
+        ar = fabs(ar);
+    } else {
+        double SinA = Y / P_R;
+        double CosA = X / P_R;
+        vr = Vx * CosA + Vy * SinA;
+        vt = -Vx * SinA + Vy * CosA;
+        ar = ar * CosA;
+    }
+    p.Z = zz; p.R = P_R; p.Vz = vz; p.Vr = vr; p.Vt = vt; p.Az = az; p.Ar = ar;
+}
+
+// ---------------------------------------------------------- absorption
+// MaterialAborptionParticle for one particle,
+// R/src/pic/particleboundary/ParticleMaterial.F90:22-85.
+// Returns true if the particle is deleted.  Reflection edits p.R/Vr/Ar.
+__device__ __forceinline__ bool absorb_one(const DevState &st, double charge, Part &p)
+{
+    const int nz = st.nz, nr = st.nr;
+    if (p.Z <= 0.0 || p.Z >= (double)(nz - 1) || p.R >= (double)(nr - 1)) return true;
+    if (p.R <= 0.0) {
+        p.R = fabs(p.R);
+        p.Vr = fabs(p.Vr);
+        p.Ar = fabs(p.Ar);
+        return false;
+    }
+    int NzL = (int)ceil(p.Z);
+    int NrL = (int)ceil(p.R);
+    int m = st.cell_material[(size_t)(NzL - 1) + (size_t)(NrL - 1) * (nz - 1)];
+    int t = st.mat_type[m + st.ndiel];
+    if (t == kMatMetal) {
+        unsafeAtomicAdd(&st.metal_charge[m - 1], p.WQ * charge);
+        return true;
+    }
+    if (t == kMatDiel) {
+        const int *he = st.hedge, *ve = st.vedge;
+        double *sg = st.sigma;
+        auto HE = [&](int i, int j) { return he[(size_t)(i - 1) + (size_t)(j - 1) * nz]; };
+        auto VE = [&](int i, int j) { return ve[(size_t)(i - 1) + (size_t)(j - 1) * (nz - 1)]; };
+        auto SG = [&](int i, int j) { return sg + (size_t)(i - 1) + (size_t)(j - 1) * nz; };
+        int h0 = HE(NzL, NrL), h1 = HE(NzL + 1, NrL);
+        int v0 = VE(NzL, NrL), v1 = VE(NzL, NrL + 1);
+        if (h0 == kEdgeV2D || h0 == kEdgeD2V) {
+            double RL = (double)NrL - p.R, RU = 1.0 - RL;
+            unsafeAtomicAdd(SG(NzL, NrL), RL * p.WQ * charge);
+            unsafeAtomicAdd(SG(NzL, NrL + 1), RU * p.WQ * charge);
+        } else if (h1 == kEdgeV2D || h1 == kEdgeD2V) {
+            double RL = (double)NrL - p.R, RU = 1.0 - RL;
+            unsafeAtomicAdd(SG(NzL + 1, NrL), RL * p.WQ * charge);
+            unsafeAtomicAdd(SG(NzL + 1, NrL + 1), RU * p.WQ * charge);
+        } else if (v0 == kEdgeV2D || v0 == kEdgeD2V) {
+            double ZL = (double)NzL - p.Z, ZU = 1.0 - ZL;
+            unsafeAtomicAdd(SG(NzL, NrL), ZL * p.WQ * charge);
+            unsafeAtomicAdd(SG(NzL + 1, NrL), ZU * p.WQ * charge);
+        } else if (v1 == kEdgeV2D || v1 == kEdgeD2V) {
+            double ZL = (double)NzL - p.Z, ZU = 1.0 - ZL;
+            unsafeAtomicAdd(SG(NzL, NrL + 1), ZL * p.WQ * charge);
+            unsafeAtomicAdd(SG(NzL + 1, NrL + 1), ZU * p.WQ * charge);
+        }
+        return true;
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------- MCC
+struct Mc {
+    double miu, mass_ratio, energy;
+    double gx, gy, gz, gper, g;
+    Part poi, pot;
+};
+
+// VecRanXYZInit, R/src/pic/particle/ParticleOne.f90:108-123
+__device__ __forceinline__ void vec_ran_xyz(Part &p, double V, Rng &rg)
+{
+    double R = rg.next();
+    double CosTheta = 1.0 - 2.0 * R;
+    double SinTheta = sqrt(1.0 - CosTheta * CosTheta);
+    R = rg.next();
+    double Phi = 2.0 * kPI * R;
+    double s, c;
+    sincos(Phi, &s, &c);
+    p.Vz = V * CosTheta;
+    p.Vr = V * SinTheta * c;
+    p.Vt = V * SinTheta * s;
+}
+
+// VelMaxInit, R/src/pic/particle/ParticleOne.f90:76-95
+__device__ __forceinline__ void vel_max_init(Part &p, double Mass, double Temperature, Rng &rg)
+{
+    double Beta = 1.0 / (kKB * Temperature);
+    double FuncA = 1.0, FuncB = 0.0, R = 0.5;
+    while (FuncA > FuncB) {
+        R = rg.next();
+        FuncA = R * R;
+        R = rg.next();
+        FuncB = -kEXP * R * log(R);
+    }
+    double V = sqrt(-3.0 * log(R) / Beta / Mass);
+    vec_ran_xyz(p, V, rg);
+}
+
+// Updater, R/src/mc/TypeMCC.f90:60-92
+__device__ __forceinline__ void mc_update(Mc &m, double sm, const DevTable &t, Rng &rg)
+{
+    const double gm = t.gas_mass;
+    m.pot = m.poi;
+    m.miu = sm * gm / (sm + gm);
+    m.mass_ratio = gm / (sm + gm);
+    vel_max_init(m.pot, gm, t.gas_temperature, rg);
+    m.gx = m.poi.Vr - m.pot.Vr;
+    m.gy = m.poi.Vt - m.pot.Vt;
+    m.gz = m.poi.Vz - m.pot.Vz;
+    double Gtemp = m.gy * m.gy + m.gz * m.gz;
+    m.gper = sqrt(Gtemp);
+    double G2 = Gtemp + m.gx * m.gx;
+    m.g = sqrt(G2);
+    m.energy = 0.5 * m.miu * G2 / kJTOEV;
+    m.pot.Ar = m.pot.Ar * sm / gm;
+    m.pot.Az = m.pot.Az * sm / gm;
+}
+
+// VelocityUpdater, R/src/mc/TypeMCC.f90:94-112
+__device__ __forceinline__ void mc_vel_update(Mc &m, double VF)
+{
+    m.gx = VF * m.gx; m.gy = VF * m.gy; m.gz = VF * m.gz;
+    m.gper = VF * m.gper; m.g = VF * m.g;
+    m.poi.Vr = VF * m.poi.Vr; m.poi.Vt = VF * m.poi.Vt; m.poi.Vz = VF * m.poi.Vz;
+}
+
+// PostCollisionVelocity, R/src/mc/TypeMCC.f90:114-146 with IsotropicCosKai or
+// CosKaiVahedi (R/src/mc/MCCChi.f90:5-25)
+__device__ __forceinline__ void post_collision(Mc &m, bool vahedi, Rng &rg)
+{
+    if (fabs(m.gper) < kMINREAL) {
+        vec_ran_xyz(m.poi, m.g, rg);
+        return;
+    }
+    double CosTheta;
+    if (vahedi) {
+        if (m.energy < kMINREAL) {
+            CosTheta = 1.0;
+        } else {
+            double R = rg.next();
+            CosTheta = (m.energy + 2.0 - 2.0 * pow(m.energy + 1.0, R)) / m.energy;
+        }
+    } else {
+        double R = rg.next();
+        CosTheta = 1.0 - 2.0 * R;
+    }
+    double FcosTheta = 1.0 - CosTheta;
+    double SinTheta = sqrt(1.0 - CosTheta * CosTheta);
+    double R = rg.next();
+    double APhi = 2.0 * kPI * R;
+    double SinPhi, CosPhi;
+    sincos(APhi, &SinPhi, &CosPhi);
+    const double gx = m.gx, gy = m.gy, gz = m.gz, g = m.g, gper = m.gper;
+    double hx = gper * CosPhi;
+    double hy = -(gx * gy * CosPhi + g * gz * SinPhi) / gper;
+    double hz = -(gx * gz * CosPhi - g * gy * SinPhi) / gper;
+    m.poi.Vr = m.poi.Vr - m.mass_ratio * (gx * FcosTheta + hx * SinTheta);
+    m.poi.Vt = m.poi.Vt - m.mass_ratio * (gy * FcosTheta + hy * SinTheta);
+    m.poi.Vz = m.poi.Vz - m.mass_ratio * (gz * FcosTheta + hz * SinTheta);
+}
+
+// SelectProbility, R/src/mc/TypeMCC.f90:173-216.  The uniform R is drawn
+// before the scan (the table read does not consume draws, so the order is
+// immaterial) so that no per-reaction temporary array is needed.
+__device__ __forceinline__ int select_probability(const Mc &m, const DevTable &t, Rng &rg)
+{
+    const int Nr = t.nreaction, Ns = t.nsigma;
+    const double E = m.energy;
+    const double *__restrict__ P = t.prob;
+    const double R = rg.next();
+    double lx = (log10(E) - t.emin) / t.einterval;
+    int N = (E > 0.0) ? (int)ceil(lx) : 0;
+    int mode;
+    double S1 = 0.0, S2 = 0.0;
+    const double *a, *b = nullptr;
+    if (N < 1) {
+        mode = 0; a = P;
+    } else if (N < Ns) {
+        mode = 1;
+        S1 = (double)N - (log10(E) - t.emin) / t.einterval;
+        S2 = 1.0 - S1;
+        a = P + (size_t)(N - 1) * Nr;
+        b = P + (size_t)N * Nr;
+    } else {
+        mode = 2; a = P + (size_t)(Ns - 1) * Nr;
+    }
+    for (int j = 0; j < Nr; ++j) {
+        double v;
+        if (mode == 0) v = a[j];
+        else if (mode == 1) v = a[j] * S1 + b[j] * S2;
+        else v = a[j] * E / t.emax;
+        if (E < t.threshold[j]) v = 0.0;
+        if (R < v) return j + 1;
+    }
+    return 0;
+}
+
+// SelectProbilityReactive, R/src/mc/TypeMCC.f90:218-236
+__device__ __forceinline__ int select_probability_reactive(const Mc &m, const DevTable &t, Rng &rg)
+{
+    double R = rg.next();
+    double beta = t.beta_max * sqrt(R);
+    if (beta <= 1.0) {
+        if (m.energy <= t.threshold[1]) return 1;
+        return select_probability(m, t, rg);
+    }
+    return 1;
+}
+
+}  // namespace ipm

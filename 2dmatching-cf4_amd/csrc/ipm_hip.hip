@@ -1,0 +1,1312 @@
+// gfx950 particle hot path of iPM2D: kernels and the C ABI of include/ipm_hip.h.
+//
+// Layout in HBM: every species owns two buffer sets (current / sort target)
+// of 8 FP64 arrays (Z R Vz Vr Vt Az Ar WQ) of its capacity, 256-byte aligned.
+// Deleted particles stay in place with Z = kDead until the next cell sort
+// compacts them; products of MCC are appended past the slots in use.
+//
+// One kernel template serves every per-particle entry point.  The fused step
+// reads each particle once (64 B) and writes it once (56 B): push, absorb,
+// deposit and MCC all happen in registers between the load and the store.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdarg.h>
+#include <vector>
+#include <algorithm>
+#include "ipm_device.hpp"
+
+using namespace ipm;
+
+namespace {
+
+constexpr int kBlock = 256;          // threads per block (4 wave64)
+constexpr int kItems = 4;            // particles per thread per tile
+constexpr int kTile = kBlock * kItems;
+constexpr int kWinZ = 64;            // LDS deposit window (nodes in z)
+constexpr int kWinR = 4;             // LDS deposit window (nodes in r)
+
+enum : int { M_PUSH = 1, M_ABSORB = 2, M_DEPOSIT = 4, M_MCC = 8 };
+
+__device__ __forceinline__ bool is_dead(double z) { return z < -1.0e299; }
+
+// ------------------------------------------------------------ products
+__device__ __forceinline__ void emit_product(const DevState &st, DevCounters *ctr, Part q, int d)
+{
+    if (d < 0 || d >= st.ns) {
+        atomicOr(&ctr->overflow, 2);
+        return;
+    }
+    q.Vz = q.Vz * st.inv_vfactor;
+    q.Vr = q.Vr * st.inv_vfactor;
+    q.Vt = q.Vt * st.inv_vfactor;
+    unsigned long long k = atomicAdd(&ctr->n_new[d], 1ull);
+    unsigned long long pos = ctr->n[d] + k;
+    if (pos >= (unsigned long long)st.cap[d]) {
+        atomicOr(&ctr->overflow, 1);
+        return;
+    }
+    double *const *F = st.f[d];
+    F[0][pos] = q.Z; F[1][pos] = q.R; F[2][pos] = q.Vz; F[3][pos] = q.Vr;
+    F[4][pos] = q.Vt; F[5][pos] = q.Az; F[6][pos] = q.Ar; F[7][pos] = q.WQ;
+}
+
+// PostVelocityIonReactive, R/src/mc/MCCIon.F90:54-74 (Vy from Vz as written there)
+__device__ __forceinline__ void ion_reactive(Mc &m, double th)
+{
+    double VF = sqrt(1.0 - th / m.energy);
+    m.energy = m.energy - th;
+    mc_vel_update(m, VF);
+    double A = m.mass_ratio, B = 1.0 - A;
+    m.poi.Vr = B * m.poi.Vr + A * m.pot.Vr + B * m.gx;
+    m.poi.Vt = B * m.poi.Vz + A * m.pot.Vt + B * m.gy;
+    m.poi.Vz = B * m.poi.Vz + A * m.pot.Vz + B * m.gz;
+}
+
+// One selected particle of species s against gas table t: the body of the
+// MCC loop, R/src/mc/MCCPublic.F90:61-97, with SelectCollisionElectron
+// (R/src/mc/MCCElectron.F90:13-147) and SelectCollisionIon (R/src/mc/MCCIon.F90:6-51).
+__device__ __forceinline__ void mcc_collide(const DevState &st, DevCounters *ctr, int s, int g,
+                                         const DevTable &t, Part &p, Rng &rg)
+{
+    Mc m;
+    m.poi = p;
+    m.poi.Vz = m.poi.Vz * st.vfactor;
+    m.poi.Vr = m.poi.Vr * st.vfactor;
+    m.poi.Vt = m.poi.Vt * st.vfactor;
+    mc_update(m, st.mass[s], t, rg);
+    int idx = 0;
+    if (t.model == 1 || t.model == 2) idx = select_probability(m, t, rg);
+    else if (t.model == 3) idx = select_probability_reactive(m, t, rg);
+    if (idx > 0) {
+        const int ty = t.rtype[idx - 1];
+        const int res = t.resultant[idx - 1];
+        const double th = t.threshold[idx - 1];
+        if (t.model == 1) {
+            switch (ty) {
+            case 11:
+                post_collision(m, true, rg);
+                break;
+            case 12: {
+                double VF = sqrt(1.0 - th / m.energy);
+                m.energy = m.energy - th;
+                mc_vel_update(m, VF);
+                post_collision(m, true, rg);
+            } break;
+            case 13: {
+                Part solid = m.poi;
+                double E0 = m.energy;
+                double EnergyTemp = m.energy - th;
+                double R = rg.next();
+                double EnergyOld = 10.0 * tan(R * atan(EnergyTemp) / 20.0);
+                if (fabs(EnergyOld) < kMINREAL) EnergyOld = kMINREAL;
+                double VF = sqrt(EnergyOld / m.energy);
+                mc_vel_update(m, VF);
+                post_collision(m, true, rg);
+                double EnergyNew = EnergyTemp - EnergyOld;
+                double VF2 = sqrt(EnergyNew / E0);
+                solid.Vz = solid.Vz * VF2;
+                solid.Vr = solid.Vr * VF2;
+                solid.Vt = solid.Vt * VF2;
+                emit_product(st, ctr, solid, s);
+                emit_product(st, ctr, m.pot, res);
+            } break;
+            case 14:
+                emit_product(st, ctr, m.pot, res);
+                break;
+            case 15: {
+                double VF = sqrt(1.0 - th / m.energy);
+                m.energy = m.energy - th;
+                mc_vel_update(m, VF);
+                post_collision(m, true, rg);
+                emit_product(st, ctr, m.pot, res % 10);
+                emit_product(st, ctr, m.pot, res / 10);
+            } break;
+            default:
+                break;
+            }
+        } else {
+            switch (ty) {
+            case 101: post_collision(m, false, rg); break;
+            case 102:
+            case 103: m.poi = m.pot; break;
+            case 111: ion_reactive(m, th); break;
+            case 112:
+                ion_reactive(m, th);
+                emit_product(st, ctr, m.pot, res);
+                break;
+            case 113:
+                ion_reactive(m, th);
+                emit_product(st, ctr, m.poi, res);
+                break;
+            default: break;   // 100 and -1 change nothing
+            }
+        }
+    }
+    if (st.count_events)
+        atomicAdd(&st.mcc_counts[((size_t)s * st.ng + g) * 128 + (idx & 127)], 1ull);
+    m.poi.Vz = m.poi.Vz * st.inv_vfactor;
+    m.poi.Vr = m.poi.Vr * st.inv_vfactor;
+    m.poi.Vt = m.poi.Vt * st.inv_vfactor;
+    p = m.poi;
+}
+
+__device__ __forceinline__ void lds_add(double *a, double v)
+{
+    __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// ------------------------------------------------------- particle kernel
+// Persistent launch over every (species, tile) pair.  Tiles are contiguous
+// slot ranges; the tile range is split into 8 contiguous parts, one per XCD
+// (blocks are dispatched to XCDs round-robin), so neighbouring tiles -- which
+// after a cell sort touch neighbouring grid nodes -- share one L2.
+template <int MODE>
+__global__ void __launch_bounds__(kBlock)
+particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr, int sel)
+{
+    const DevState &st = *stp;
+    __shared__ double win[kWinZ * kWinR];
+    __shared__ int anchor[2];
+    __shared__ unsigned long long dead_blk[IPM_NS_MAX];
+
+    const int ns = st.ns;
+    long long T = 0;
+    for (int s = 0; s < ns; ++s)
+        if (sel < 0 || sel == s) T += ((long long)ctr->n[s] + kTile - 1) / kTile;
+
+    if (MODE & M_DEPOSIT)
+        for (int q = threadIdx.x; q < kWinZ * kWinR; q += kBlock) win[q] = 0.0;
+    if (threadIdx.x < IPM_NS_MAX) dead_blk[threadIdx.x] = 0;
+    __syncthreads();
+
+    const unsigned G = gridDim.x;
+    long long t0, t1, tstride, tfirst;
+    if (G >= 8) {
+        const unsigned x = blockIdx.x & 7u;
+        const unsigned nb = (G - x + 7u) / 8u;        // blocks living on XCD x
+        t0 = T * x / 8;
+        t1 = T * (x + 1) / 8;
+        tfirst = t0 + blockIdx.x / 8u;
+        tstride = nb;
+    } else {
+        t0 = 0; t1 = T; tfirst = blockIdx.x; tstride = G;
+    }
+
+    const unsigned long long step = ctr->step;
+    const uint32_t key0 = (uint32_t)ctr->seed, key1 = (uint32_t)(ctr->seed >> 32);
+    const int lane = threadIdx.x & 63;
+
+    for (long long t = tfirst; t < t1; t += tstride) {
+        // species owning tile t: walk the (at most 8) per-species tile counts
+        int s = 0;
+        long long tbeg = 0;
+        for (int q = 0; q < ns; ++q) {
+            const long long nt = (sel < 0 || sel == q) ? ((long long)ctr->n[q] + kTile - 1) / kTile : 0;
+            if (t < tbeg + nt) { s = q; break; }
+            tbeg += nt;
+        }
+        const long long n_s = (long long)ctr->n[s];
+        const long long base = (t - tbeg) * kTile;
+        double *const *F = st.f[s];
+        const double charge = st.charge[s];
+        const double efactor = st.efactor[s];
+        double *__restrict__ raw = st.raw[s];
+
+        if (MODE & M_DEPOSIT) {
+            if (threadIdx.x == 0) {
+                double z0 = F[0][base], r0 = F[1][base];
+                if (is_dead(z0) || !(z0 > 0.0) || !(r0 >= 0.0) || z0 > 1e9 || r0 > 1e9) {
+                    anchor[0] = -1000000; anchor[1] = -1000000;
+                } else {
+                    anchor[0] = (int)ceil(z0) - 4;
+                    anchor[1] = (int)ceil(r0) - 1;
+                }
+            }
+            __syncthreads();
+        }
+        const int zw0 = (MODE & M_DEPOSIT) ? anchor[0] : 0;
+        const int rw0 = (MODE & M_DEPOSIT) ? anchor[1] : 0;
+
+#pragma unroll 1
+        for (int k = 0; k < kItems; ++k) {
+            const long long i = base + threadIdx.x + (long long)k * kBlock;
+            bool valid = i < n_s;
+            Part p;
+            bool live = false, deleted = false;
+            if (valid) {
+                p.Z = F[0][i];
+                live = !is_dead(p.Z);
+            }
+            if (live) {
+                if (MODE & (M_PUSH | M_ABSORB | M_DEPOSIT)) {
+                    p.R = F[1][i];
+                    p.WQ = F[7][i];
+                }
+                if (MODE & (M_PUSH | M_ABSORB)) {
+                    p.Vz = F[2][i]; p.Vr = F[3][i]; p.Vt = F[4][i];
+                    p.Az = F[5][i]; p.Ar = F[6][i];
+                }
+                if (MODE & M_PUSH) push_one(st, efactor, p);
+                if (MODE & M_ABSORB) deleted = absorb_one(st, charge, p);
+            }
+            const bool alive = live && !deleted;
+
+            if ((MODE & M_DEPOSIT) && alive) {
+                // P2CES, R/src/pic/field/FieldOne.f90:151-166
+                int NzL = (int)ceil(p.Z);
+                double ZL = (double)NzL - p.Z, ZU = 1.0 - ZL;
+                int NrL = (int)ceil(p.R);
+                double RL = (double)NrL - p.R, RU = 1.0 - RL;
+                NzL = min(max(NzL, 1), st.nz - 1);
+                const bool row0 = NrL < 1;                // R == 0 exactly: weight RL is 0
+                NrL = min(max(NrL, 1), st.nr - 1);
+                double wLL = ZL * RL * p.WQ, wLU = ZL * RU * p.WQ;
+                double wUL = ZU * RL * p.WQ, wUU = ZU * RU * p.WQ;
+                if (row0) {       // the reference's row 0 terms (weight 0) vanish; row 1 takes RU terms
+                    wLL = wLU; wUL = wUU; wLU = 0.0; wUU = 0.0;
+                }
+                const int wz = NzL - zw0, wr = NrL - rw0;
+                if (wz >= 0 && wz + 1 < kWinZ && wr >= 0 && wr + 1 < kWinR) {
+                    double *w = win + wz + wr * kWinZ;
+                    lds_add(w, wLL);
+                    lds_add(w + kWinZ, wLU);
+                    lds_add(w + 1, wUL);
+                    lds_add(w + 1 + kWinZ, wUU);
+                } else {
+                    double *g0 = raw + (size_t)(NzL - 1) + (size_t)(NrL - 1) * st.nz;
+                    unsafeAtomicAdd(g0, wLL);
+                    unsafeAtomicAdd(g0 + st.nz, wLU);
+                    unsafeAtomicAdd(g0 + 1, wUL);
+                    unsafeAtomicAdd(g0 + 1 + st.nz, wUU);
+                }
+            }
+
+            // Null-collision selection (SelectParticleVW, R/src/mc/MCCPublic.F90:123-155):
+            // draw 0 of the (slot, step, species|gas<<8) stream against
+            // CollisionRatio.  Selected particles go to the collision queue
+            // with their post-push state; collide_kernel finishes them.
+            bool selected = false;
+            if ((MODE & M_MCC) && alive) {
+                for (int g = 0; g < st.ng; ++g) {
+                    const DevTable &tb = st.tab[s * st.ng + g];
+                    if (tb.model == 0) continue;
+                    Rng rg{(uint32_t)i, (uint32_t)step, (uint32_t)s | ((uint32_t)g << 8), 0u, key0, key1};
+                    if (rg.next() < tb.ratio) selected = true;
+                }
+            }
+            if (MODE & M_MCC) {
+                const unsigned long long m = __ballot(selected);
+                if (m) {
+                    const int leader = __ffsll((long long)m) - 1;
+                    unsigned long long qb = 0;
+                    if (lane == leader) qb = atomicAdd(&ctr->q_count, (unsigned long long)__popcll(m));
+                    qb = __shfl(qb, leader);
+                    if (selected) {
+                        const unsigned long long q = qb + __popcll(m & ((1ull << lane) - 1ull));
+                        if (q < (unsigned long long)st.q_cap) {
+                            if (!(MODE & (M_PUSH | M_ABSORB | M_DEPOSIT))) {
+                                p.R = F[1][i]; p.WQ = F[7][i];
+                                p.Vz = F[2][i]; p.Vr = F[3][i]; p.Vt = F[4][i];
+                                p.Az = F[5][i]; p.Ar = F[6][i];
+                            } else if (!(MODE & (M_PUSH | M_ABSORB))) {
+                                p.Vz = F[2][i]; p.Vr = F[3][i]; p.Vt = F[4][i];
+                                p.Az = F[5][i]; p.Ar = F[6][i];
+                            }
+                            st.q_f[0][q] = p.Z; st.q_f[1][q] = p.R; st.q_f[2][q] = p.Vz; st.q_f[3][q] = p.Vr;
+                            st.q_f[4][q] = p.Vt; st.q_f[5][q] = p.Az; st.q_f[6][q] = p.Ar; st.q_f[7][q] = p.WQ;
+                            st.q_slot[q] = i;
+                            st.q_species[q] = s;
+                        } else {
+                            atomicOr(&ctr->overflow, 4);
+                        }
+                    }
+                }
+            }
+
+            if (MODE & M_ABSORB) {
+                const unsigned long long m = __ballot(deleted);
+                if (m && lane == (int)(__ffsll((long long)m) - 1))
+                    atomicAdd(&dead_blk[s], (unsigned long long)__popcll(m));
+            }
+            if (deleted) {
+                F[0][i] = kDead;
+            } else if (alive) {
+                if (MODE & M_PUSH) {
+                    F[0][i] = p.Z; F[1][i] = p.R; F[2][i] = p.Vz; F[3][i] = p.Vr;
+                    F[4][i] = p.Vt; F[5][i] = p.Az; F[6][i] = p.Ar;
+                } else if (MODE & M_ABSORB) {
+                    F[1][i] = p.R; F[3][i] = p.Vr; F[6][i] = p.Ar;
+                }
+            }
+        }
+
+        if (MODE & M_DEPOSIT) {
+            __syncthreads();
+            for (int q = threadIdx.x; q < kWinZ * kWinR; q += kBlock) {
+                const double v = win[q];
+                if (v != 0.0) {
+                    const int iz = zw0 + (q % kWinZ), ir = rw0 + (q / kWinZ);
+                    unsafeAtomicAdd(raw + (size_t)(iz - 1) + (size_t)(ir - 1) * st.nz, v);
+                    win[q] = 0.0;
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    if (MODE & M_ABSORB) {
+        __syncthreads();
+        if (threadIdx.x < (unsigned)ns && dead_blk[threadIdx.x])
+            atomicAdd(&ctr->n_dead[threadIdx.x], dead_blk[threadIdx.x]);
+    }
+
+}
+
+// ------------------------------------------------------- collision pass
+// Finishes every queued particle: for each gas in order, the selection draw
+// is repeated and, if it hits, the collision of MCCPublic.F90:61-97 follows
+// on the state the previous gas left.  Results go back to the particle's
+// slot; products are appended to their species (MCCAddNewParticle,
+// R/src/mc/MCCPublic.F90:209-226).  Launched with a fixed grid that loops
+// over the queue length read from device memory, so it can be captured.
+__global__ void __launch_bounds__(kBlock)
+collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
+{
+    const DevState &st = *stp;
+    const unsigned long long nq_all = ctr->q_count;
+    const unsigned long long nq = nq_all < (unsigned long long)st.q_cap ? nq_all : (unsigned long long)st.q_cap;
+    const unsigned step = ctr->step;
+    const uint32_t key0 = (uint32_t)ctr->seed, key1 = (uint32_t)(ctr->seed >> 32);
+    const unsigned long long stride = (unsigned long long)gridDim.x * kBlock;
+    for (unsigned long long q = (unsigned long long)blockIdx.x * kBlock + threadIdx.x; q < nq; q += stride) {
+        Part p;
+        p.Z = st.q_f[0][q]; p.R = st.q_f[1][q]; p.Vz = st.q_f[2][q]; p.Vr = st.q_f[3][q];
+        p.Vt = st.q_f[4][q]; p.Az = st.q_f[5][q]; p.Ar = st.q_f[6][q]; p.WQ = st.q_f[7][q];
+        const long long i = st.q_slot[q];
+        const int s = st.q_species[q];
+        for (int g = 0; g < st.ng; ++g) {
+            const DevTable &tb = st.tab[s * st.ng + g];
+            if (tb.model == 0) continue;
+            Rng rg{(uint32_t)i, step, (uint32_t)s | ((uint32_t)g << 8), 0u, key0, key1};
+            if (!(rg.next() < tb.ratio)) continue;
+            mcc_collide(st, ctr, s, g, tb, p, rg);
+        }
+        double *const *F = st.f[s];
+        F[2][i] = p.Vz; F[3][i] = p.Vr; F[4][i] = p.Vt; F[5][i] = p.Az; F[6][i] = p.Ar;
+    }
+    // The last block to finish folds this pass's products into n[], empties
+    // the queue and advances the random stream's step key.
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        unsigned prev = atomicAdd(&ctr->blocks_done, 1u);
+        if (prev == gridDim.x - 1) {
+            __threadfence();
+            for (int s = 0; s < st.ns; ++s) {
+                unsigned long long add = atomicAdd(&ctr->n_new[s], 0ull);
+                unsigned long long n0 = ctr->n[s];
+                unsigned long long cap = (unsigned long long)st.cap[s];
+                ctr->n[s] = (n0 + add > cap) ? cap : n0 + add;
+                ctr->n_new[s] = 0;
+            }
+            ctr->q_count = 0;
+            ctr->step = ctr->step + 1;
+            ctr->blocks_done = 0;
+            __threadfence();
+        }
+    }
+}
+
+// ------------------------------------------------------- grid epilogue
+// P2CES epilogue (R/src/pic/field/FieldOne.f90:168-175) for species in
+// [s0, s1) and, if sum != 0, FieldSource%Sum (R/src/pic/field/FieldSource.f90:77-87).
+// Raw deposits are cleared here so that no memset enters a captured graph.
+__global__ void grid_kernel(const DevState *__restrict__ stp, const double *__restrict__ inv_vol,
+                            const double *__restrict__ chifac, double *__restrict__ rho_one,
+                            double *__restrict__ chi_one, double *__restrict__ rho_tot,
+                            double *__restrict__ chi_tot, int s0, int s1, int sum)
+{
+    const DevState &st = *stp;
+    const int nz = st.nz, nr = st.nr;
+    const size_t nn = (size_t)nz * nr;
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t ng = (size_t)(nz + 2) * (nr + 2);
+    if (tid < nn) {
+        for (int s = s0; s < s1; ++s) {
+            double v = st.raw[s][tid] * inv_vol[tid];
+            v = v * st.charge[s];
+            rho_one[(size_t)s * nn + tid] = v;
+            chi_one[(size_t)s * nn + tid] = v * chifac[s];
+            st.raw[s][tid] = 0.0;
+        }
+        if (sum) {
+            double r = 0.0, c = 0.0;
+            for (int s = 0; s < st.ns; ++s) {
+                r = r + rho_one[(size_t)s * nn + tid];
+                c = c + chi_one[(size_t)s * nn + tid];
+            }
+            rho_tot[tid] = r;
+            const size_t iz = tid % nz, ir = tid / nz;
+            chi_tot[(iz + 1) + (ir + 1) * (size_t)(nz + 2)] = c;
+        }
+    }
+    if (sum && tid < ng) {
+        const size_t iz = tid % (nz + 2), ir = tid / (nz + 2);
+        if (iz == 0 || ir == 0 || iz == (size_t)nz + 1 || ir == (size_t)nr + 1) chi_tot[tid] = 0.0;
+    }
+}
+
+__global__ void clear_kernel(double *a, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0.0;
+}
+
+// ------------------------------------------------------------ cell sort
+__device__ __forceinline__ unsigned long long cell_of(const DevState &st, double z, double r)
+{
+    int iz = min(max((int)ceil(z), 1), st.nz - 1);
+    int ir = min(max((int)ceil(r), 1), st.nr - 1);
+    return (unsigned long long)(iz - 1) + (unsigned long long)(ir - 1) * (st.nz - 1);
+}
+
+// Wave-aggregated increment: lanes with equal keys share one atomic.
+// Returns each lane's position among the increments made to its key.
+__device__ __forceinline__ unsigned long long wave_bump(unsigned long long *ctr, unsigned long long key,
+                                                        bool valid)
+{
+    const int lane = threadIdx.x & 63;
+    unsigned long long active = __ballot(valid);
+    unsigned long long mine = 0;
+    while (active) {
+        const int leader = __ffsll((long long)active) - 1;
+        const unsigned long long lkey = __shfl(key, leader);
+        const unsigned long long same = __ballot(valid && key == lkey) & active;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(&ctr[lkey], (unsigned long long)__popcll(same));
+        base = __shfl(base, leader);
+        if ((same >> lane) & 1ull) mine = base + __popcll(same & ((1ull << lane) - 1ull));
+        active &= ~same;
+    }
+    return mine;
+}
+
+__global__ void __launch_bounds__(kBlock)
+sort_hist_kernel(const DevState *__restrict__ stp, const DevCounters *__restrict__ ctr, int s,
+                 unsigned long long *__restrict__ hist)
+{
+    const DevState &st = *stp;
+    const long long n = (long long)ctr->n[s];
+    const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+    bool valid = false;
+    unsigned long long key = 0;
+    if (i < n) {
+        double z = st.f[s][0][i];
+        if (!is_dead(z)) {
+            valid = true;
+            key = cell_of(st, z, st.f[s][1][i]);
+        }
+    }
+    (void)wave_bump(hist, key, valid);
+}
+
+__global__ void __launch_bounds__(kBlock)
+sort_scatter_kernel(const DevState *__restrict__ stp, const DevCounters *__restrict__ ctr, int s,
+                    const unsigned long long *__restrict__ offs, unsigned long long *__restrict__ cursor,
+                    double *const *__restrict__ dst)
+{
+    const DevState &st = *stp;
+    const long long n = (long long)ctr->n[s];
+    const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+    bool valid = false;
+    unsigned long long key = 0;
+    double z = 0.0, r = 0.0;
+    if (i < n) {
+        z = st.f[s][0][i];
+        if (!is_dead(z)) {
+            valid = true;
+            r = st.f[s][1][i];
+            key = cell_of(st, z, r);
+        }
+    }
+    unsigned long long rank = wave_bump(cursor, key, valid);
+    if (valid) {
+        const unsigned long long pos = offs[key] + rank;
+        dst[0][pos] = z;
+        dst[1][pos] = r;
+        for (int f = 2; f < 8; ++f) dst[f][pos] = st.f[s][f][i];
+    }
+}
+
+// Exclusive scan of n counts: per-block scan + block totals, then offsets.
+constexpr int kScanBlock = 1024;
+__global__ void __launch_bounds__(kScanBlock)
+scan_block_kernel(const unsigned long long *__restrict__ in, unsigned long long *__restrict__ out,
+                  unsigned long long *__restrict__ partial, size_t n)
+{
+    __shared__ unsigned long long wsum[kScanBlock / 64];
+    const size_t i = (size_t)blockIdx.x * kScanBlock + threadIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned long long v = (i < n) ? in[i] : 0ull;
+    unsigned long long x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        unsigned long long y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    if (w == 0) {
+        unsigned long long a = (lane < kScanBlock / 64) ? wsum[lane] : 0ull;
+        for (int o = 1; o < 64; o <<= 1) {
+            unsigned long long y = __shfl_up(a, o);
+            if (lane >= o) a += y;
+        }
+        if (lane < kScanBlock / 64) wsum[lane] = a;
+    }
+    __syncthreads();
+    const unsigned long long before = (w > 0) ? wsum[w - 1] : 0ull;
+    if (i < n) out[i] = before + x - v;
+    if (threadIdx.x == kScanBlock - 1) partial[blockIdx.x] = before + x;
+}
+
+// add[b] = exclusive prefix of the block totals
+__global__ void scan_add_kernel(unsigned long long *__restrict__ out, const unsigned long long *__restrict__ add,
+                                size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * kScanBlock + threadIdx.x;
+    if (i < n) out[i] += add[blockIdx.x];
+}
+
+__global__ void set_count_kernel(DevCounters *ctr, int s, const unsigned long long *offs,
+                                 const unsigned long long *hist, size_t ncell)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctr->n[s] = offs[ncell - 1] + hist[ncell - 1];
+        ctr->n_dead[s] = 0;
+    }
+}
+
+}  // namespace
+
+// ====================================================================
+//                              host side
+// ====================================================================
+static thread_local char g_err[512] = "";
+
+static int fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(x)                                                                      \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess)                                                          \
+            return fail(IPM_EHIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_),  \
+                        __FILE__, __LINE__);                                           \
+    } while (0)
+
+struct ipm_ctx {
+    int device = 0;
+    int num_cu = 256;
+    hipStream_t stream = nullptr;
+    int nz = 0, nr = 0, ns = 0, ng = 0;
+    double dx = 0, dt = 0;
+    DevState h{};
+    DevState *d_state = nullptr;
+    DevCounters *d_ctr = nullptr;
+    double *buf[2][IPM_NS_MAX] = {};
+    int cur[IPM_NS_MAX] = {};
+    bool species_set[IPM_NS_MAX] = {};
+    double chifac[IPM_NS_MAX] = {};
+    double *d_chifac = nullptr;
+    double *raw_all = nullptr, *rho_one = nullptr, *chi_one = nullptr;
+    double *rho_tot = nullptr, *chi_tot = nullptr;
+    double *inv_vol = nullptr;
+    int *cell_material = nullptr, *hedge = nullptr, *vedge = nullptr, *mat_type = nullptr;
+    double *metal_charge = nullptr, *sigma = nullptr;
+    double *Ez = nullptr, *Er = nullptr;
+    unsigned long long *mcc_counts = nullptr;
+    std::vector<void *> tab_mem[IPM_NS_MAX * IPM_NG_MAX];
+    unsigned long long *hist = nullptr, *offs = nullptr, *cursor = nullptr, *partial = nullptr,
+                       *partial2 = nullptr;
+    double **d_dst = nullptr;
+    double *qbuf = nullptr;
+    long long *q_slot = nullptr;
+    int *q_species = nullptr;
+    long long q_cap = 0;
+    hipGraphExec_t graph = nullptr;
+    int block = kBlock, blocks_per_cu = 4;
+    // optional HIP-event timing of the kernels ipm_step enqueues
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, int>> ev_pairs[3];   // (start, stop) per kernel class
+    size_t ev_used = 0;
+    bool geometry_set = false, fields_set = false;
+};
+
+static int sync_state(ipm_ctx *c)
+{
+    HIPCHK(hipMemcpyAsync(c->d_state, &c->h, sizeof(DevState), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return IPM_OK;
+}
+
+// Collision queue sized for the expected selections with a wide margin:
+// capacity_s * min(1, 2 * sum_g CollisionRatio) + 64k entries.
+static int ensure_queue(ipm_ctx *c)
+{
+    long long need = 65536;
+    for (int s = 0; s < c->ns; ++s) {
+        if (!c->species_set[s]) continue;
+        double r = 0.0;
+        for (int g = 0; g < c->ng; ++g)
+            if (c->h.tab[s * c->ng + g].model != 0) r += c->h.tab[s * c->ng + g].ratio;
+        double f = std::min(1.0, 2.0 * r);
+        need += (long long)((double)c->h.cap[s] * f) + 1;
+    }
+    if (need <= c->q_cap) return IPM_OK;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->qbuf) HIPCHK(hipFree(c->qbuf));
+    if (c->q_slot) HIPCHK(hipFree(c->q_slot));
+    if (c->q_species) HIPCHK(hipFree(c->q_species));
+    c->qbuf = nullptr; c->q_slot = nullptr; c->q_species = nullptr; c->q_cap = 0;
+    const size_t cap = ((size_t)need + 31) & ~(size_t)31;
+    HIPCHK(hipMalloc(&c->qbuf, sizeof(double) * 8 * cap));
+    HIPCHK(hipMalloc(&c->q_slot, sizeof(long long) * cap));
+    HIPCHK(hipMalloc(&c->q_species, sizeof(int) * cap));
+    c->q_cap = (long long)cap;
+    for (int f = 0; f < 8; ++f) c->h.q_f[f] = c->qbuf + (size_t)f * cap;
+    c->h.q_slot = c->q_slot;
+    c->h.q_species = c->q_species;
+    c->h.q_cap = c->q_cap;
+    if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    return IPM_OK;
+}
+
+static int read_counters(ipm_ctx *c, DevCounters *out)
+{
+    HIPCHK(hipMemcpyAsync(out, c->d_ctr, sizeof(DevCounters), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (out->overflow & 1)
+        return fail(IPM_ECAPACITY, "The size of particle bundle reaches the maximum (MCC products dropped).");
+    if (out->overflow & 2) return fail(IPM_EINVAL, "MCC product with an invalid species index.");
+    if (out->overflow & 4) return fail(IPM_ECAPACITY, "collision queue overflow (selections were not collided)");
+    return IPM_OK;
+}
+
+static int check_ready(ipm_ctx *c, bool need_fields, bool need_geom)
+{
+    if (!c) return fail(IPM_EINVAL, "null context");
+    for (int s = 0; s < c->ns; ++s)
+        if (!c->species_set[s]) return fail(IPM_EINVAL, "species %d has not been set", s);
+    if (need_fields && !c->fields_set) return fail(IPM_EINVAL, "fields have not been set");
+    if (need_geom && !c->geometry_set) return fail(IPM_EINVAL, "geometry has not been set");
+    return IPM_OK;
+}
+
+static int grid_blocks(ipm_ctx *c)
+{
+    int g = c->num_cu * c->blocks_per_cu;
+    return g < 8 ? 8 : g;
+}
+
+template <int MODE>
+static int launch_particles(ipm_ctx *c, int sel)
+{
+    hipLaunchKernelGGL(particle_kernel<MODE>, dim3(grid_blocks(c)), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr,
+                       sel);
+    HIPCHK(hipGetLastError());
+    return IPM_OK;
+}
+
+static int launch_collide(ipm_ctx *c)
+{
+    hipLaunchKernelGGL(collide_kernel, dim3((unsigned)(c->num_cu * 4)), dim3(kBlock), 0, c->stream, c->d_state,
+                       c->d_ctr);
+    HIPCHK(hipGetLastError());
+    return IPM_OK;
+}
+
+static int launch_grid(ipm_ctx *c, int s0, int s1, int sum)
+{
+    const size_t nn = (size_t)c->nz * c->nr;
+    const size_t ngh = (size_t)(c->nz + 2) * (c->nr + 2);
+    const size_t nt = std::max(nn, ngh);
+    hipLaunchKernelGGL(grid_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, c->d_state, c->inv_vol,
+                       c->d_chifac, c->rho_one, c->chi_one, c->rho_tot, c->chi_tot, s0, s1, sum);
+    HIPCHK(hipGetLastError());
+    return IPM_OK;
+}
+
+extern "C" {
+
+const char *ipm_last_error(void) { return g_err; }
+
+int ipm_create(ipm_ctx **out, int nz, int nr, double dx, double dt, int ns, int ng)
+{
+    if (!out) return fail(IPM_EINVAL, "null output pointer");
+    *out = nullptr;
+    if (nz < 3 || nr < 3 || !(dx > 0) || !(dt > 0) || ns < 1 || ns > IPM_NS_MAX || ng < 0 ||
+        ng > IPM_NG_MAX)
+        return fail(IPM_EINVAL, "invalid grid/species: nz=%d nr=%d dx=%g dt=%g ns=%d ng=%d", nz, nr, dx,
+                    dt, ns, ng);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(IPM_ENODEV, "no HIP device");
+    ipm_ctx *c = new ipm_ctx();
+    HIPCHK(hipGetDevice(&c->device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, c->device));
+    c->num_cu = prop.multiProcessorCount;
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->nz = nz; c->nr = nr; c->ns = ns; c->ng = ng; c->dx = dx; c->dt = dt;
+    DevState &h = c->h;
+    h.nz = nz; h.nr = nr; h.ns = ns; h.ng = ng; h.dx = dx; h.dt = dt;
+    h.vfactor = dx / dt;               // ParticleBundle.f90:346
+    h.inv_vfactor = 1.0 / h.vfactor;   // MCCPublic.F90:90 multiplies by 1/VFactor
+    const size_t nn = (size_t)nz * nr;
+    HIPCHK(hipMalloc(&c->d_state, sizeof(DevState)));
+    HIPCHK(hipMalloc(&c->d_ctr, sizeof(DevCounters)));
+    HIPCHK(hipMemset(c->d_ctr, 0, sizeof(DevCounters)));
+    HIPCHK(hipMalloc(&c->raw_all, sizeof(double) * nn * ns));
+    HIPCHK(hipMemset(c->raw_all, 0, sizeof(double) * nn * ns));
+    HIPCHK(hipMalloc(&c->rho_one, sizeof(double) * nn * ns));
+    HIPCHK(hipMemset(c->rho_one, 0, sizeof(double) * nn * ns));
+    HIPCHK(hipMalloc(&c->chi_one, sizeof(double) * nn * ns));
+    HIPCHK(hipMemset(c->chi_one, 0, sizeof(double) * nn * ns));
+    HIPCHK(hipMalloc(&c->rho_tot, sizeof(double) * nn));
+    HIPCHK(hipMalloc(&c->chi_tot, sizeof(double) * (nz + 2) * (nr + 2)));
+    HIPCHK(hipMalloc(&c->d_chifac, sizeof(double) * IPM_NS_MAX));
+    HIPCHK(hipMalloc(&c->Ez, sizeof(double) * (nz + 1) * nr));
+    HIPCHK(hipMalloc(&c->Er, sizeof(double) * nz * (nr + 1)));
+    HIPCHK(hipMemset(c->Ez, 0, sizeof(double) * (nz + 1) * nr));
+    HIPCHK(hipMemset(c->Er, 0, sizeof(double) * nz * (nr + 1)));
+    HIPCHK(hipMalloc(&c->sigma, sizeof(double) * nn));
+    HIPCHK(hipMemset(c->sigma, 0, sizeof(double) * nn));
+    HIPCHK(hipMalloc(&c->mcc_counts, sizeof(unsigned long long) * IPM_NS_MAX * IPM_NG_MAX * 128));
+    HIPCHK(hipMemset(c->mcc_counts, 0, sizeof(unsigned long long) * IPM_NS_MAX * IPM_NG_MAX * 128));
+    const size_t ncell = (size_t)(nz - 1) * (nr - 1);
+    const size_t nblk = (ncell + kScanBlock - 1) / kScanBlock;
+    HIPCHK(hipMalloc(&c->hist, sizeof(unsigned long long) * ncell));
+    HIPCHK(hipMalloc(&c->offs, sizeof(unsigned long long) * ncell));
+    HIPCHK(hipMalloc(&c->cursor, sizeof(unsigned long long) * ncell));
+    HIPCHK(hipMalloc(&c->partial, sizeof(unsigned long long) * (nblk + 1)));
+    HIPCHK(hipMalloc(&c->partial2, sizeof(unsigned long long) * (nblk + 1)));
+    HIPCHK(hipMalloc(&c->d_dst, sizeof(double *) * 8));
+    for (int s = 0; s < ns; ++s) h.raw[s] = c->raw_all + (size_t)s * nn;
+    h.Ez = c->Ez; h.Er = c->Er; h.sigma = c->sigma; h.mcc_counts = c->mcc_counts;
+    // no geometry yet: a 1-metal, all-vacuum placeholder keeps pointers valid
+    h.ndiel = 0; h.nmetal = 0;
+    int rc = sync_state(c);
+    if (rc) { ipm_destroy(c); return rc; }
+    *out = c;
+    return IPM_OK;
+}
+
+int ipm_destroy(ipm_ctx *c)
+{
+    if (!c) return IPM_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    for (int b = 0; b < 2; ++b)
+        for (int s = 0; s < IPM_NS_MAX; ++s)
+            if (c->buf[b][s]) (void)hipFree(c->buf[b][s]);
+    for (auto &v : c->tab_mem)
+        for (void *p : v) (void)hipFree(p);
+    void *ptrs[] = {c->d_state, c->d_ctr, c->raw_all, c->rho_one, c->chi_one, c->rho_tot, c->chi_tot,
+                    c->d_chifac, c->inv_vol, c->cell_material, c->hedge, c->vedge, c->mat_type,
+                    c->metal_charge, c->sigma, c->Ez, c->Er, c->mcc_counts, c->hist, c->offs,
+                    c->cursor, c->partial, c->partial2, c->d_dst, c->qbuf, c->q_slot, c->q_species};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return IPM_OK;
+}
+
+int ipm_set_species(ipm_ctx *c, int s, double charge, double mass, int64_t capacity)
+{
+    if (!c || s < 0 || s >= c->ns) return fail(IPM_EINVAL, "invalid species index %d", s);
+    if (!(mass > 0) || capacity < 1) return fail(IPM_EINVAL, "invalid mass/capacity for species %d", s);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int b = 0; b < 2; ++b) {
+        if (c->buf[b][s]) HIPCHK(hipFree(c->buf[b][s]));
+        c->buf[b][s] = nullptr;
+    }
+    const size_t cap = ((size_t)capacity + 31) & ~(size_t)31;   // 256-byte aligned arrays
+    for (int b = 0; b < 2; ++b) HIPCHK(hipMalloc(&c->buf[b][s], sizeof(double) * 8 * cap));
+    c->cur[s] = 0;
+    for (int f = 0; f < 8; ++f) c->h.f[s][f] = c->buf[0][s] + (size_t)f * cap;
+    c->h.cap[s] = (long long)capacity;
+    c->h.charge[s] = charge;
+    c->h.mass[s] = mass;
+    // MoveESIm: EFactor = Charge/Mass*dt/(dx/dt)*0.5 (ParticleBundle.f90:682)
+    c->h.efactor[s] = charge / mass * c->dt / (c->dx / c->dt) * 0.5;
+    // ResetFieldOne: ChiFactor = 0.5*Charge/Mass*dt*dt/Epsilon (FieldOne.f90:53)
+    c->chifac[s] = 0.5 * charge / mass * c->dt * c->dt / kEPS0;
+    HIPCHK(hipMemcpy(c->d_chifac, c->chifac, sizeof(double) * IPM_NS_MAX, hipMemcpyHostToDevice));
+    c->species_set[s] = true;
+    DevCounters ctr;
+    HIPCHK(hipMemcpy(&ctr, c->d_ctr, sizeof ctr, hipMemcpyDeviceToHost));
+    ctr.n[s] = 0; ctr.n_new[s] = 0; ctr.n_dead[s] = 0;
+    HIPCHK(hipMemcpy(c->d_ctr, &ctr, sizeof ctr, hipMemcpyHostToDevice));
+    int rc = ensure_queue(c);
+    if (rc) return rc;
+    return sync_state(c);
+}
+
+int ipm_set_geometry(ipm_ctx *c, const double *inv_volume, const int *cell_material, const int *hedge_type,
+                     const int *vedge_type, int ndiel, int nmetal, const int *material_type)
+{
+    if (!c || !inv_volume || !cell_material || !hedge_type || !vedge_type || !material_type || ndiel < 0 ||
+        nmetal < 0)
+        return fail(IPM_EINVAL, "invalid geometry arguments");
+    const int nz = c->nz, nr = c->nr;
+    const size_t ncell = (size_t)(nz - 1) * (nr - 1);
+    for (size_t k = 0; k < ncell; ++k)
+        if (cell_material[k] < -ndiel || cell_material[k] > nmetal)
+            return fail(IPM_EINVAL, "cell_material[%zu]=%d outside [-%d, %d]", k, cell_material[k], ndiel, nmetal);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    void *old[] = {c->inv_vol, c->cell_material, c->hedge, c->vedge, c->mat_type, c->metal_charge};
+    for (void *p : old)
+        if (p) HIPCHK(hipFree(p));
+    const size_t nn = (size_t)nz * nr;
+    HIPCHK(hipMalloc(&c->inv_vol, sizeof(double) * nn));
+    HIPCHK(hipMemcpy(c->inv_vol, inv_volume, sizeof(double) * nn, hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&c->cell_material, sizeof(int) * ncell));
+    HIPCHK(hipMemcpy(c->cell_material, cell_material, sizeof(int) * ncell, hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&c->hedge, sizeof(int) * nz * (nr - 1)));
+    HIPCHK(hipMemcpy(c->hedge, hedge_type, sizeof(int) * nz * (nr - 1), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&c->vedge, sizeof(int) * (nz - 1) * nr));
+    HIPCHK(hipMemcpy(c->vedge, vedge_type, sizeof(int) * (nz - 1) * nr, hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&c->mat_type, sizeof(int) * (ndiel + nmetal + 1)));
+    HIPCHK(hipMemcpy(c->mat_type, material_type, sizeof(int) * (ndiel + nmetal + 1), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&c->metal_charge, sizeof(double) * (nmetal + 1)));
+    HIPCHK(hipMemset(c->metal_charge, 0, sizeof(double) * (nmetal + 1)));
+    c->h.cell_material = c->cell_material;
+    c->h.hedge = c->hedge;
+    c->h.vedge = c->vedge;
+    c->h.mat_type = c->mat_type;
+    c->h.ndiel = ndiel;
+    c->h.nmetal = nmetal;
+    c->h.metal_charge = c->metal_charge;
+    c->geometry_set = true;
+    return sync_state(c);
+}
+
+int ipm_upload_species(ipm_ctx *c, int s, int64_t n, const double *po, int layout)
+{
+    if (!c || s < 0 || s >= c->ns || !c->species_set[s]) return fail(IPM_EINVAL, "invalid species %d", s);
+    if (n < 0 || n > c->h.cap[s]) return fail(IPM_ECAPACITY, "upload of %lld particles exceeds capacity %lld",
+                                               (long long)n, (long long)c->h.cap[s]);
+    if (n > 0 && !po) return fail(IPM_EINVAL, "null particle array");
+    if (layout != IPM_LAYOUT_AOS && layout != IPM_LAYOUT_SOA) return fail(IPM_EINVAL, "invalid layout");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (n > 0) {
+        if (layout == IPM_LAYOUT_SOA) {
+            for (int f = 0; f < 8; ++f)
+                HIPCHK(hipMemcpy(c->h.f[s][f], po + (size_t)f * n, sizeof(double) * n, hipMemcpyHostToDevice));
+        } else {
+            std::vector<double> tmp((size_t)n);
+            for (int f = 0; f < 8; ++f) {
+                for (int64_t i = 0; i < n; ++i) tmp[i] = po[(size_t)i * 8 + f];
+                HIPCHK(hipMemcpy(c->h.f[s][f], tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+            }
+        }
+    }
+    DevCounters ctr;
+    HIPCHK(hipMemcpy(&ctr, c->d_ctr, sizeof ctr, hipMemcpyDeviceToHost));
+    ctr.n[s] = (unsigned long long)n; ctr.n_new[s] = 0; ctr.n_dead[s] = 0;
+    HIPCHK(hipMemcpy(c->d_ctr, &ctr, sizeof ctr, hipMemcpyHostToDevice));
+    return IPM_OK;
+}
+
+int ipm_download_species(ipm_ctx *c, int s, int64_t *n_out, double *po, int64_t cap, int layout)
+{
+    if (!c || s < 0 || s >= c->ns || !c->species_set[s] || !n_out) return fail(IPM_EINVAL, "invalid species %d", s);
+    if (layout != IPM_LAYOUT_AOS && layout != IPM_LAYOUT_SOA) return fail(IPM_EINVAL, "invalid layout");
+    DevCounters ctr;
+    int rc = read_counters(c, &ctr);
+    if (rc) return rc;
+    const size_t n = (size_t)ctr.n[s];
+    const size_t live = n - (size_t)ctr.n_dead[s];
+    *n_out = (int64_t)live;
+    if ((int64_t)live > cap) return fail(IPM_ECAPACITY, "download buffer holds %lld < %zu particles", (long long)cap, live);
+    if (live == 0) return IPM_OK;
+    if (!po) return fail(IPM_EINVAL, "null particle array");
+    std::vector<double> tmp(n * 8);
+    for (int f = 0; f < 8; ++f)
+        HIPCHK(hipMemcpy(tmp.data() + f * n, c->h.f[s][f], sizeof(double) * n, hipMemcpyDeviceToHost));
+    size_t j = 0;
+    for (size_t i = 0; i < n; ++i) {
+        if (tmp[i] < -1.0e299) continue;
+        if (j >= live) return fail(IPM_EINVAL, "live count mismatch on species %d", s);
+        for (int f = 0; f < 8; ++f) {
+            double v = tmp[f * n + i];
+            if (layout == IPM_LAYOUT_SOA) po[(size_t)f * live + j] = v;
+            else po[j * 8 + f] = v;
+        }
+        ++j;
+    }
+    if (j != live) return fail(IPM_EINVAL, "live count mismatch on species %d (%zu vs %zu)", s, j, live);
+    return IPM_OK;
+}
+
+int ipm_count(ipm_ctx *c, int s, int64_t *n)
+{
+    if (!c || s < 0 || s >= c->ns || !n) return fail(IPM_EINVAL, "invalid species %d", s);
+    DevCounters ctr;
+    int rc = read_counters(c, &ctr);
+    if (rc) return rc;
+    *n = (int64_t)(ctr.n[s] - ctr.n_dead[s]);
+    return IPM_OK;
+}
+
+int ipm_set_fields(ipm_ctx *c, const double *Ez, const double *Er)
+{
+    if (!c || !Ez || !Er) return fail(IPM_EINVAL, "null field array");
+    HIPCHK(hipMemcpyAsync(c->Ez, Ez, sizeof(double) * (c->nz + 1) * c->nr, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->Er, Er, sizeof(double) * c->nz * (c->nr + 1), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->fields_set = true;
+    return IPM_OK;
+}
+
+static int check_sel(ipm_ctx *c, int s)
+{
+    if (s < -1 || s >= c->ns) return fail(IPM_EINVAL, "invalid species index %d", s);
+    return IPM_OK;
+}
+
+int ipm_move_es(ipm_ctx *c, int s)
+{
+    int rc = check_ready(c, true, false);
+    if (rc || (rc = check_sel(c, s))) return rc;
+    if ((rc = launch_particles<M_PUSH>(c, s))) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return IPM_OK;
+}
+
+int ipm_absorb(ipm_ctx *c, int s)
+{
+    int rc = check_ready(c, false, true);
+    if (rc || (rc = check_sel(c, s))) return rc;
+    if ((rc = launch_particles<M_ABSORB>(c, s))) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return IPM_OK;
+}
+
+int ipm_move_absorb(ipm_ctx *c, int s)
+{
+    int rc = check_ready(c, true, true);
+    if (rc || (rc = check_sel(c, s))) return rc;
+    if ((rc = launch_particles<M_PUSH | M_ABSORB>(c, s))) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return IPM_OK;
+}
+
+int ipm_take_tallies(ipm_ctx *c, double *metal_charge, double *sigma)
+{
+    if (!c || !c->geometry_set) return fail(IPM_EINVAL, "geometry has not been set");
+    const size_t nn = (size_t)c->nz * c->nr;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (metal_charge && c->h.nmetal > 0) {
+        std::vector<double> t(c->h.nmetal);
+        HIPCHK(hipMemcpy(t.data(), c->metal_charge, sizeof(double) * c->h.nmetal, hipMemcpyDeviceToHost));
+        for (int m = 0; m < c->h.nmetal; ++m) metal_charge[m] += t[m];
+    }
+    if (sigma) {
+        std::vector<double> t(nn);
+        HIPCHK(hipMemcpy(t.data(), c->sigma, sizeof(double) * nn, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < nn; ++k) sigma[k] += t[k];
+    }
+    hipLaunchKernelGGL(clear_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream, c->sigma, nn);
+    hipLaunchKernelGGL(clear_kernel, dim3((unsigned)((c->h.nmetal + 256) / 256)), dim3(256), 0, c->stream,
+                       c->metal_charge, (size_t)(c->h.nmetal + 1));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return IPM_OK;
+}
+
+int ipm_p2c(ipm_ctx *c, int s, double *rho_one, double *chi_one)
+{
+    int rc = check_ready(c, false, true);
+    if (rc || (rc = check_sel(c, s))) return rc;
+    if ((rc = launch_particles<M_DEPOSIT>(c, s))) return rc;
+    const int s0 = s < 0 ? 0 : s, s1 = s < 0 ? c->ns : s + 1;
+    if ((rc = launch_grid(c, s0, s1, 0))) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const size_t nn = (size_t)c->nz * c->nr;
+    if (rho_one) HIPCHK(hipMemcpy(rho_one, c->rho_one + (size_t)s0 * nn, sizeof(double) * nn * (s1 - s0), hipMemcpyDeviceToHost));
+    if (chi_one) HIPCHK(hipMemcpy(chi_one, c->chi_one + (size_t)s0 * nn, sizeof(double) * nn * (s1 - s0), hipMemcpyDeviceToHost));
+    return IPM_OK;
+}
+
+static int field_sum_device(ipm_ctx *c)
+{
+    return launch_grid(c, 0, 0, 1);
+}
+
+int ipm_field_sum(ipm_ctx *c, double *rho, double *chi)
+{
+    int rc = check_ready(c, false, false);
+    if (rc) return rc;
+    if ((rc = field_sum_device(c))) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (rho) HIPCHK(hipMemcpy(rho, c->rho_tot, sizeof(double) * c->nz * c->nr, hipMemcpyDeviceToHost));
+    if (chi) HIPCHK(hipMemcpy(chi, c->chi_tot, sizeof(double) * (c->nz + 2) * (c->nr + 2), hipMemcpyDeviceToHost));
+    return IPM_OK;
+}
+
+int ipm_set_mcc(ipm_ctx *c, int s, int g, int model, int nreaction, int nsigma, double emin, double einterval,
+                double emax, double collision_ratio, const int *rtype, const int *resultant, const double *threshold,
+                const double *prob, double gas_mass, double gas_temperature, double beta_max)
+{
+    if (!c || s < 0 || s >= c->ns || g < 0 || g >= c->ng) return fail(IPM_EINVAL, "invalid (species, gas) = (%d, %d)", s, g);
+    if (model < 0 || model > 3) return fail(IPM_EINVAL, "invalid MCC model %d", model);
+    DevTable &t = c->h.tab[s * c->ng + g];
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (void *p : c->tab_mem[s * c->ng + g]) HIPCHK(hipFree(p));
+    c->tab_mem[s * c->ng + g].clear();
+    memset(&t, 0, sizeof t);
+    t.model = model;
+    if (model != 0) {
+        if (nreaction < 1 || nreaction > IPM_NR_MAX || nsigma < 1 || !rtype || !resultant || !threshold || !prob ||
+            !(gas_mass > 0) || !(gas_temperature > 0) || !(einterval > 0))
+            return fail(IPM_EINVAL, "invalid MCC table for (species, gas) = (%d, %d)", s, g);
+        if (model == 3 && nreaction < 2) return fail(IPM_EINVAL, "reactive table needs >= 2 reactions");
+        for (int j = 0; j < nreaction; ++j)
+            if (resultant[j] < -1 || resultant[j] >= 100)
+                return fail(IPM_EINVAL, "invalid resultant %d", resultant[j]);
+        int *d_rt, *d_res;
+        double *d_th, *d_p;
+        HIPCHK(hipMalloc(&d_rt, sizeof(int) * nreaction));
+        HIPCHK(hipMalloc(&d_res, sizeof(int) * nreaction));
+        HIPCHK(hipMalloc(&d_th, sizeof(double) * nreaction));
+        HIPCHK(hipMalloc(&d_p, sizeof(double) * nreaction * nsigma));
+        c->tab_mem[s * c->ng + g] = {d_rt, d_res, d_th, d_p};
+        HIPCHK(hipMemcpy(d_rt, rtype, sizeof(int) * nreaction, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_res, resultant, sizeof(int) * nreaction, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_th, threshold, sizeof(double) * nreaction, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_p, prob, sizeof(double) * nreaction * nsigma, hipMemcpyHostToDevice));
+        t.nreaction = nreaction; t.nsigma = nsigma;
+        t.emin = emin; t.einterval = einterval; t.emax = emax; t.ratio = collision_ratio;
+        t.gas_mass = gas_mass; t.gas_temperature = gas_temperature; t.beta_max = beta_max;
+        t.rtype = d_rt; t.resultant = d_res; t.threshold = d_th; t.prob = d_p;
+    }
+    int rc = ensure_queue(c);
+    if (rc) return rc;
+    return sync_state(c);
+}
+
+int ipm_set_rng(ipm_ctx *c, uint64_t seed, uint32_t step)
+{
+    if (!c) return fail(IPM_EINVAL, "null context");
+    DevCounters ctr;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(&ctr, c->d_ctr, sizeof ctr, hipMemcpyDeviceToHost));
+    ctr.seed = seed;
+    ctr.step = step;
+    HIPCHK(hipMemcpy(c->d_ctr, &ctr, sizeof ctr, hipMemcpyHostToDevice));
+    return IPM_OK;
+}
+
+int ipm_mcc(ipm_ctx *c)
+{
+    int rc = check_ready(c, false, false);
+    if (rc) return rc;
+    if ((rc = launch_particles<M_MCC>(c, -1))) return rc;
+    if ((rc = launch_collide(c))) return rc;
+    DevCounters ctr;
+    return read_counters(c, &ctr);
+}
+
+int ipm_set_event_counting(ipm_ctx *c, int on)
+{
+    if (!c) return fail(IPM_EINVAL, "null context");
+    c->h.count_events = on ? 1 : 0;
+    return sync_state(c);
+}
+
+int ipm_take_mcc_counts(ipm_ctx *c, int64_t *counts)
+{
+    if (!c || !counts) return fail(IPM_EINVAL, "null argument");
+    const size_t n = (size_t)c->ns * c->ng * 128;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(counts, c->mcc_counts, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemset(c->mcc_counts, 0, sizeof(unsigned long long) * IPM_NS_MAX * IPM_NG_MAX * 128));
+    return IPM_OK;
+}
+
+static int timed_event(ipm_ctx *c, int *idx)
+{
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        c->ev_pool.push_back(e);
+    }
+    *idx = (int)c->ev_used++;
+    HIPCHK(hipEventRecord(c->ev_pool[*idx], c->stream));
+    return IPM_OK;
+}
+
+// Enqueues one step.  With timing on (and not capturing), each of the three
+// kernels is bracketed by HIP events on the library's stream.
+static int step_enqueue(ipm_ctx *c, bool capturing)
+{
+    const bool t = c->timing && !capturing;
+    int a = 0, b = 0, rc;
+    if (t && (rc = timed_event(c, &a))) return rc;
+    if ((rc = launch_particles<M_PUSH | M_ABSORB | M_DEPOSIT | M_MCC>(c, -1))) return rc;
+    if (t) {
+        if ((rc = timed_event(c, &b))) return rc;
+        c->ev_pairs[0].push_back({a, b});
+        a = b;
+    }
+    if ((rc = launch_collide(c))) return rc;
+    if (t) {
+        if ((rc = timed_event(c, &b))) return rc;
+        c->ev_pairs[1].push_back({a, b});
+        a = b;
+    }
+    if ((rc = launch_grid(c, 0, c->ns, 1))) return rc;
+    if (t) {
+        if ((rc = timed_event(c, &b))) return rc;
+        c->ev_pairs[2].push_back({a, b});
+    }
+    return IPM_OK;
+}
+
+int ipm_step(ipm_ctx *c)
+{
+    int rc = check_ready(c, true, true);
+    if (rc || (rc = step_enqueue(c, false))) return rc;
+    return IPM_OK;
+}
+
+int ipm_set_timing(ipm_ctx *c, int on)
+{
+    if (!c) return fail(IPM_EINVAL, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->timing = on != 0;
+    for (auto &v : c->ev_pairs) v.clear();
+    c->ev_used = 0;
+    return IPM_OK;
+}
+
+int ipm_take_timing(ipm_ctx *c, double *ms, int64_t *launches)
+{
+    if (!c || !ms || !launches) return fail(IPM_EINVAL, "null argument");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 3; ++k) {
+        double tot = 0.0;
+        for (auto &pr : c->ev_pairs[k]) {
+            float e = 0.0f;
+            HIPCHK(hipEventElapsedTime(&e, c->ev_pool[pr.first], c->ev_pool[pr.second]));
+            tot += e;
+        }
+        ms[k] = tot;
+        launches[k] = (int64_t)c->ev_pairs[k].size();
+        c->ev_pairs[k].clear();
+    }
+    c->ev_used = 0;
+    return IPM_OK;
+}
+
+int ipm_step_graph(ipm_ctx *c)
+{
+    int rc = check_ready(c, true, true);
+    if (rc) return rc;
+    if (!c->graph) {
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        rc = step_enqueue(c, true);
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc) return rc;
+        if (e != hipSuccess) return fail(IPM_EHIP, "graph capture failed: %s", hipGetErrorString(e));
+        HIPCHK(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
+        HIPCHK(hipGraphDestroy(g));
+    }
+    HIPCHK(hipGraphLaunch(c->graph, c->stream));
+    return IPM_OK;
+}
+
+int ipm_synchronize(ipm_ctx *c)
+{
+    if (!c) return fail(IPM_EINVAL, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    DevCounters ctr;
+    return read_counters(c, &ctr);
+}
+
+int ipm_sort(ipm_ctx *c, int s)
+{
+    int rc = check_ready(c, false, false);
+    if (rc || (rc = check_sel(c, s))) return rc;
+    const int s0 = s < 0 ? 0 : s, s1 = s < 0 ? c->ns : s + 1;
+    const size_t ncell = (size_t)(c->nz - 1) * (c->nr - 1);
+    const unsigned nblk = (unsigned)((ncell + kScanBlock - 1) / kScanBlock);
+    if (nblk > kScanBlock) return fail(IPM_EINVAL, "grid too large for the two-level scan");
+    DevCounters ctr;
+    if ((rc = read_counters(c, &ctr))) return rc;
+    for (int sp = s0; sp < s1; ++sp) {
+        const size_t n = (size_t)ctr.n[sp];
+        HIPCHK(hipMemsetAsync(c->hist, 0, sizeof(unsigned long long) * ncell, c->stream));
+        HIPCHK(hipMemsetAsync(c->cursor, 0, sizeof(unsigned long long) * ncell, c->stream));
+        const unsigned pblk = (unsigned)((n + kBlock - 1) / kBlock);
+        if (pblk > 0)
+            hipLaunchKernelGGL(sort_hist_kernel, dim3(pblk), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr, sp, c->hist);
+        hipLaunchKernelGGL(scan_block_kernel, dim3(nblk), dim3(kScanBlock), 0, c->stream, c->hist, c->offs, c->partial, ncell);
+        hipLaunchKernelGGL(scan_block_kernel, dim3(1), dim3(kScanBlock), 0, c->stream, c->partial, c->partial2,
+                           c->partial + nblk, (size_t)nblk);
+        hipLaunchKernelGGL(scan_add_kernel, dim3(nblk), dim3(kScanBlock), 0, c->stream, c->offs, c->partial2, ncell);
+        HIPCHK(hipGetLastError());
+        const int other = 1 - c->cur[sp];
+        const size_t cap = ((size_t)c->h.cap[sp] + 31) & ~(size_t)31;
+        double *dst[8];
+        for (int f = 0; f < 8; ++f) dst[f] = c->buf[other][sp] + (size_t)f * cap;
+        HIPCHK(hipMemcpyAsync(c->d_dst, dst, sizeof dst, hipMemcpyHostToDevice, c->stream));
+        if (pblk > 0)
+            hipLaunchKernelGGL(sort_scatter_kernel, dim3(pblk), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr, sp,
+                               c->offs, c->cursor, c->d_dst);
+        hipLaunchKernelGGL(set_count_kernel, dim3(1), dim3(64), 0, c->stream, c->d_ctr, sp, c->offs, c->hist, ncell);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->cur[sp] = other;
+        for (int f = 0; f < 8; ++f) c->h.f[sp][f] = dst[f];
+        if ((rc = sync_state(c))) return rc;
+    }
+    return IPM_OK;
+}
+
+int ipm_set_launch(ipm_ctx *c, int block, int blocks_per_cu)
+{
+    if (!c) return fail(IPM_EINVAL, "null context");
+    if (block != 0 && block != kBlock) return fail(IPM_EINVAL, "block size is fixed at %d", kBlock);
+    if (blocks_per_cu < 0 || blocks_per_cu > 64) return fail(IPM_EINVAL, "invalid blocks_per_cu %d", blocks_per_cu);
+    c->blocks_per_cu = blocks_per_cu == 0 ? 4 : blocks_per_cu;
+    if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    return IPM_OK;
+}
+
+int ipm_species_device_ptrs(ipm_ctx *c, int s, double **fields8)
+{
+    if (!c || s < 0 || s >= c->ns || !fields8 || !c->species_set[s]) return fail(IPM_EINVAL, "invalid species %d", s);
+    for (int f = 0; f < 8; ++f) fields8[f] = c->h.f[s][f];
+    return IPM_OK;
+}
+
+}  // extern "C"

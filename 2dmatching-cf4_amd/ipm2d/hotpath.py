@@ -1,0 +1,243 @@
+"""Python host mirror of the reference's per-step particle calls.
+
+``HotPath`` owns one device context (include/ipm_hip.h).  Its methods carry
+the reference's names and argument meaning:
+
+    move_es(s)                     PB(s)%MoveES(FG)            ParticleBundle.f90:649
+    material_absorption(s)         MaterialAborptionParticle   ParticleMaterial.F90:10
+    p2ces(s) -> RhoOne, ChiOne     FO(s)%P2CES(PB(s), Geom)    FieldOne.f90:132
+    field_source_sum() -> Rho, Chi FT%Sum(FO)                  FieldSource.f90:72
+    mcc()                          MCC(Ns, Ng, PB, ...)        MCCPublic.F90:38
+    one_step()                     OneStep's particle calls    onestep.F90:141-177
+
+Errors raise ``IPMError`` (``CapacityError`` where the reference would
+``stop`` on a full particle bundle).  Grids are returned as numpy arrays
+shaped (nz, nr) (Rho(iz, ir) at [iz-1, ir-1]); Chi keeps its ghost layers,
+(nz+2, nr+2).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from .geometry import fortran
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+class HotPath:
+    def __init__(self, nz, nr, dx, dt, ns, ng=1):
+        N.load()
+        self.nz, self.nr, self.dx, self.dt, self.ns, self.ng = nz, nr, dx, dt, ns, ng
+        h = C.c_void_p()
+        N.call("ipm_create", C.byref(h), nz, nr, float(dx), float(dt), ns, ng)
+        self._h = h
+        self.nmetal = 0
+        self.capacity = [0] * ns
+
+    # ------------------------------------------------------------ lifecycle
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            N.load().ipm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # --------------------------------------------------------------- set-up
+    def set_species(self, s, charge, mass, capacity):
+        N.call("ipm_set_species", self._h, s, float(charge), float(mass), int(capacity))
+        self.capacity[s] = int(capacity)
+
+    def set_geometry(self, geom):
+        inv = fortran(geom.inv_volume).astype(np.float64)
+        cm = fortran(geom.cell_material).astype(np.int32)
+        he = fortran(geom.hedge_type).astype(np.int32)
+        ve = fortran(geom.vedge_type).astype(np.int32)
+        mt = np.ascontiguousarray(geom.material_type, dtype=np.int32)
+        N.call("ipm_set_geometry", self._h, _d(inv), _i(cm), _i(he), _i(ve), geom.ndiel, geom.nmetal, _i(mt))
+        self.nmetal = geom.nmetal
+
+    def set_mcc(self, s, g, b):
+        """Upload one MCCBundle (ipm2d.xsec.MCCBundle or model 0 = none)."""
+        if b is None or b.model == 0:
+            z = np.zeros(1, dtype=np.int32)
+            zd = np.zeros(1)
+            N.call("ipm_set_mcc", self._h, s, g, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, _i(z), _i(z), _d(zd), _d(zd),
+                   0.0, 0.0, 0.0)
+            return
+        rt = np.ascontiguousarray(b.rtype, dtype=np.int32)
+        rs = np.ascontiguousarray(b.resultant, dtype=np.int32)
+        th = np.ascontiguousarray(b.threshold, dtype=np.float64)
+        pr = np.ascontiguousarray(b.prob, dtype=np.float64)        # (nsigma, nreaction) == Probility(j,i)
+        assert pr.shape == (b.nsigma, b.nreaction)
+        N.call("ipm_set_mcc", self._h, s, g, int(b.model), int(b.nreaction), int(b.nsigma), float(b.emin),
+               float(b.einterval), float(b.emax), float(b.collision_ratio), _i(rt), _i(rs), _d(th), _d(pr),
+               float(b.gas_mass), float(b.gas_temperature), float(b.beta_max))
+
+    def set_rng(self, seed, step=0):
+        N.call("ipm_set_rng", self._h, C.c_uint64(int(seed)), C.c_uint32(int(step)))
+
+    def set_event_counting(self, on=True):
+        N.call("ipm_set_event_counting", self._h, 1 if on else 0)
+
+    def set_launch(self, blocks_per_cu=0):
+        N.call("ipm_set_launch", self._h, 0, int(blocks_per_cu))
+
+    # ------------------------------------------------------------ particles
+    def upload(self, s, soa):
+        soa = np.ascontiguousarray(soa, dtype=np.float64)
+        assert soa.ndim == 2 and soa.shape[0] == 8
+        n = soa.shape[1]
+        N.call("ipm_upload_species", self._h, s, n, _d(soa), N.LAYOUT_SOA)
+
+    def upload_aos(self, s, aos):
+        aos = np.ascontiguousarray(aos, dtype=np.float64)
+        assert aos.ndim == 2 and aos.shape[1] == 8
+        N.call("ipm_upload_species", self._h, s, aos.shape[0], _d(aos), N.LAYOUT_AOS)
+
+    def count(self, s):
+        n = C.c_int64()
+        N.call("ipm_count", self._h, s, C.byref(n))
+        return n.value
+
+    def download(self, s):
+        n = self.count(s)
+        out = np.empty((8, max(n, 1)))
+        got = C.c_int64()
+        N.call("ipm_download_species", self._h, s, C.byref(got), _d(out), max(n, 1), N.LAYOUT_SOA)
+        assert got.value == n
+        return np.ascontiguousarray(out.reshape(-1)[:8 * n].reshape(8, n))
+
+    def download_aos(self, s):
+        n = self.count(s)
+        out = np.empty((max(n, 1), 8))
+        got = C.c_int64()
+        N.call("ipm_download_species", self._h, s, C.byref(got), _d(out), max(n, 1), N.LAYOUT_AOS)
+        return out[:got.value].copy()
+
+    # --------------------------------------------------------------- fields
+    def set_fields(self, ez, er):
+        ez = np.ascontiguousarray(ez, dtype=np.float64).ravel()
+        er = np.ascontiguousarray(er, dtype=np.float64).ravel()
+        assert ez.size == (self.nz + 1) * self.nr and er.size == self.nz * (self.nr + 1)
+        N.call("ipm_set_fields", self._h, _d(ez), _d(er))
+
+    # ------------------------------------------------- reference-named calls
+    def move_es(self, s=-1):
+        N.call("ipm_move_es", self._h, s)
+
+    def material_absorption(self, s=-1):
+        N.call("ipm_absorb", self._h, s)
+
+    def move_absorb(self, s=-1):
+        N.call("ipm_move_absorb", self._h, s)
+
+    def take_tallies(self):
+        mc = np.zeros(max(self.nmetal, 1))
+        sg = np.zeros(self.nz * self.nr)
+        N.call("ipm_take_tallies", self._h, _d(mc), _d(sg))
+        return mc[:self.nmetal], sg.reshape(self.nr, self.nz).T
+
+    def p2ces(self, s, copy=True):
+        if not copy:
+            N.call("ipm_p2c", self._h, s, None, None)
+            return None
+        rho = np.zeros(self.nz * self.nr)
+        chi = np.zeros(self.nz * self.nr)
+        N.call("ipm_p2c", self._h, s, _d(rho), _d(chi))
+        return rho.reshape(self.nr, self.nz).T, chi.reshape(self.nr, self.nz).T
+
+    def p2ces_all(self):
+        rho = np.zeros(self.ns * self.nz * self.nr)
+        chi = np.zeros(self.ns * self.nz * self.nr)
+        N.call("ipm_p2c", self._h, -1, _d(rho), _d(chi))
+        sh = (self.ns, self.nr, self.nz)
+        return rho.reshape(sh).transpose(0, 2, 1), chi.reshape(sh).transpose(0, 2, 1)
+
+    def field_source_sum(self, copy=True):
+        if not copy:
+            N.call("ipm_field_sum", self._h, None, None)
+            return None
+        rho = np.zeros(self.nz * self.nr)
+        chi = np.zeros((self.nz + 2) * (self.nr + 2))
+        N.call("ipm_field_sum", self._h, _d(rho), _d(chi))
+        return rho.reshape(self.nr, self.nz).T, chi.reshape(self.nr + 2, self.nz + 2).T
+
+    def mcc(self):
+        N.call("ipm_mcc", self._h)
+
+    def take_mcc_counts(self):
+        c = np.zeros(self.ns * self.ng * 128, dtype=np.int64)
+        N.call("ipm_take_mcc_counts", self._h, c.ctypes.data_as(C.POINTER(C.c_int64)))
+        return c.reshape(self.ns, self.ng, 128)
+
+    def one_step(self):
+        N.call("ipm_step", self._h)
+
+    def one_step_graph(self):
+        N.call("ipm_step_graph", self._h)
+
+    def set_timing(self, on=True):
+        N.call("ipm_set_timing", self._h, 1 if on else 0)
+
+    def take_timing(self):
+        """{'particle'|'collide'|'grid': (total_ms, launches)} since the last call."""
+        ms = np.zeros(3)
+        n = np.zeros(3, dtype=np.int64)
+        N.call("ipm_take_timing", self._h, _d(ms), n.ctypes.data_as(C.POINTER(C.c_int64)))
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(("particle", "collide", "grid"))}
+
+    def sort(self, s=-1):
+        N.call("ipm_sort", self._h, s)
+
+    def synchronize(self):
+        N.call("ipm_synchronize", self._h)
+
+    def device_ptrs(self, s):
+        arr = (_dp * 8)()
+        N.call("ipm_species_device_ptrs", self._h, s, arr)
+        return [C.cast(arr[f], C.c_void_p).value for f in range(8)]
+
+
+def from_config(cfg, geom=None, tables=None, capacity_factor=1.5, rng=None, particles=None, dielectric_ring=0):
+    """Build a HotPath for a workload config: species, geometry, tables,
+    particles (loaded with numpy's generator seeded by cfg.seed) and fields."""
+    from . import workload as W
+    geom = geom if geom is not None else W.make_geometry(cfg, dielectric_ring=dielectric_ring)
+    tables = tables if tables is not None else W.build_tables(cfg)
+    rng = rng if rng is not None else np.random.default_rng(cfg.seed)
+    hp = HotPath(cfg.nz, cfg.nr, cfg.dx, cfg.dt, cfg.ns, 1)
+    if particles is None:
+        particles = [W.load_species(cfg, geom, s, rng) for s in range(cfg.ns)]
+    for s in range(cfg.ns):
+        q, m = W.species_params(cfg, s)
+        cap = int(particles[s].shape[1] * capacity_factor) + 4096
+        hp.set_species(s, q, m, cap)
+    hp.set_geometry(geom)
+    for s in range(cfg.ns):
+        hp.set_mcc(s, 0, tables.get((s, 0)))
+    for s in range(cfg.ns):
+        hp.upload(s, particles[s])
+    ez, er = W.fields(cfg, 0.0)
+    hp.set_fields(ez, er)
+    hp.set_rng(cfg.seed, 0)
+    return hp, geom, tables, particles

@@ -1,0 +1,149 @@
+/*
+ * ipm_hip.h -- C ABI of the MI355X (gfx950) particle hot path for iPM2D.
+ *
+ * Every entry point returns 0 on success or a negative IPM_E* code; the text
+ * of the last error is available from ipm_last_error().  Plain pointers and
+ * sizes only, so Fortran (iso_c_binding), C, C++ and Python (ctypes) bind it
+ * directly.  Grid arrays use the reference's Fortran column-major layouts
+ * (z fastest) including the ghost layers the reference allocates:
+ *   Ez(0:Nz, 1:Nr)   (Nz+1)*Nr doubles     R/src/pic/field/FieldEM.f90:72
+ *   Er(1:Nz, 0:Nr)   Nz*(Nr+1) doubles     R/src/pic/field/FieldEM.f90:73
+ *   Rho/RhoOne/ChiOne/sigma(1:Nz, 1:Nr)    Nz*Nr doubles
+ *   Chi(0:Nz+1, 0:Nr+1)                    (Nz+2)*(Nr+2) doubles
+ *                                          R/src/pic/field/FieldSource.f90:50-53
+ * Particles are the 8 ParticleOne fields Z R Vz Vr Vt Az Ar WQ
+ * (R/src/pic/particle/ParticleOne.f90:8) in grid units, resident on the GPU
+ * as structure-of-arrays FP64.  Species are indexed 0..ns-1 as in the
+ * reference (0 = electrons); gases 0..ng-1 (reference gas i+1).
+ */
+#ifndef IPM_HIP_H
+#define IPM_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IPM_OK 0
+#define IPM_EINVAL (-1)    /* bad argument / index / shape */
+#define IPM_EHIP (-2)      /* HIP runtime error */
+#define IPM_ECAPACITY (-3) /* particle capacity exceeded (the reference stops:
+                              R/src/pic/particle/ParticleBundle.f90:587-590) */
+#define IPM_ENODEV (-4)    /* no usable GPU */
+
+#define IPM_LAYOUT_AOS 0   /* n records of 8 doubles (the reference's PO(:) array) */
+#define IPM_LAYOUT_SOA 1   /* 8 consecutive arrays of n doubles */
+
+#define IPM_NS_MAX 8
+#define IPM_NG_MAX 3
+#define IPM_NR_MAX 128     /* reactions per (species, gas) table */
+
+typedef struct ipm_ctx ipm_ctx;
+
+/* Context: one simulation on the current HIP device.  nz, nr = node counts
+ * (Domain%GlobalShape), dx = dz = dr (R/src/base/init.F90:131-138), dt. */
+int ipm_create(ipm_ctx **ctx, int nz, int nr, double dx, double dt, int ns, int ng);
+int ipm_destroy(ipm_ctx *ctx);
+const char *ipm_last_error(void);
+
+/* Species parameters: ParticleBundle%Charge/%Mass (ParticleBundle.f90:350-351)
+ * and the device capacity (npar_max, ParticleBundle.f90:354-359). */
+int ipm_set_species(ipm_ctx *ctx, int s, double charge, double mass, int64_t capacity);
+
+/* Geometry for absorption and deposit (R/src/base/Geometry.F90:21-45,
+ * R/src/base/init.F90:402-424): inv_volume = point_inv_volume_weighting
+ * (Nz*Nr); cell_material (Nz-1)*(Nr-1); hedge_type Nz*(Nr-1);
+ * vedge_type (Nz-1)*Nr; material_type[m + ndiel] for m in [-ndiel, nmetal]
+ * (R/src/pic/materials/Materials.F90:52-56). */
+int ipm_set_geometry(ipm_ctx *ctx, const double *inv_volume, const int *cell_material,
+                     const int *hedge_type, const int *vedge_type,
+                     int ndiel, int nmetal, const int *material_type);
+
+/* Particle transfer (restart Dump/Load, ParticleBundle.f90:117-150, and the
+ * explicit host sync the reference's other per-particle readers need). */
+int ipm_upload_species(ipm_ctx *ctx, int s, int64_t n, const double *po, int layout);
+/* Downloads the live particles (dead slots dropped) into po (capacity cap). */
+int ipm_download_species(ipm_ctx *ctx, int s, int64_t *n, double *po, int64_t cap, int layout);
+/* Live particle count (ParticleBundle%NPar). */
+int ipm_count(ipm_ctx *ctx, int s, int64_t *n);
+
+/* FieldEM%Ez/%Er after SetE (R/src/pic/field/FieldSource.f90:95-120). */
+int ipm_set_fields(ipm_ctx *ctx, const double *Ez, const double *Er);
+
+/* PB(s)%MoveES(FG) -> MoveESIm (R/src/pic/particle/ParticleBundle.f90:649-748);
+ * s = -1 runs every species. */
+int ipm_move_es(ipm_ctx *ctx, int s);
+/* MaterialAborptionParticle(CF, Geom, MT, PB(s))
+ * (R/src/pic/particleboundary/ParticleMaterial.F90:10-90); s = -1: all. */
+int ipm_absorb(ipm_ctx *ctx, int s);
+/* Both of the above in one pass (OneStep's loop body, R/src/onestep.F90:141-144). */
+int ipm_move_absorb(ipm_ctx *ctx, int s);
+/* Adds the absorption tallies accumulated since the last call to
+ * MT%metls(1:nmetal)%charge_one_step (metal_charge) and MT%sigma_one_step
+ * (sigma, Nz*Nr), then clears them on the device.  Either may be NULL. */
+int ipm_take_tallies(ipm_ctx *ctx, double *metal_charge, double *sigma);
+
+/* FO(s)%P2CES(PB(s), Geom) (R/src/pic/field/FieldOne.f90:132-180):
+ * rho_one/chi_one (Nz*Nr each) receive RhoOne/ChiOne; NULL keeps them on
+ * the device only.  s = -1 deposits every species. */
+int ipm_p2c(ipm_ctx *ctx, int s, double *rho_one, double *chi_one);
+/* FT%Sum(FO) (R/src/pic/field/FieldSource.f90:72-92): rho (Nz*Nr) and
+ * chi with ghosts ((Nz+2)*(Nr+2); ghosts set to 0 as FT%Zero does). */
+int ipm_field_sum(ipm_ctx *ctx, double *rho, double *chi);
+
+/* MCC cross-section bundle MCCBundle(s, g) (R/src/mc/TypeMCC.f90:23-33),
+ * as built by MCCBundleInit (R/src/mc/MCCInit.F90:15-309).
+ * rtype/resultant/threshold: per reaction; prob: Probility(j,i) column-major
+ * (reaction fastest), nreaction*nsigma doubles. emax = 10**EnergyMax.
+ * gas_mass [kg], gas_temperature [K], beta_max: the gas's GasOne fields. */
+int ipm_set_mcc(ipm_ctx *ctx, int s, int g, int model, int nreaction, int nsigma,
+                double emin, double einterval, double emax, double collision_ratio,
+                const int *rtype, const int *resultant, const double *threshold,
+                const double *prob, double gas_mass, double gas_temperature, double beta_max);
+/* Counter-based random stream (Philox4x32-10) replacing the reference's
+ * unseeded random_number: draws are keyed by (seed, step, species|gas<<8,
+ * slot, draw).  step advances by one after every MCC pass. */
+int ipm_set_rng(ipm_ctx *ctx, uint64_t seed, uint32_t step);
+/* MCC(Ns, Ng, PB, SO, GO, MCCB) (R/src/mc/MCCPublic.F90:38-121) including
+ * MCCAddNewParticle (209-226): products are appended to their species. */
+int ipm_mcc(ipm_ctx *ctx);
+/* Enables (on != 0) or disables the per-event counters below (off by default:
+ * counting adds one atomic per collision). */
+int ipm_set_event_counting(ipm_ctx *ctx, int on);
+/* Per-(species, gas, reaction index) event counts since the last call
+ * (counts[(s*ng + g)*128 + j], j = 0 for a null event); cleared after read. */
+int ipm_take_mcc_counts(ipm_ctx *ctx, int64_t *counts);
+
+/* The whole per-step particle path in one persistent launch over every
+ * species: move + absorb + deposit + MCC (+ grid epilogue producing Rho and
+ * Chi).  Equivalent to the reference's OneStep particle calls
+ * (R/src/onestep.F90:141-177) because MCC reads no field: the Poisson solve
+ * that the reference runs between deposit and MCC does not feed it. */
+int ipm_step(ipm_ctx *ctx);
+/* Same step replayed from a captured hipGraph (captured on first use). */
+int ipm_step_graph(ipm_ctx *ctx);
+/* ipm_step and ipm_step_graph only enqueue work: errors they raise on the
+ * device (capacity overflow) are returned by the next synchronizing call
+ * (ipm_synchronize, ipm_count, downloads, ipm_field_sum, ...). */
+
+/* HIP-event timing of the kernels ipm_step enqueues (eager mode only):
+ * ms[0..2] / launches[0..2] = summed milliseconds and launch counts of the
+ * particle pass, the collision pass and the grid epilogue since the last
+ * call; cleared after read. */
+int ipm_set_timing(ipm_ctx *ctx, int on);
+int ipm_take_timing(ipm_ctx *ctx, double *ms, int64_t *launches);
+/* Cell sort (counting sort by cell; drops dead slots). s = -1: all. */
+int ipm_sort(ipm_ctx *ctx, int s);
+
+/* Launch shape: threads per block and persistent blocks per CU (0 = default). */
+int ipm_set_launch(ipm_ctx *ctx, int block, int blocks_per_cu);
+int ipm_synchronize(ipm_ctx *ctx);
+
+/* Device addresses of species s's 8 SoA arrays (for zero-copy interop) and
+ * of the raw per-species deposit grid. */
+int ipm_species_device_ptrs(ipm_ctx *ctx, int s, double **fields8);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,704 @@
+/*
+ * CPU oracle for the iPM2D particle hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Statement-by-statement C restatement of the reference routines cited at
+ * each function.  Compiled with -ffp-contract=off so that every product and
+ * sum is rounded exactly as the Fortran source orders it.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may use this code,
+ * always as the checker; the product never links or calls it.
+ *
+ * Pinning (what checks this restatement, beyond retyping the reference):
+ *   ora_philox4x32_10     Random123 known-answer vectors
+ *   ora_push              closed forms (E=0 straight Cartesian line, uniform
+ *                         Ez quadratic z), invariants (R*Vt, Vr^2+Vt^2),
+ *                         axis branch, brute-force Cartesian re-derivation
+ *   ora_absorb            rule table on hand-built particles, exact
+ *   ora_deposit_*         charge conservation, on-node known answers,
+ *                         uniform density vs analytic node volumes
+ *   ora_node_volumes      closed-form cylindrical shell volumes
+ *   ora_mcc               selection ratio 1-exp(-nu dt), channel fractions
+ *                         n sigma v, Vahedi/isotropic cos-chi CDFs,
+ *                         Maxwellian mean energy, energy bookkeeping of
+ *                         excitation/ionization, momentum of elastic events
+ *   ora_sigma_* / ora_probability_*  closed forms on synthetic tables
+ * Parity unpinned: the reference's own random stream (Fortran
+ * random_number, unseeded) -- by construction no check can reproduce it.
+ */
+#include "ipm_oracle.h"
+#include <math.h>
+#include <string.h>
+#include <stdlib.h>
+
+/* R/src/base/Constants.F90:3-15 */
+#define C_PI 3.141592653589793238
+#define C_KB 1.3807e-23
+#define C_EXP 2.71828
+#define C_E 1.6022e-19
+#define C_EPS0 8.8542e-12
+#define C_JTOEV 1.6022e-19
+#define C_MINREAL 1.0e-15
+
+/* ---------------------------------------------------------------- RNG */
+void ora_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+    uint32_t k0 = key_in[0], k1 = key_in[1];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+double ora_uniform(uint64_t seed, uint32_t slot, uint32_t step, uint32_t draw, uint32_t tag)
+{
+    uint32_t ctr[4] = {slot, step, draw, tag};
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t o[4];
+    ora_philox4x32_10(ctr, key, o);
+    uint64_t x = ((((uint64_t)o[1]) << 32) | o[0]) >> 12;   /* 52 bits */
+    return ((double)x + 0.5) * 0x1p-52;                    /* in (0,1) */
+}
+
+typedef struct { uint64_t seed; uint32_t slot, step, tag, k; } rng_t;
+static double rnext(rng_t *r) { return ora_uniform(r->seed, r->slot, r->step, r->k++, r->tag); }
+
+/* ------------------------------------------------------- node volumes */
+void ora_node_volumes(int nz, int nr, double dx, const double *factor, double *vol, double *inv_vol)
+{
+    for (int j = 1; j <= nr; ++j)
+        for (int i = 1; i <= nz; ++i) {
+            size_t id = (size_t)(i - 1) + (size_t)(j - 1) * nz;
+            double f = factor[id], v;
+            if (j == 1)
+                v = f * 1.0 / 3.0 * C_PI * dx * dx * dx;
+            else if (j == nr)
+                v = f * 1.0 / 3.0 * ((double)(j - 2) + 2.0 * (double)(j - 1)) * C_PI * dx * dx * dx;
+            else
+                v = f * 1.0 / 3.0 * ((double)j * ((double)(j - 1) + (double)j) -
+                                     (double)(j - 2) * ((double)(j - 2) + (double)(j - 1))) * C_PI * dx * dx * dx;
+            vol[id] = v;
+            inv_vol[id] = (f > 1.0e-6) ? 1.0 / v : 0.0;
+        }
+}
+
+/* --------------------------------------------------------------- push */
+void ora_push(int nz, int nr, double dx, double dt, double charge, double mass,
+              const double *Ez, const double *Er, int64_t n,
+              double *Z, double *R, double *Vz, double *Vr, double *Vt, double *Az, double *Ar)
+{
+    (void)nr;
+    const double EFactor = charge / mass * dt / (dx / dt) * 0.5;
+    const size_t ldz = (size_t)nz + 1;   /* Ez(0:nz, 1:nr) */
+    const size_t ldr = (size_t)nz;       /* Er(1:nz, 0:nr) */
+#define EZ(i, j) Ez[(size_t)(i) + ((size_t)(j) - 1) * ldz]
+#define ER(i, j) Er[((size_t)(i) - 1) + (size_t)(j) * ldr]
+    for (int64_t p = 0; p < n; ++p) {
+        double z = Z[p], r = R[p];
+        int N1 = (int)ceil(z);
+        double Z1 = (double)N1 - z, Z2 = 1.0 - Z1;
+        int N1H = (int)(z + 0.5);
+        double Z1H = (double)N1H + 0.5 - z, Z2H = 1.0 - Z1H;
+        int N2 = (int)ceil(r);
+        double R1 = (double)N2 - r, R2 = 1.0 - R1;
+        int N2H = (int)(r + 0.5);
+        double R1H = (double)N2H + 0.5 - r, R2H = 1.0 - R1H;
+        /* R == 0 exactly gives N2 == 0 (a row outside Ez); its weight R1 is 0 */
+        int N2e = N2 < 1 ? 1 : N2;
+
+        double E1 = ER(N1, N2H) * Z1 * R1H + ER(N1 + 1, N2H) * Z2 * R1H +
+                    ER(N1, N2H + 1) * Z1 * R2H + ER(N1 + 1, N2H + 1) * Z2 * R2H;
+        double E2 = EZ(N1H, N2e) * Z1H * R1 + EZ(N1H + 1, N2e) * Z2H * R1 +
+                    EZ(N1H, N2 + 1) * Z1H * R2 + EZ(N1H + 1, N2 + 1) * Z2H * R2;
+        E1 = E1 * EFactor;
+        E2 = E2 * EFactor;
+
+        double vz = Vz[p] + E2;
+        z = z + E2;
+        double az = (Az[p] + E2) * 0.5;
+        double vr = Vr[p] + E1;
+        r = r + E1;
+        double ar = (Ar[p] + E1) * 0.5;
+
+        vz = vz + az;
+        z = z + vz;
+
+        double P_R = r;
+        double Vx = vr + ar;
+        double Vy = Vt[p];
+        double X = P_R + Vx;
+        double Y = Vy;
+        P_R = sqrt(X * X + Y * Y);
+        r = P_R;
+        double vt;
+        if (r < C_MINREAL) {
+            vr = sqrt(Vx * Vx + Vy * Vy);
+            vt = C_MINREAL;
+            ar = fabs(ar);
+        } else {
+            double SinA = Y / P_R;
+            double CosA = X / P_R;
+            vr = Vx * CosA + Vy * SinA;
+            vt = -Vx * SinA + Vy * CosA;
+            ar = ar * CosA;
+        }
+        Z[p] = z; R[p] = r; Vz[p] = vz; Vr[p] = vr; Vt[p] = vt; Az[p] = az; Ar[p] = ar;
+    }
+#undef EZ
+#undef ER
+}
+
+/* --------------------------------------------------------- absorption */
+#define MAT_METAL 1
+#define MAT_DIEL 2
+#define EDGE_V2D 1
+#define EDGE_D2V 3
+
+static void delone(double *P[8], int64_t *n, int64_t i)
+{
+    for (int f = 0; f < 8; ++f) P[f][i] = P[f][*n - 1];
+    *n -= 1;
+}
+
+void ora_absorb(const ora_geom *g, double charge, int64_t *n, double *P[8],
+                double *metal_charge, double *sigma)
+{
+    const int nz = g->nz, nr = g->nr;
+#define CM(i, j) g->cell_material[((size_t)(i) - 1) + ((size_t)(j) - 1) * (nz - 1)]
+#define HE(i, j) g->hedge_type[((size_t)(i) - 1) + ((size_t)(j) - 1) * nz]
+#define VE(i, j) g->vedge_type[((size_t)(i) - 1) + ((size_t)(j) - 1) * (nz - 1)]
+#define SG(i, j) sigma[((size_t)(i) - 1) + ((size_t)(j) - 1) * nz]
+    for (int64_t i = *n - 1; i >= 0; --i) {
+        double *Z = P[0], *R = P[1], *Vr = P[3], *Ar = P[6], *WQ = P[7];
+        if (Z[i] <= 0.0 || Z[i] >= (double)(nz - 1) || R[i] >= (double)(nr - 1)) {
+            delone(P, n, i);
+        } else if (R[i] <= 0.0) {
+            R[i] = fabs(R[i]);
+            Vr[i] = fabs(Vr[i]);
+            Ar[i] = fabs(Ar[i]);
+        } else {
+            int NzL = (int)ceil(Z[i]);
+            int NrL = (int)ceil(R[i]);
+            int m = CM(NzL, NrL);
+            int t = g->material_type[m + g->ndiel];
+            if (t == MAT_METAL) {
+                metal_charge[m - 1] = metal_charge[m - 1] + WQ[i] * charge;
+                delone(P, n, i);
+            } else if (t == MAT_DIEL) {
+                int h0 = HE(NzL, NrL), h1 = HE(NzL + 1, NrL);
+                int v0 = VE(NzL, NrL), v1 = VE(NzL, NrL + 1);
+                if (h0 == EDGE_V2D || h0 == EDGE_D2V) {
+                    double RL = (double)NrL - R[i], RU = 1.0 - RL;
+                    SG(NzL, NrL) = SG(NzL, NrL) + RL * WQ[i] * charge;
+                    SG(NzL, NrL + 1) = SG(NzL, NrL + 1) + RU * WQ[i] * charge;
+                } else if (h1 == EDGE_V2D || h1 == EDGE_D2V) {
+                    double RL = (double)NrL - R[i], RU = 1.0 - RL;
+                    SG(NzL + 1, NrL) = SG(NzL + 1, NrL) + RL * WQ[i] * charge;
+                    SG(NzL + 1, NrL + 1) = SG(NzL + 1, NrL + 1) + RU * WQ[i] * charge;
+                } else if (v0 == EDGE_V2D || v0 == EDGE_D2V) {
+                    double ZL = (double)NzL - Z[i], ZU = 1.0 - ZL;
+                    SG(NzL, NrL) = SG(NzL, NrL) + ZL * WQ[i] * charge;
+                    SG(NzL + 1, NrL) = SG(NzL + 1, NrL) + ZU * WQ[i] * charge;
+                } else if (v1 == EDGE_V2D || v1 == EDGE_D2V) {
+                    double ZL = (double)NzL - Z[i], ZU = 1.0 - ZL;
+                    SG(NzL, NrL + 1) = SG(NzL, NrL + 1) + ZL * WQ[i] * charge;
+                    SG(NzL + 1, NrL + 1) = SG(NzL + 1, NrL + 1) + ZU * WQ[i] * charge;
+                }
+                delone(P, n, i);
+            }
+        }
+    }
+#undef CM
+#undef HE
+#undef VE
+#undef SG
+}
+
+/* ------------------------------------------------------------ deposit */
+void ora_deposit_raw(int nz, int nr, int64_t n, const double *Z, const double *R,
+                     const double *WQ, double *rho_raw)
+{
+    (void)nr;
+#define RH(i, j) rho_raw[((size_t)(i) - 1) + ((size_t)(j) - 1) * nz]
+    for (int64_t p = 0; p < n; ++p) {
+        int NzL = (int)ceil(Z[p]);
+        int NzU = NzL + 1;
+        double ZL = (double)NzL - Z[p];
+        double ZU = 1.0 - ZL;
+        int NrL = (int)ceil(R[p]);
+        int NrU = NrL + 1;
+        double RL = (double)NrL - R[p];
+        double RU = 1.0 - RL;
+        /* R == 0 exactly: row NrL = 0 lies outside Rho and its weight RL is 0 */
+        if (NrL >= 1) RH(NzL, NrL) = RH(NzL, NrL) + ZL * RL * WQ[p];
+        RH(NzL, NrU) = RH(NzL, NrU) + ZL * RU * WQ[p];
+        if (NrL >= 1) RH(NzU, NrL) = RH(NzU, NrL) + ZU * RL * WQ[p];
+        RH(NzU, NrU) = RH(NzU, NrU) + ZU * RU * WQ[p];
+    }
+#undef RH
+}
+
+void ora_deposit_finish(int nz, int nr, double dt, double charge, double mass,
+                        const double *rho_raw, const double *inv_vol, double *rho, double *chi)
+{
+    const double ChiFactor = 0.5 * charge / mass * dt * dt / C_EPS0;
+    size_t nn = (size_t)nz * nr;
+    for (size_t k = 0; k < nn; ++k) {
+        double v = rho_raw[k] * inv_vol[k];
+        v = v * charge;
+        rho[k] = v;
+        chi[k] = v * ChiFactor;
+    }
+}
+
+/* ---------------------------------------------------------------- MCC */
+typedef struct { double Z, R, Vz, Vr, Vt, Az, Ar, WQ; } po_t;
+
+typedef struct {
+    double miu, mass_ratio, energy, beta;
+    double gx, gy, gz, gper, g;
+    po_t poi, pot;
+    int nnew;
+    po_t pon[2];
+    int pon_idx[2];
+} mcpo_t;
+
+/* VecRanXYZInit, ParticleOne.f90:108-123 */
+static void vec_ran_xyz(po_t *p, double V, rng_t *rg)
+{
+    double R = rnext(rg);
+    double CosTheta = 1.0 - 2.0 * R;
+    double SinTheta = sqrt(1.0 - CosTheta * CosTheta);
+    R = rnext(rg);
+    double Phi = 2.0 * C_PI * R;
+    p->Vz = V * CosTheta;
+    p->Vr = V * SinTheta * cos(Phi);
+    p->Vt = V * SinTheta * sin(Phi);
+}
+
+/* VelMaxInit, ParticleOne.f90:76-95 */
+static void vel_max_init(po_t *p, double Mass, double Temperature, rng_t *rg)
+{
+    double Beta = 1.0 / (C_KB * Temperature);
+    double FuncA = 1.0, FuncB = 0.0, R = 0.0;
+    while (FuncA > FuncB) {
+        R = rnext(rg);
+        FuncA = R * R;
+        R = rnext(rg);
+        FuncB = -C_EXP * R * log(R);
+    }
+    double V = sqrt(-3.0 * log(R) / Beta / Mass);
+    vec_ran_xyz(p, V, rg);
+}
+
+uint32_t ora_vel_maxwell(double mass, double temperature, uint64_t seed, uint32_t slot,
+                         uint32_t step, uint32_t draw, uint32_t tag,
+                         double *vz, double *vr, double *vt)
+{
+    rng_t rg = {seed, slot, step, tag, draw};
+    po_t p;
+    memset(&p, 0, sizeof p);
+    vel_max_init(&p, mass, temperature, &rg);
+    *vz = p.Vz; *vr = p.Vr; *vt = p.Vt;
+    return rg.k;
+}
+
+/* Updater, TypeMCC.f90:60-92 */
+static void mc_update(mcpo_t *m, double sm, double gm, double gT, rng_t *rg)
+{
+    m->pot = m->poi;
+    m->miu = sm * gm / (sm + gm);
+    m->mass_ratio = gm / (sm + gm);
+    vel_max_init(&m->pot, gm, gT, rg);
+    m->gx = m->poi.Vr - m->pot.Vr;
+    m->gy = m->poi.Vt - m->pot.Vt;
+    m->gz = m->poi.Vz - m->pot.Vz;
+    double Gtemp = m->gy * m->gy + m->gz * m->gz;
+    m->gper = sqrt(Gtemp);
+    double G2 = Gtemp + m->gx * m->gx;
+    m->g = sqrt(G2);
+    m->energy = 0.5 * m->miu * G2 / C_JTOEV;
+    m->pot.Ar = m->pot.Ar * sm / gm;
+    m->pot.Az = m->pot.Az * sm / gm;
+}
+
+/* VelocityUpdater, TypeMCC.f90:94-112 */
+static void mc_vel_update(mcpo_t *m, double VF)
+{
+    m->gx = VF * m->gx; m->gy = VF * m->gy; m->gz = VF * m->gz;
+    m->gper = VF * m->gper; m->g = VF * m->g;
+    m->poi.Vr = VF * m->poi.Vr; m->poi.Vt = VF * m->poi.Vt; m->poi.Vz = VF * m->poi.Vz;
+}
+
+/* MCCChi.f90:5-25 */
+static double cos_isotropic(double E, rng_t *rg) { (void)E; double R = rnext(rg); return 1.0 - 2.0 * R; }
+static double cos_vahedi(double E, rng_t *rg)
+{
+    if (E < C_MINREAL) return 1.0;
+    double R = rnext(rg);
+    return (E + 2.0 - 2.0 * pow(E + 1.0, R)) / E;
+}
+/* MCCChi.f90:49-59 */
+static double create_energy_vahedi(double E, rng_t *rg)
+{
+    double R = rnext(rg);
+    double v = 10.0 * tan(R * atan(E) / 20.0);
+    if (fabs(v) < C_MINREAL) v = C_MINREAL;
+    return v;
+}
+
+/* PostCollisionVelocity, TypeMCC.f90:114-146 */
+static void post_collision(mcpo_t *m, int vahedi, rng_t *rg)
+{
+    if (fabs(m->gper) < C_MINREAL) {
+        vec_ran_xyz(&m->poi, m->g, rg);
+    } else {
+        double CosTheta = vahedi ? cos_vahedi(m->energy, rg) : cos_isotropic(m->energy, rg);
+        double FcosTheta = 1.0 - CosTheta;
+        double SinTheta = sqrt(1.0 - CosTheta * CosTheta);
+        double R = rnext(rg);
+        double APhi = 2.0 * C_PI * R;
+        double CosPhi = cos(APhi), SinPhi = sin(APhi);
+        double gx = m->gx, gy = m->gy, gz = m->gz, g = m->g, gper = m->gper;
+        double hx = gper * CosPhi;
+        double hy = -(gx * gy * CosPhi + g * gz * SinPhi) / gper;
+        double hz = -(gx * gz * CosPhi - g * gy * SinPhi) / gper;
+        m->poi.Vr = m->poi.Vr - m->mass_ratio * (gx * FcosTheta + hx * SinTheta);
+        m->poi.Vt = m->poi.Vt - m->mass_ratio * (gy * FcosTheta + hy * SinTheta);
+        m->poi.Vz = m->poi.Vz - m->mass_ratio * (gz * FcosTheta + hz * SinTheta);
+    }
+}
+
+/* SelectProbility, TypeMCC.f90:173-216 */
+static int select_probability(const mcpo_t *m, const ora_mcc_table *t, rng_t *rg)
+{
+    double tmp[256];
+    const int Nr = t->nreaction, Ns = t->nsigma;
+    double E = m->energy;
+    double lx = (log10(E) - t->emin) / t->einterval;
+    int N = (E > 0.0) ? (int)ceil(lx) : 0;
+    if (N < 1) {
+        for (int j = 0; j < Nr; ++j) tmp[j] = t->prob[j];
+    } else if (N < Ns) {
+        double S1 = (double)N - (log10(E) - t->emin) / t->einterval;
+        double S2 = 1.0 - S1;
+        const double *a = t->prob + (size_t)(N - 1) * Nr, *b = t->prob + (size_t)N * Nr;
+        for (int j = 0; j < Nr; ++j) tmp[j] = a[j] * S1 + b[j] * S2;
+    } else {
+        const double *a = t->prob + (size_t)(Ns - 1) * Nr;
+        for (int j = 0; j < Nr; ++j) tmp[j] = a[j] * E / t->emax;
+    }
+    for (int j = 0; j < Nr; ++j)
+        if (E < t->threshold[j]) tmp[j] = 0.0;
+    double R = rnext(rg);
+    for (int j = 0; j < Nr; ++j)
+        if (R < tmp[j]) return j + 1;
+    return 0;
+}
+
+/* SelectProbilityReactive, TypeMCC.f90:218-236 */
+static int select_probability_reactive(mcpo_t *m, const ora_mcc_table *t, rng_t *rg)
+{
+    double R = rnext(rg);
+    m->beta = t->beta_max * sqrt(R);
+    if (m->beta <= 1.0) {
+        if (m->energy <= t->threshold[1]) return 1;
+        return select_probability(m, t, rg);
+    }
+    return 1;
+}
+
+/* SelectCollisionElectron, MCCElectron.F90:13-147 */
+static void collide_electron(mcpo_t *m, int species, int rtype, int resultant, double th, rng_t *rg)
+{
+    switch (rtype) {
+    case 11:
+        post_collision(m, 1, rg);
+        break;
+    case 12: {
+        double VF = sqrt(1.0 - th / m->energy);
+        m->energy = m->energy - th;
+        mc_vel_update(m, VF);
+        post_collision(m, 1, rg);
+    } break;
+    case 13: {
+        po_t solid = m->poi;
+        double E0 = m->energy;
+        double EnergyTemp = m->energy - th;
+        double EnergyOld = create_energy_vahedi(EnergyTemp, rg);
+        double VF = sqrt(EnergyOld / m->energy);
+        mc_vel_update(m, VF);
+        post_collision(m, 1, rg);
+        double EnergyNew = EnergyTemp - EnergyOld;
+        double VF2 = sqrt(EnergyNew / E0);
+        m->nnew = 2;
+        m->pon[0] = solid;
+        m->pon[0].Vz = m->pon[0].Vz * VF2;
+        m->pon[0].Vr = m->pon[0].Vr * VF2;
+        m->pon[0].Vt = m->pon[0].Vt * VF2;
+        m->pon_idx[0] = species;
+        m->pon[1] = m->pot;
+        m->pon_idx[1] = resultant;
+    } break;
+    case 14:
+        m->nnew = 1;
+        m->pon[0] = m->pot;
+        m->pon_idx[0] = resultant;
+        break;
+    case 15: {
+        double VF = sqrt(1.0 - th / m->energy);
+        m->energy = m->energy - th;
+        mc_vel_update(m, VF);
+        post_collision(m, 1, rg);
+        m->nnew = 2;
+        m->pon[0] = m->pot;
+        m->pon[1] = m->pot;
+        m->pon_idx[0] = resultant % 10;
+        m->pon_idx[1] = resultant / 10;
+    } break;
+    default:
+        break;
+    }
+}
+
+/* PostVelocityIonReactive, MCCIon.F90:54-74 (Vy takes Vz, as in the source) */
+static void ion_reactive(mcpo_t *m, double th)
+{
+    double VF = sqrt(1.0 - th / m->energy);
+    m->energy = m->energy - th;
+    mc_vel_update(m, VF);
+    double A = m->mass_ratio, B = 1.0 - A;
+    m->poi.Vr = B * m->poi.Vr + A * m->pot.Vr + B * m->gx;
+    m->poi.Vt = B * m->poi.Vz + A * m->pot.Vt + B * m->gy;
+    m->poi.Vz = B * m->poi.Vz + A * m->pot.Vz + B * m->gz;
+}
+
+/* SelectCollisionIon, MCCIon.F90:6-51 */
+static void collide_ion(mcpo_t *m, int rtype, int resultant, double th, rng_t *rg)
+{
+    switch (rtype) {
+    case 101: post_collision(m, 0, rg); break;
+    case 102:
+    case 103: m->poi = m->pot; break;
+    case 100: break;
+    case 111: ion_reactive(m, th); break;
+    case 112:
+        ion_reactive(m, th);
+        m->nnew = 1; m->pon[0] = m->pot; m->pon_idx[0] = resultant;
+        break;
+    case 113:
+        ion_reactive(m, th);
+        m->nnew = 1; m->pon[0] = m->poi; m->pon_idx[0] = resultant;
+        break;
+    default: break;
+    }
+}
+
+int ora_mcc(const ora_mcc_setup *st, uint64_t seed, uint32_t step,
+            int64_t *n, const int64_t *cap, double *(*P)[8], int64_t *counts)
+{
+    const double VFactor = st->dx / st->dt;
+    const double invVF = 1.0 / VFactor;
+    int64_t nst = 0, nstcap = 1024;
+    po_t *stage = (po_t *)malloc(sizeof(po_t) * nstcap);
+    int *stage_idx = (int *)malloc(sizeof(int) * nstcap);
+    int rc = 0;
+    for (int g = 0; g < st->ng; ++g)
+        for (int s = 0; s < st->ns; ++s) {
+            const ora_mcc_table *t = &st->tab[s * st->ng + g];
+            if (t->model == 0 || n[s] <= 0) continue;
+            const int64_t np = n[s];
+            double *const *A = P[s];
+            for (int64_t i = 0; i < np; ++i) {
+                rng_t rg = {seed, (uint32_t)i, step, (uint32_t)s | ((uint32_t)g << 8), 0};
+                double R = rnext(&rg);
+                if (!(R < t->collision_ratio)) continue;
+                mcpo_t m;
+                memset(&m, 0, sizeof m);
+                m.poi.Z = A[0][i]; m.poi.R = A[1][i]; m.poi.Vz = A[2][i]; m.poi.Vr = A[3][i];
+                m.poi.Vt = A[4][i]; m.poi.Az = A[5][i]; m.poi.Ar = A[6][i]; m.poi.WQ = A[7][i];
+                m.poi.Vz = m.poi.Vz * VFactor; m.poi.Vr = m.poi.Vr * VFactor; m.poi.Vt = m.poi.Vt * VFactor;
+                mc_update(&m, st->mass[s], t->gas_mass, t->gas_temperature, &rg);
+                int idx = 0;
+                if (t->model == 1 || t->model == 2) idx = select_probability(&m, t, &rg);
+                else if (t->model == 3) idx = select_probability_reactive(&m, t, &rg);
+                if (idx > 0) {
+                    int ty = t->rtype[idx - 1];
+                    if (t->model == 1)
+                        collide_electron(&m, s, ty, t->resultant[idx - 1], t->threshold[idx - 1], &rg);
+                    else
+                        collide_ion(&m, ty, t->resultant[idx - 1], t->threshold[idx - 1], &rg);
+                }
+                if (counts) counts[((int64_t)s * st->ng + g) * 128 + (idx & 127)] += 1;
+                m.poi.Vz = m.poi.Vz * invVF; m.poi.Vr = m.poi.Vr * invVF; m.poi.Vt = m.poi.Vt * invVF;
+                A[0][i] = m.poi.Z; A[1][i] = m.poi.R; A[2][i] = m.poi.Vz; A[3][i] = m.poi.Vr;
+                A[4][i] = m.poi.Vt; A[5][i] = m.poi.Az; A[6][i] = m.poi.Ar; A[7][i] = m.poi.WQ;
+                for (int k = 0; k < m.nnew; ++k) {
+                    if (nst == nstcap) {
+                        nstcap *= 2;
+                        stage = (po_t *)realloc(stage, sizeof(po_t) * nstcap);
+                        stage_idx = (int *)realloc(stage_idx, sizeof(int) * nstcap);
+                    }
+                    stage[nst] = m.pon[k];
+                    stage_idx[nst] = m.pon_idx[k];
+                    ++nst;
+                }
+            }
+        }
+    /* MCCAddNewParticle, MCCPublic.F90:209-226 */
+    for (int64_t k = 0; k < nst; ++k) {
+        int d = stage_idx[k];
+        if (d < 0 || d >= st->ns || n[d] >= cap[d]) { rc = -1; continue; }
+        po_t p = stage[k];
+        p.Vz = p.Vz * invVF; p.Vr = p.Vr * invVF; p.Vt = p.Vt * invVF;
+        int64_t j = n[d]++;
+        P[d][0][j] = p.Z; P[d][1][j] = p.R; P[d][2][j] = p.Vz; P[d][3][j] = p.Vr;
+        P[d][4][j] = p.Vt; P[d][5][j] = p.Az; P[d][6][j] = p.Ar; P[d][7][j] = p.WQ;
+    }
+    free(stage);
+    free(stage_idx);
+    return rc;
+}
+
+/* ------------------------------------------------ cross-section tables */
+int ora_locate(const double *xx, int n, double x)
+{
+    /* xx is 1-based in the reference; here xx[0] is xx(1) */
+    int jl = 0, ju = n + 1;
+    while (ju - jl > 1) {
+        int jm = (ju + jl) / 2;
+        if ((xx[n - 1] >= xx[0]) == (x >= xx[jm - 1])) jl = jm;
+        else ju = jm;
+    }
+    if (x == xx[0]) return 1;
+    if (x == xx[n - 1]) return n - 1;
+    return jl;
+}
+
+#define NSIGMA_RAW_MAX 500
+
+void ora_sigma_normalize(int nreaction, int nsigma, double emin, double einterval,
+                         const double *threshold, const double *raw_e, const double *raw_s,
+                         double *sigma_out)
+{
+    double xx[NSIGMA_RAW_MAX + 2];
+    for (int j = 0; j < nreaction; ++j) {
+        const double *e = raw_e + (size_t)j * NSIGMA_RAW_MAX;
+        const double *s = raw_s + (size_t)j * NSIGMA_RAW_MAX;
+        int nraw = 1;   /* Maxloc: first location of the maximum, 1-based */
+        for (int i = 2; i <= NSIGMA_RAW_MAX; ++i)
+            if (e[i - 1] > e[nraw - 1]) nraw = i;
+        double MaxEnergy = e[nraw - 1];
+        double Alfa = s[nraw - 1] * MaxEnergy;
+        for (int i = 0; i < NSIGMA_RAW_MAX; ++i) xx[i] = e[i];
+        xx[NSIGMA_RAW_MAX] = 0.0; xx[NSIGMA_RAW_MAX + 1] = 0.0;
+        for (int i = 1; i <= nraw; ++i) {
+            if (xx[i - 1] < pow(10.0, emin)) xx[i - 1] = emin;
+            else xx[i - 1] = (log10(xx[i - 1]) - emin) / einterval;
+        }
+        double Th = threshold[j];
+        for (int i = 1; i <= nsigma; ++i) {
+            double NEnergy = (double)(i - 1);
+            double Energy = pow(10.0, emin + (double)(i - 1) * einterval);
+            double v;
+            if (Energy <= Th) v = 0.0;
+            else if (Energy >= MaxEnergy) v = Alfa / Energy;
+            else {
+                int k = ora_locate(xx, nraw, NEnergy);
+                if (k < 1) k = 1;
+                double LE = xx[k - 1], UE = xx[k];
+                double S1 = ((double)i - LE) / (UE - LE);
+                double S2 = 1.0 - S1;
+                double sk1 = (k < NSIGMA_RAW_MAX) ? s[k] : 0.0;
+                v = S2 * s[k - 1] + S1 * sk1;
+            }
+            sigma_out[(size_t)(i - 1) * nreaction + j] = v;
+        }
+    }
+}
+
+void ora_sigma_normalize_reactive(int nreaction, int nsigma, double emin, double einterval,
+                                  const double *threshold, double natom_species, double natom_gas,
+                                  double *sigma_out)
+{
+    double NComplex = natom_species + natom_gas;
+    double S = (3.0 * NComplex - 6.0) / 2.0 - 1.0;
+    double PTemp[256];
+    for (int i = 1; i <= nsigma; ++i) {
+        double Energy = pow(10.0, emin + (double)(i - 1) * einterval);
+        double PSum = 0.0;
+        for (int j = 0; j < nreaction; ++j) {
+            PTemp[j] = (Energy > threshold[j]) ? pow(Energy - threshold[j], S) : 0.0;
+        }
+        for (int j = 0; j < nreaction; ++j) PSum = PSum + PTemp[j];
+        for (int j = 0; j < nreaction; ++j) sigma_out[(size_t)(i - 1) * nreaction + j] = PTemp[j] / PSum;
+    }
+}
+
+double ora_probability_nonreactive(int nreaction, int nsigma, double emin, double einterval,
+                                   double species_mass, double gas_mass, double gas_density,
+                                   const double *sigma, double *prob)
+{
+    double Miu = species_mass * gas_mass / (species_mass + gas_mass);
+    for (int i = 1; i <= nsigma; ++i) {
+        double Energy = pow(10.0, emin + (double)(i - 1) * einterval);
+        double V = sqrt(2.0 * Energy * C_JTOEV / Miu);
+        for (int j = 0; j < nreaction; ++j) {
+            size_t id = (size_t)(i - 1) * nreaction + j;
+            prob[id] = sigma[id] * V * gas_density;
+        }
+    }
+    for (int i = 1; i <= nsigma; ++i)
+        for (int j = 1; j < nreaction; ++j) {
+            size_t id = (size_t)(i - 1) * nreaction + j;
+            prob[id] = prob[id] + prob[id - 1];
+        }
+    double Max = prob[0];
+    size_t nt = (size_t)nsigma * nreaction;
+    for (size_t k = 1; k < nt; ++k) if (prob[k] > Max) Max = prob[k];
+    for (size_t k = 0; k < nt; ++k) prob[k] = prob[k] / Max;
+    return Max;
+}
+
+double ora_probability_reactive(int nreaction, int nsigma, double species_mass, double gas_mass,
+                                double gas_density, double beta_max, double polarizability,
+                                double dt, const double *sigma, double *prob)
+{
+    size_t nt = (size_t)nsigma * nreaction;
+    for (size_t k = 0; k < nt; ++k) prob[k] = sigma[k];
+    double Miu = species_mass * gas_mass / (species_mass + gas_mass);
+    double ratio = sqrt(C_PI * polarizability * C_E * C_E / (C_EPS0 * Miu)) * beta_max * beta_max * gas_density * dt;
+    for (int i = 1; i <= nsigma; ++i)
+        for (int j = 1; j < nreaction; ++j) {
+            size_t id = (size_t)(i - 1) * nreaction + j;
+            prob[id] = prob[id] + prob[id - 1];
+        }
+    return ratio;
+}
+
+/* ------------------------------------------------ one step, in place ----
+ * The reference's OneStep particle sequence (R/src/onestep.F90:141-177):
+ * for every species MoveESIm then MaterialAborptionParticle, then P2CES raw
+ * deposits, then MCC.  P[s] arrays have capacity cap[s]; n[s] is updated.
+ * rho_raw: ns*nz*nr (accumulated, caller clears). Returns ora_mcc's code. */
+int ora_step(const ora_geom *g, const ora_mcc_setup *st, const double *Ez, const double *Er,
+             uint64_t seed, uint32_t step, int64_t *n, const int64_t *cap, double *(*P)[8],
+             double *rho_raw, double *metal_charge, double *sigma)
+{
+    const size_t nn = (size_t)g->nz * g->nr;
+    for (int s = 0; s < st->ns; ++s) {
+        double **A = P[s];
+        ora_push(g->nz, g->nr, st->dx, st->dt, st->charge[s], st->mass[s], Ez, Er, n[s],
+                 A[0], A[1], A[2], A[3], A[4], A[5], A[6]);
+        ora_absorb(g, st->charge[s], &n[s], A, metal_charge, sigma);
+    }
+    for (int s = 0; s < st->ns; ++s)
+        ora_deposit_raw(g->nz, g->nr, n[s], P[s][0], P[s][1], P[s][7], rho_raw + (size_t)s * nn);
+    return ora_mcc(st, seed, step, n, cap, P, NULL);
+}

@@ -1,0 +1,136 @@
+/*
+ * CPU oracle for the iPM2D particle hot path (TEST INFRASTRUCTURE ONLY).
+ *
+ * A plain-C restatement, statement by statement, of the reference routines
+ * listed below.  It exists to check the HIP product path; nothing in the
+ * product links, loads or calls it.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may use it.
+ *
+ * Array conventions follow the reference's Fortran column-major layouts with
+ * z fastest (A(iz, ir)):
+ *   Ez(0:Nz, 1:Nr)        -> Ez[iz + (ir-1)*(Nz+1)]
+ *   Er(1:Nz, 0:Nr)        -> Er[(iz-1) + ir*Nz]
+ *   Rho(1:Nz, 1:Nr)       -> Rho[(iz-1) + (ir-1)*Nz]
+ *   cell_material(1:Nz-1, 1:Nr-1), hedge_type(1:Nz, 1:Nr-1),
+ *   vedge_type(1:Nz-1, 1:Nr)
+ * Particles are structure-of-arrays of the 8 ParticleOne fields
+ * (R/src/pic/particle/ParticleOne.f90:8): Z R Vz Vr Vt Az Ar WQ, positions in
+ * grid units and velocities in grid units per step.
+ */
+#ifndef IPM_ORACLE_H
+#define IPM_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- counter-based RNG: Philox4x32-10 (Salmon et al., SC'11) ---------- */
+void ora_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* one uniform double in (0,1) for draw `draw` of particle `slot` */
+double ora_uniform(uint64_t seed, uint32_t slot, uint32_t step, uint32_t draw, uint32_t tag);
+
+/* ---- grid / geometry -------------------------------------------------- */
+typedef struct {
+    int nz, nr;              /* node counts (GlobalShape) */
+    double dx, dt;
+    const int *cell_material;  /* (nz-1)*(nr-1) */
+    const int *hedge_type;     /* nz*(nr-1) */
+    const int *vedge_type;     /* (nz-1)*nr */
+    const int *material_type;  /* material_type[m + ndiel], m in [-ndiel, nmetal] */
+    int ndiel, nmetal;
+} ora_geom;
+
+/* node weighting volumes, R/src/base/init.F90:402-424 */
+void ora_node_volumes(int nz, int nr, double dx, const double *factor,
+                      double *vol, double *inv_vol);
+
+/* MoveESIm, R/src/pic/particle/ParticleBundle.f90:671-748 */
+void ora_push(int nz, int nr, double dx, double dt, double charge, double mass,
+              const double *Ez, const double *Er, int64_t n,
+              double *Z, double *R, double *Vz, double *Vr, double *Vt,
+              double *Az, double *Ar);
+
+/* MaterialAborptionParticle, R/src/pic/particleboundary/ParticleMaterial.F90:10-90,
+ * with DelOne's swap-with-last deletion (ParticleBundle.f90:556-576).
+ * *n is updated; metal_charge[m-1] and sigma (nz*nr) are accumulated (+=). */
+void ora_absorb(const ora_geom *g, double charge, int64_t *n, double *P[8],
+                double *metal_charge, double *sigma);
+
+/* P2CES raw accumulation, R/src/pic/field/FieldOne.f90:151-166 (rho_raw += ...) */
+void ora_deposit_raw(int nz, int nr, int64_t n, const double *Z, const double *R,
+                     const double *WQ, double *rho_raw);
+/* P2CES epilogue (FieldOne.f90:168-175): rho = raw*inv_vol*q, chi = rho*ChiFactor */
+void ora_deposit_finish(int nz, int nr, double dt, double charge, double mass,
+                        const double *rho_raw, const double *inv_vol,
+                        double *rho, double *chi);
+
+/* ---- MCC ---------------------------------------------------------------- */
+typedef struct {
+    int model;            /* 0 none, 1 electron, 2 non-reactive ion, 3 reactive ion */
+    int nreaction, nsigma;
+    double emin, einterval, emax;   /* emax = 10**EnergyMax */
+    double collision_ratio;
+    const int *rtype;       /* ReactionType per reaction */
+    const int *resultant;   /* Resultant species index per reaction */
+    const double *threshold;
+    const double *prob;     /* Probility(j,i) -> prob[(i-1)*nreaction + (j-1)] */
+    double gas_mass, gas_temperature, beta_max;
+} ora_mcc_table;
+
+typedef struct {
+    int ns;       /* number of species (0 = electrons) */
+    int ng;       /* number of gases */
+    double dx, dt;
+    const double *charge, *mass;     /* per species */
+    const ora_mcc_table *tab;        /* tab[s*ng + g] */
+} ora_mcc_setup;
+
+/* MCC, R/src/mc/MCCPublic.F90:38-226 and callees.  Draws come from
+ * ora_uniform(seed, slot, step, k, species) with k counting per particle.
+ * Species arrays P[s][0..7] have capacity cap[s]; n[s] updated with products
+ * appended at the end (in reference order).  counts[s*ng*64 + g*64 + type%64]
+ * (if non-NULL) tallies collision events per species, gas and reaction type.
+ * Returns 0, or -1 on capacity overflow. */
+int ora_mcc(const ora_mcc_setup *st, uint64_t seed, uint32_t step,
+            int64_t *n, const int64_t *cap, double *(*P)[8], int64_t *counts);
+
+/* Maxwellian speed + isotropic direction, ParticleOne.f90:76-123, with draws
+ * from a per-particle stream; returns the next draw index. */
+uint32_t ora_vel_maxwell(double mass, double temperature, uint64_t seed, uint32_t slot,
+                         uint32_t step, uint32_t draw, uint32_t tag,
+                         double *vz, double *vr, double *vt);
+
+/* OneStep's particle sequence in place: push + absorb per species, raw
+ * deposits (rho_raw: ns*nz*nr, accumulated), then MCC. */
+int ora_step(const ora_geom *g, const ora_mcc_setup *st, const double *Ez, const double *Er,
+             uint64_t seed, uint32_t step, int64_t *n, const int64_t *cap, double *(*P)[8],
+             double *rho_raw, double *metal_charge, double *sigma);
+
+/* ---- cross-section tables (R/src/mc/MCCSigma.f90, MCCInit.F90) --------- */
+/* locate, R/src/base/numerical/Numrical.F90:112-145 (1-based result) */
+int ora_locate(const double *xx, int n, double x);
+
+/* SigmaNormalizedUpdate (MCCSigma.f90:131-190) for Model 1/2.
+ * raw_e/raw_s: NSigmaRawMax(=500) entries per reaction, zero padded.
+ * sigma_out: Sigma(j,i) -> sigma_out[(i-1)*nreaction + (j-1)] */
+void ora_sigma_normalize(int nreaction, int nsigma, double emin, double einterval,
+                         const double *threshold, const double *raw_e, const double *raw_s,
+                         double *sigma_out);
+/* SigmaNormalizedUpdateReactive (MCCSigma.f90:192-220) */
+void ora_sigma_normalize_reactive(int nreaction, int nsigma, double emin, double einterval,
+                                  const double *threshold, double natom_species, double natom_gas,
+                                  double *sigma_out);
+/* ProbilityNonReactive (MCCInit.F90:125-170); returns SigmaMax */
+double ora_probability_nonreactive(int nreaction, int nsigma, double emin, double einterval,
+                                   double species_mass, double gas_mass, double gas_density,
+                                   const double *sigma, double *prob);
+/* ProbilityReactive (MCCInit.F90:272-309); returns CollisionRatio */
+double ora_probability_reactive(int nreaction, int nsigma, double species_mass, double gas_mass,
+                                double gas_density, double beta_max, double polarizability,
+                                double dt, const double *sigma, double *prob);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
