@@ -93,18 +93,51 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
     }
 }
 
+// Per-particle random stream.  The counter carries 96 bits folded from the
+// particle's phase-space state at the start of the MCC pass plus the draw
+// index; the key carries the seed, the step and the (species, gas) tag.  The
+// stream therefore depends on what the particle is, not on where it is
+// stored: sorting, compaction and the order in which products are appended
+// do not change any draw.
 struct Rng {
-    uint32_t slot, step, tag, k;
+    uint32_t c0, c1, c2, k;
     uint32_t key0, key1;
     __device__ __forceinline__ double next()
     {
-        uint32_t c[4] = {slot, step, k, tag};
+        uint32_t c[4] = {c0, c1, c2, k};
         ++k;
         philox4x32_10(c, key0, key1);
         uint64_t x = ((((uint64_t)c[1]) << 32) | c[0]) >> 12;   // 52 bits
         return ((double)x + 0.5) * 0x1p-52;                     // (0,1)
     }
 };
+
+__device__ __forceinline__ uint32_t rotl32(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+
+struct StateKey {
+    uint32_t c0, c1, c2;
+};
+
+__device__ __forceinline__ StateKey state_key(double Z, double R, double Vz, double Vr, double Vt)
+{
+    const uint64_t z = (uint64_t)__double_as_longlong(Z), r = (uint64_t)__double_as_longlong(R);
+    const uint64_t vz = (uint64_t)__double_as_longlong(Vz), vr = (uint64_t)__double_as_longlong(Vr);
+    const uint64_t vt = (uint64_t)__double_as_longlong(Vt);
+    StateKey s;
+    s.c0 = (uint32_t)z ^ (uint32_t)(vr >> 32) ^ rotl32((uint32_t)(vt >> 32), 13);
+    s.c1 = (uint32_t)r ^ (uint32_t)(vz >> 32) ^ rotl32((uint32_t)vt, 7);
+    s.c2 = (uint32_t)vz ^ rotl32((uint32_t)vr, 11) ^ rotl32((uint32_t)(z >> 32), 3) ^ rotl32((uint32_t)(r >> 32), 19);
+    return s;
+}
+
+__device__ __forceinline__ Rng make_rng(const StateKey &sk, unsigned long long seed, uint32_t step, uint32_t tag)
+{
+    Rng g;
+    g.c0 = sk.c0; g.c1 = sk.c1; g.c2 = sk.c2; g.k = 0;
+    g.key0 = (uint32_t)seed + step * 0x9E3779B9u;
+    g.key1 = (uint32_t)(seed >> 32) ^ (tag * 0x85EBCA6Bu);
+    return g;
+}
 
 // ---------------------------------------------------------------- push
 struct Part {
@@ -191,8 +224,10 @@ __device__ __forceinline__ bool absorb_one(const DevState &st, double charge, Pa
         p.Ar = fabs(p.Ar);
         return false;
     }
-    int NzL = (int)ceil(p.Z);
-    int NrL = (int)ceil(p.R);
+    // 0 < Z < nz-1 and 0 < R < nr-1 here, so the clamps only matter for a
+    // non-finite position, which must not read outside the geometry arrays.
+    int NzL = min(max((int)ceil(p.Z), 1), nz - 1);
+    int NrL = min(max((int)ceil(p.R), 1), nr - 1);
     int m = st.cell_material[(size_t)(NzL - 1) + (size_t)(NrL - 1) * (nz - 1)];
     int t = st.mat_type[m + st.ndiel];
     if (t == kMatMetal) {

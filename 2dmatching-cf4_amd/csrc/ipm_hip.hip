@@ -194,8 +194,8 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
         t0 = 0; t1 = T; tfirst = blockIdx.x; tstride = G;
     }
 
-    const unsigned long long step = ctr->step;
-    const uint32_t key0 = (uint32_t)ctr->seed, key1 = (uint32_t)(ctr->seed >> 32);
+    const uint32_t step = ctr->step;
+    const unsigned long long seed = ctr->seed;
     const int lane = threadIdx.x & 63;
 
     for (long long t = tfirst; t < t1; t += tstride) {
@@ -289,10 +289,15 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
             // with their post-push state; collide_kernel finishes them.
             bool selected = false;
             if ((MODE & M_MCC) && alive) {
+                if (!(MODE & (M_PUSH | M_ABSORB))) {
+                    p.R = F[1][i];
+                    p.Vz = F[2][i]; p.Vr = F[3][i]; p.Vt = F[4][i];
+                }
+                const StateKey sk = state_key(p.Z, p.R, p.Vz, p.Vr, p.Vt);
                 for (int g = 0; g < st.ng; ++g) {
                     const DevTable &tb = st.tab[s * st.ng + g];
                     if (tb.model == 0) continue;
-                    Rng rg{(uint32_t)i, (uint32_t)step, (uint32_t)s | ((uint32_t)g << 8), 0u, key0, key1};
+                    Rng rg = make_rng(sk, seed, step, (uint32_t)s | ((uint32_t)g << 8));
                     if (rg.next() < tb.ratio) selected = true;
                 }
             }
@@ -306,12 +311,8 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
                     if (selected) {
                         const unsigned long long q = qb + __popcll(m & ((1ull << lane) - 1ull));
                         if (q < (unsigned long long)st.q_cap) {
-                            if (!(MODE & (M_PUSH | M_ABSORB | M_DEPOSIT))) {
-                                p.R = F[1][i]; p.WQ = F[7][i];
-                                p.Vz = F[2][i]; p.Vr = F[3][i]; p.Vt = F[4][i];
-                                p.Az = F[5][i]; p.Ar = F[6][i];
-                            } else if (!(MODE & (M_PUSH | M_ABSORB))) {
-                                p.Vz = F[2][i]; p.Vr = F[3][i]; p.Vt = F[4][i];
+                            if (!(MODE & (M_PUSH | M_ABSORB | M_DEPOSIT))) p.WQ = F[7][i];
+                            if (!(MODE & (M_PUSH | M_ABSORB))) {
                                 p.Az = F[5][i]; p.Ar = F[6][i];
                             }
                             st.q_f[0][q] = p.Z; st.q_f[1][q] = p.R; st.q_f[2][q] = p.Vz; st.q_f[3][q] = p.Vr;
@@ -377,8 +378,8 @@ collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
     const DevState &st = *stp;
     const unsigned long long nq_all = ctr->q_count;
     const unsigned long long nq = nq_all < (unsigned long long)st.q_cap ? nq_all : (unsigned long long)st.q_cap;
-    const unsigned step = ctr->step;
-    const uint32_t key0 = (uint32_t)ctr->seed, key1 = (uint32_t)(ctr->seed >> 32);
+    const uint32_t step = ctr->step;
+    const unsigned long long seed = ctr->seed;
     const unsigned long long stride = (unsigned long long)gridDim.x * kBlock;
     for (unsigned long long q = (unsigned long long)blockIdx.x * kBlock + threadIdx.x; q < nq; q += stride) {
         Part p;
@@ -386,10 +387,11 @@ collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
         p.Vt = st.q_f[4][q]; p.Az = st.q_f[5][q]; p.Ar = st.q_f[6][q]; p.WQ = st.q_f[7][q];
         const long long i = st.q_slot[q];
         const int s = st.q_species[q];
+        const StateKey sk = state_key(p.Z, p.R, p.Vz, p.Vr, p.Vt);
         for (int g = 0; g < st.ng; ++g) {
             const DevTable &tb = st.tab[s * st.ng + g];
             if (tb.model == 0) continue;
-            Rng rg{(uint32_t)i, step, (uint32_t)s | ((uint32_t)g << 8), 0u, key0, key1};
+            Rng rg = make_rng(sk, seed, step, (uint32_t)s | ((uint32_t)g << 8));
             if (!(rg.next() < tb.ratio)) continue;
             mcc_collide(st, ctr, s, g, tb, p, rg);
         }

@@ -56,18 +56,41 @@ void ora_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint3
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
-double ora_uniform(uint64_t seed, uint32_t slot, uint32_t step, uint32_t draw, uint32_t tag)
+/* Draw `draw` of the stream whose 96-bit counter prefix is (c0, c1, c2):
+ * counter {c0, c1, c2, draw}, key {seed_lo + step*0x9E3779B9,
+ * seed_hi ^ tag*0x85EBCA6B}, 52 random bits -> (k + 1/2) 2^-52 in (0,1). */
+double ora_uniform(uint64_t seed, uint32_t step, uint32_t tag, uint32_t c0, uint32_t c1, uint32_t c2,
+                   uint32_t draw)
 {
-    uint32_t ctr[4] = {slot, step, draw, tag};
-    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t ctr[4] = {c0, c1, c2, draw};
+    uint32_t key[2] = {(uint32_t)seed + step * 0x9E3779B9u, (uint32_t)(seed >> 32) ^ (tag * 0x85EBCA6Bu)};
     uint32_t o[4];
     ora_philox4x32_10(ctr, key, o);
     uint64_t x = ((((uint64_t)o[1]) << 32) | o[0]) >> 12;   /* 52 bits */
     return ((double)x + 0.5) * 0x1p-52;                    /* in (0,1) */
 }
 
-typedef struct { uint64_t seed; uint32_t slot, step, tag, k; } rng_t;
-static double rnext(rng_t *r) { return ora_uniform(r->seed, r->slot, r->step, r->k++, r->tag); }
+static uint32_t rotl32(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+
+static uint64_t dbits(double d)
+{
+    uint64_t u;
+    memcpy(&u, &d, sizeof u);
+    return u;
+}
+
+/* 96-bit stream prefix folded from a particle's phase-space state: the
+ * stream follows the particle, not its storage slot. */
+void ora_state_key(double Z, double R, double Vz, double Vr, double Vt, uint32_t out[3])
+{
+    uint64_t z = dbits(Z), r = dbits(R), vz = dbits(Vz), vr = dbits(Vr), vt = dbits(Vt);
+    out[0] = (uint32_t)z ^ (uint32_t)(vr >> 32) ^ rotl32((uint32_t)(vt >> 32), 13);
+    out[1] = (uint32_t)r ^ (uint32_t)(vz >> 32) ^ rotl32((uint32_t)vt, 7);
+    out[2] = (uint32_t)vz ^ rotl32((uint32_t)vr, 11) ^ rotl32((uint32_t)(z >> 32), 3) ^ rotl32((uint32_t)(r >> 32), 19);
+}
+
+typedef struct { uint64_t seed; uint32_t step, tag, c0, c1, c2, k; } rng_t;
+static double rnext(rng_t *r) { return ora_uniform(r->seed, r->step, r->tag, r->c0, r->c1, r->c2, r->k++); }
 
 /* ------------------------------------------------------- node volumes */
 void ora_node_volumes(int nz, int nr, double dx, const double *factor, double *vol, double *inv_vol)
@@ -301,7 +324,7 @@ uint32_t ora_vel_maxwell(double mass, double temperature, uint64_t seed, uint32_
                          uint32_t step, uint32_t draw, uint32_t tag,
                          double *vz, double *vr, double *vt)
 {
-    rng_t rg = {seed, slot, step, tag, draw};
+    rng_t rg = {seed, step, tag, slot, 0, 0, draw};
     po_t p;
     memset(&p, 0, sizeof p);
     vel_max_init(&p, mass, temperature, &rg);
@@ -509,6 +532,13 @@ int ora_mcc(const ora_mcc_setup *st, uint64_t seed, uint32_t step,
     po_t *stage = (po_t *)malloc(sizeof(po_t) * nstcap);
     int *stage_idx = (int *)malloc(sizeof(int) * nstcap);
     int rc = 0;
+    /* every particle's stream is keyed by its state at the start of MCC */
+    uint32_t *keys[64] = {0};
+    for (int s = 0; s < st->ns && s < 64; ++s) {
+        keys[s] = (uint32_t *)malloc(sizeof(uint32_t) * 3 * (size_t)(n[s] > 0 ? n[s] : 1));
+        for (int64_t i = 0; i < n[s]; ++i)
+            ora_state_key(P[s][0][i], P[s][1][i], P[s][2][i], P[s][3][i], P[s][4][i], keys[s] + 3 * i);
+    }
     for (int g = 0; g < st->ng; ++g)
         for (int s = 0; s < st->ns; ++s) {
             const ora_mcc_table *t = &st->tab[s * st->ng + g];
@@ -516,7 +546,8 @@ int ora_mcc(const ora_mcc_setup *st, uint64_t seed, uint32_t step,
             const int64_t np = n[s];
             double *const *A = P[s];
             for (int64_t i = 0; i < np; ++i) {
-                rng_t rg = {seed, (uint32_t)i, step, (uint32_t)s | ((uint32_t)g << 8), 0};
+                const uint32_t *kk = keys[s] + 3 * i;
+                rng_t rg = {seed, step, (uint32_t)s | ((uint32_t)g << 8), kk[0], kk[1], kk[2], 0};
                 double R = rnext(&rg);
                 if (!(R < t->collision_ratio)) continue;
                 mcpo_t m;
@@ -563,6 +594,7 @@ int ora_mcc(const ora_mcc_setup *st, uint64_t seed, uint32_t step,
     }
     free(stage);
     free(stage_idx);
+    for (int s = 0; s < st->ns && s < 64; ++s) free(keys[s]);
     return rc;
 }
 
@@ -701,4 +733,19 @@ int ora_step(const ora_geom *g, const ora_mcc_setup *st, const double *Ez, const
     for (int s = 0; s < st->ns; ++s)
         ora_deposit_raw(g->nz, g->nr, n[s], P[s][0], P[s][1], P[s][7], rho_raw + (size_t)s * nn);
     return ora_mcc(st, seed, step, n, cap, P, NULL);
+}
+
+/* Exposes the scattering-angle samplers (IsotropicCosKai / CosKaiVahedi,
+ * R/src/mc/MCCChi.f90:5-25) and the secondary-energy sampler
+ * (CreatEnergyVehadi, MCCChi.f90:49-59) for distribution tests. */
+double ora_cos_chi(int vahedi, double energy, uint64_t seed, uint32_t slot, uint32_t draw)
+{
+    rng_t rg = {seed, 0, 0, slot, 0, 0, draw};
+    return vahedi ? cos_vahedi(energy, &rg) : cos_isotropic(energy, &rg);
+}
+
+double ora_secondary_energy(double energy, uint64_t seed, uint32_t slot, uint32_t draw)
+{
+    rng_t rg = {seed, 0, 0, slot, 0, 0, draw};
+    return create_energy_vahedi(energy, &rg);
 }

@@ -27,8 +27,11 @@ extern "C" {
 
 /* ---- counter-based RNG: Philox4x32-10 (Salmon et al., SC'11) ---------- */
 void ora_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
-/* one uniform double in (0,1) for draw `draw` of particle `slot` */
-double ora_uniform(uint64_t seed, uint32_t slot, uint32_t step, uint32_t draw, uint32_t tag);
+/* one uniform double in (0,1): draw `draw` of the stream (c0, c1, c2) */
+double ora_uniform(uint64_t seed, uint32_t step, uint32_t tag, uint32_t c0, uint32_t c1, uint32_t c2,
+                   uint32_t draw);
+/* the stream prefix of a particle: 96 bits folded from (Z, R, Vz, Vr, Vt) */
+void ora_state_key(double Z, double R, double Vz, double Vr, double Vt, uint32_t out[3]);
 
 /* ---- grid / geometry -------------------------------------------------- */
 typedef struct {
@@ -87,7 +90,8 @@ typedef struct {
 } ora_mcc_setup;
 
 /* MCC, R/src/mc/MCCPublic.F90:38-226 and callees.  Draws come from
- * ora_uniform(seed, slot, step, k, species) with k counting per particle.
+ * ora_uniform(seed, step, species|gas<<8, key(state at MCC start), k), k
+ * counting draws per particle and gas.
  * Species arrays P[s][0..7] have capacity cap[s]; n[s] updated with products
  * appended at the end (in reference order).  counts[s*ng*64 + g*64 + type%64]
  * (if non-NULL) tallies collision events per species, gas and reaction type.
@@ -106,6 +110,10 @@ uint32_t ora_vel_maxwell(double mass, double temperature, uint64_t seed, uint32_
 int ora_step(const ora_geom *g, const ora_mcc_setup *st, const double *Ez, const double *Er,
              uint64_t seed, uint32_t step, int64_t *n, const int64_t *cap, double *(*P)[8],
              double *rho_raw, double *metal_charge, double *sigma);
+
+/* Scattering-angle and secondary-energy samplers, one draw each. */
+double ora_cos_chi(int vahedi, double energy, uint64_t seed, uint32_t slot, uint32_t draw);
+double ora_secondary_energy(double energy, uint64_t seed, uint32_t slot, uint32_t draw);
 
 /* ---- cross-section tables (R/src/mc/MCCSigma.f90, MCCInit.F90) --------- */
 /* locate, R/src/base/numerical/Numrical.F90:112-145 (1-based result) */

@@ -50,7 +50,8 @@ def lib(fast=False):
     L = C.CDLL(path)
     L.ora_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     L.ora_uniform.restype = C.c_double
-    L.ora_uniform.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
+    L.ora_uniform.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
+    L.ora_state_key.argtypes = [C.c_double] * 5 + [C.POINTER(C.c_uint32)]
     L.ora_node_volumes.argtypes = [C.c_int, C.c_int, C.c_double, _dp, _dp, _dp]
     L.ora_push.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp,
                            C.c_int64] + [_dp] * 7
@@ -65,6 +66,10 @@ def lib(fast=False):
     L.ora_step.restype = C.c_int
     L.ora_step.argtypes = [C.POINTER(OraGeom), C.POINTER(OraSetup), _dp, _dp, C.c_uint64, C.c_uint32, _i64p,
                            _i64p, C.c_void_p, _dp, _dp, _dp]
+    L.ora_cos_chi.restype = C.c_double
+    L.ora_cos_chi.argtypes = [C.c_int, C.c_double, C.c_uint64, C.c_uint32, C.c_uint32]
+    L.ora_secondary_energy.restype = C.c_double
+    L.ora_secondary_energy.argtypes = [C.c_double, C.c_uint64, C.c_uint32, C.c_uint32]
     L.ora_locate.restype = C.c_int
     L.ora_locate.argtypes = [_dp, C.c_int, C.c_double]
     L.ora_sigma_normalize.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp]
@@ -108,8 +113,14 @@ def philox(ctr, key):
     return [o[i] for i in range(4)]
 
 
-def uniform(seed, slot, step, draw, tag):
-    return lib().ora_uniform(seed, slot, step, draw, tag)
+def uniform(seed, step, tag, c0, c1, c2, draw):
+    return lib().ora_uniform(seed, step, tag, c0, c1, c2, draw)
+
+
+def state_key(z, r, vz, vr, vt):
+    o = (C.c_uint32 * 3)()
+    lib().ora_state_key(z, r, vz, vr, vt, o)
+    return [o[0], o[1], o[2]]
 
 
 # ------------------------------------------------------------- kernels
@@ -214,10 +225,11 @@ def mcc(setup, seed, step, species, extra_capacity=None, counts=False, fast=Fals
     ns = setup.ns
     caps = []
     bufs = []
+    total = sum(np.asarray(P).shape[1] for P in species)
     for s in range(ns):
         P = np.asarray(species[s], dtype=np.float64)
         n = P.shape[1]
-        cap = n + (extra_capacity if extra_capacity is not None else max(1024, n))
+        cap = n + (extra_capacity if extra_capacity is not None else 2 * total + 1024)
         B = np.zeros((8, cap))
         B[:, :n] = P
         bufs.append(B)

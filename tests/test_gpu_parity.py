@@ -320,3 +320,18 @@ def test_capacity_overflow_raises(rng):
     hp.set_rng(1, 0)
     with pytest.raises(CapacityError):
         hp.mcc()
+
+
+def test_mcc_independent_of_storage_order(rng):
+    """Streams follow particle state: a permuted upload gives the same set."""
+    cfg, tables, species = _six_species_setup(pressure=800.0)
+    geom = W.make_geometry(cfg)
+    P = [_energetic(cfg, s, 12000, rng, 25.0 if s == 0 else 3.0) for s in range(cfg.ns)]
+    Q = [X[:, rng.permutation(X.shape[1])] for X in P]
+    a = _config_hp(cfg, geom, tables, P)
+    b = _config_hp(cfg, geom, tables, Q)
+    for hp in (a, b):
+        hp.set_rng(21, 4)
+        hp.mcc()
+    for s in range(cfg.ns):
+        np.testing.assert_array_equal(sort_set(a.download(s)), sort_set(b.download(s)))
