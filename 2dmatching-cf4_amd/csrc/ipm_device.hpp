@@ -55,13 +55,22 @@ struct DevState {
     unsigned long long *mcc_counts;    // optional: (s*ng+g)*128 + j
     int count_events;
     int pad;
-    // queue of particles selected for a collision (null-collision pass)
+    // Queue of particles selected for a collision (null-collision pass),
+    // split into kQueueParts partitions of q_part_cap entries; partition p
+    // holds entries [p*q_part_cap, p*q_part_cap + count[p]).  A wave always
+    // appends to the partition of its global wave index, so the append
+    // atomics spread over kQueueParts counters (one 128-byte line each).
     double *q_f[8];
     long long *q_slot;
     int *q_species;
     long long q_cap;
+    long long q_part_cap;
+    unsigned int *q_part_count;        // kQueueParts * kQueueStride
     DevTable tab[IPM_NS_MAX * IPM_NG_MAX];
 };
+
+constexpr int kQueueParts = 512;
+constexpr int kQueueStride = 32;       // u32 per counter: one 128-byte line
 
 // Mutable per-step counters (device memory).
 struct DevCounters {
@@ -73,8 +82,25 @@ struct DevCounters {
     unsigned int blocks_done;
     int overflow;                          // 1 capacity, 2 bad species, 4 queue
     int pad;
-    unsigned long long q_count;            // entries in the collision queue
 };
+
+// Global-address-space views of device pointers: loads, stores and atomics
+// through them are global_* instructions instead of flat_*.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gp(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gp(const T *p)
+{
+    return (const __attribute__((address_space(1))) T *)p;
+}
+
+__device__ __forceinline__ void global_add(double *a, double v)
+{
+    __hip_atomic_fetch_add(gp(a), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // ---------------------------------------------------------------- RNG
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
@@ -172,9 +198,9 @@ __device__ __forceinline__ void push_one(const DevState &st, double EFactor, Par
     int j2b = min(max(N2, 0), nr - 1) + 1;
     int j2h = min(max(N2H, 0), nr - 1);
 
-    const double *er0 = Er + (size_t)(i1 - 1) + (size_t)j2h * ldr;
-    const double *ez0 = Ez + (size_t)i1h + (size_t)(j2 - 1) * ldz;
-    const double *ez1 = Ez + (size_t)i1h + (size_t)(j2b - 1) * ldz;
+    const auto *er0 = gp(Er + (size_t)(i1 - 1) + (size_t)j2h * ldr);
+    const auto *ez0 = gp(Ez + (size_t)i1h + (size_t)(j2 - 1) * ldz);
+    const auto *ez1 = gp(Ez + (size_t)i1h + (size_t)(j2b - 1) * ldz);
     double E1 = er0[0] * Z1 * R1H + er0[1] * Z2 * R1H + er0[ldr] * Z1 * R2H + er0[ldr + 1] * Z2 * R2H;
     double E2 = ez0[0] * Z1H * R1 + ez0[1] * Z2H * R1 + ez1[0] * Z1H * R2 + ez1[1] * Z2H * R2;
     E1 = E1 * EFactor;
@@ -228,14 +254,14 @@ __device__ __forceinline__ bool absorb_one(const DevState &st, double charge, Pa
     // non-finite position, which must not read outside the geometry arrays.
     int NzL = min(max((int)ceil(p.Z), 1), nz - 1);
     int NrL = min(max((int)ceil(p.R), 1), nr - 1);
-    int m = st.cell_material[(size_t)(NzL - 1) + (size_t)(NrL - 1) * (nz - 1)];
-    int t = st.mat_type[m + st.ndiel];
+    int m = gp(st.cell_material)[(size_t)(NzL - 1) + (size_t)(NrL - 1) * (nz - 1)];
+    int t = gp(st.mat_type)[m + st.ndiel];
     if (t == kMatMetal) {
-        unsafeAtomicAdd(&st.metal_charge[m - 1], p.WQ * charge);
+        global_add(&st.metal_charge[m - 1], p.WQ * charge);
         return true;
     }
     if (t == kMatDiel) {
-        const int *he = st.hedge, *ve = st.vedge;
+        const auto *he = gp(st.hedge), *ve = gp(st.vedge);
         double *sg = st.sigma;
         auto HE = [&](int i, int j) { return he[(size_t)(i - 1) + (size_t)(j - 1) * nz]; };
         auto VE = [&](int i, int j) { return ve[(size_t)(i - 1) + (size_t)(j - 1) * (nz - 1)]; };
@@ -244,20 +270,20 @@ __device__ __forceinline__ bool absorb_one(const DevState &st, double charge, Pa
         int v0 = VE(NzL, NrL), v1 = VE(NzL, NrL + 1);
         if (h0 == kEdgeV2D || h0 == kEdgeD2V) {
             double RL = (double)NrL - p.R, RU = 1.0 - RL;
-            unsafeAtomicAdd(SG(NzL, NrL), RL * p.WQ * charge);
-            unsafeAtomicAdd(SG(NzL, NrL + 1), RU * p.WQ * charge);
+            global_add(SG(NzL, NrL), RL * p.WQ * charge);
+            global_add(SG(NzL, NrL + 1), RU * p.WQ * charge);
         } else if (h1 == kEdgeV2D || h1 == kEdgeD2V) {
             double RL = (double)NrL - p.R, RU = 1.0 - RL;
-            unsafeAtomicAdd(SG(NzL + 1, NrL), RL * p.WQ * charge);
-            unsafeAtomicAdd(SG(NzL + 1, NrL + 1), RU * p.WQ * charge);
+            global_add(SG(NzL + 1, NrL), RL * p.WQ * charge);
+            global_add(SG(NzL + 1, NrL + 1), RU * p.WQ * charge);
         } else if (v0 == kEdgeV2D || v0 == kEdgeD2V) {
             double ZL = (double)NzL - p.Z, ZU = 1.0 - ZL;
-            unsafeAtomicAdd(SG(NzL, NrL), ZL * p.WQ * charge);
-            unsafeAtomicAdd(SG(NzL + 1, NrL), ZU * p.WQ * charge);
+            global_add(SG(NzL, NrL), ZL * p.WQ * charge);
+            global_add(SG(NzL + 1, NrL), ZU * p.WQ * charge);
         } else if (v1 == kEdgeV2D || v1 == kEdgeD2V) {
             double ZL = (double)NzL - p.Z, ZU = 1.0 - ZL;
-            unsafeAtomicAdd(SG(NzL, NrL + 1), ZL * p.WQ * charge);
-            unsafeAtomicAdd(SG(NzL + 1, NrL + 1), ZU * p.WQ * charge);
+            global_add(SG(NzL, NrL + 1), ZL * p.WQ * charge);
+            global_add(SG(NzL + 1, NrL + 1), ZU * p.WQ * charge);
         }
         return true;
     }

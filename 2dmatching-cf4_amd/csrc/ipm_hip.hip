@@ -157,6 +157,69 @@ __device__ __forceinline__ void lds_add(double *a, double v)
     __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+
+// DPP row shift right by D within rows of 16 lanes; lanes without a source
+// in their row receive 0.
+template <int D>
+__device__ __forceinline__ double dpp_shr(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x110 + D, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x110 + D, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int D>
+__device__ __forceinline__ int dpp_shr_i(int x)
+{
+    return __builtin_amdgcn_update_dpp(0, x, 0x110 + D, 0xf, 0xf, false);
+}
+__device__ __forceinline__ double readlane_d(double x, int l)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <int D>
+__device__ __forceinline__ void seg_step(double (&v)[4], int &f)
+{
+    const double a0 = dpp_shr<D>(v[0]), a1 = dpp_shr<D>(v[1]), a2 = dpp_shr<D>(v[2]), a3 = dpp_shr<D>(v[3]);
+    const int fu = dpp_shr_i<D>(f);
+    if (!f) {
+        v[0] = v[0] + a0; v[1] = v[1] + a1; v[2] = v[2] + a2; v[3] = v[3] + a3;
+    }
+    f |= fu;
+}
+
+// Segmented inclusive scan over the wave (all 64 lanes active): after the
+// call, the last lane of each run of equal keys holds the run's sums.  Any
+// key sequence is handled; cell-sorted particles give a few long runs, so
+// only a few lanes per wave touch memory.  Returns true on run-end lanes.
+__device__ __forceinline__ bool wave_segmented_sum(int key, double (&v)[4])
+{
+    const int lane = threadIdx.x & 63;
+    const int kprev = __shfl_up(key, 1);
+    const int knext = __shfl_down(key, 1);
+    int f = (lane == 0 || kprev != key) ? 1 : 0;
+    seg_step<1>(v, f);
+    seg_step<2>(v, f);
+    seg_step<4>(v, f);
+    seg_step<8>(v, f);
+#pragma unroll
+    for (int row = 1; row < 4; ++row) {
+        const int src = row * 16 - 1;
+        const double c0 = readlane_d(v[0], src), c1 = readlane_d(v[1], src);
+        const double c2 = readlane_d(v[2], src), c3 = readlane_d(v[3], src);
+        const int cf = __builtin_amdgcn_readlane(f, src);
+        if ((lane >> 4) == row && !f) {
+            v[0] = v[0] + c0; v[1] = v[1] + c1; v[2] = v[2] + c2; v[3] = v[3] + c3;
+        }
+        if ((lane >> 4) == row) f |= cf;
+    }
+    return lane == 63 || knext != key;
+}
+
 // ------------------------------------------------------- particle kernel
 // Persistent launch over every (species, tile) pair.  Tiles are contiguous
 // slot ranges; the tile range is split into 8 contiguous parts, one per XCD
@@ -229,57 +292,74 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
         const int zw0 = (MODE & M_DEPOSIT) ? anchor[0] : 0;
         const int rw0 = (MODE & M_DEPOSIT) ? anchor[1] : 0;
 
+        // Wave w of the block takes kItems consecutive runs of 64 slots, so a
+        // wave walks a contiguous, cell-sorted stretch of the tile.
+        const int wv = threadIdx.x >> 6;
 #pragma unroll 1
         for (int k = 0; k < kItems; ++k) {
-            const long long i = base + threadIdx.x + (long long)k * kBlock;
+            const long long i = base + (long long)(wv * kItems + k) * 64 + lane;
             bool valid = i < n_s;
             Part p;
             bool live = false, deleted = false;
             if (valid) {
-                p.Z = F[0][i];
+                // all fields are requested before the liveness test so that the
+                // eight loads are in flight together
+                p.Z = gp(F[0])[i];
+                if (MODE & (M_PUSH | M_ABSORB | M_DEPOSIT)) {
+                    p.R = gp(F[1])[i];
+                    p.WQ = gp(F[7])[i];
+                }
+                if (MODE & (M_PUSH | M_ABSORB)) {
+                    p.Vz = gp(F[2])[i]; p.Vr = gp(F[3])[i]; p.Vt = gp(F[4])[i];
+                    p.Az = gp(F[5])[i]; p.Ar = gp(F[6])[i];
+                }
                 live = !is_dead(p.Z);
             }
             if (live) {
-                if (MODE & (M_PUSH | M_ABSORB | M_DEPOSIT)) {
-                    p.R = F[1][i];
-                    p.WQ = F[7][i];
-                }
-                if (MODE & (M_PUSH | M_ABSORB)) {
-                    p.Vz = F[2][i]; p.Vr = F[3][i]; p.Vt = F[4][i];
-                    p.Az = F[5][i]; p.Ar = F[6][i];
-                }
                 if (MODE & M_PUSH) push_one(st, efactor, p);
                 if (MODE & M_ABSORB) deleted = absorb_one(st, charge, p);
             }
             const bool alive = live && !deleted;
 
-            if ((MODE & M_DEPOSIT) && alive) {
-                // P2CES, R/src/pic/field/FieldOne.f90:151-166
-                int NzL = (int)ceil(p.Z);
-                double ZL = (double)NzL - p.Z, ZU = 1.0 - ZL;
-                int NrL = (int)ceil(p.R);
-                double RL = (double)NrL - p.R, RU = 1.0 - RL;
-                NzL = min(max(NzL, 1), st.nz - 1);
-                const bool row0 = NrL < 1;                // R == 0 exactly: weight RL is 0
-                NrL = min(max(NrL, 1), st.nr - 1);
-                double wLL = ZL * RL * p.WQ, wLU = ZL * RU * p.WQ;
-                double wUL = ZU * RL * p.WQ, wUU = ZU * RU * p.WQ;
-                if (row0) {       // the reference's row 0 terms (weight 0) vanish; row 1 takes RU terms
-                    wLL = wLU; wUL = wUU; wLU = 0.0; wUU = 0.0;
+            if (MODE & M_DEPOSIT) {
+                // P2CES, R/src/pic/field/FieldOne.f90:151-166.  Contributions
+                // of a run of particles in one cell are summed across the wave
+                // first; the run's last lane adds them to the LDS window (or to
+                // HBM when the cell lies outside it).
+                int key = -1 - lane;
+                double w[4] = {0.0, 0.0, 0.0, 0.0};
+                int NzL = 1, NrL = 1;
+                if (alive) {
+                    NzL = (int)ceil(p.Z);
+                    double ZL = (double)NzL - p.Z, ZU = 1.0 - ZL;
+                    NrL = (int)ceil(p.R);
+                    double RL = (double)NrL - p.R, RU = 1.0 - RL;
+                    NzL = min(max(NzL, 1), st.nz - 1);
+                    const bool row0 = NrL < 1;            // R == 0 exactly: weight RL is 0
+                    NrL = min(max(NrL, 1), st.nr - 1);
+                    w[0] = ZL * RL * p.WQ; w[1] = ZL * RU * p.WQ;
+                    w[2] = ZU * RL * p.WQ; w[3] = ZU * RU * p.WQ;
+                    if (row0) {   // the reference's row-0 terms carry weight 0; row 1 takes the RU terms
+                        w[0] = w[1]; w[2] = w[3]; w[1] = 0.0; w[3] = 0.0;
+                    }
+                    key = (NzL - 1) + (NrL - 1) * (st.nz - 1);
                 }
-                const int wz = NzL - zw0, wr = NrL - rw0;
-                if (wz >= 0 && wz + 1 < kWinZ && wr >= 0 && wr + 1 < kWinR) {
-                    double *w = win + wz + wr * kWinZ;
-                    lds_add(w, wLL);
-                    lds_add(w + kWinZ, wLU);
-                    lds_add(w + 1, wUL);
-                    lds_add(w + 1 + kWinZ, wUU);
-                } else {
-                    double *g0 = raw + (size_t)(NzL - 1) + (size_t)(NrL - 1) * st.nz;
-                    unsafeAtomicAdd(g0, wLL);
-                    unsafeAtomicAdd(g0 + st.nz, wLU);
-                    unsafeAtomicAdd(g0 + 1, wUL);
-                    unsafeAtomicAdd(g0 + 1 + st.nz, wUU);
+                const bool run_end = wave_segmented_sum(key, w);
+                if (run_end && key >= 0) {
+                    const int wz = NzL - zw0, wr = NrL - rw0;
+                    if (wz >= 0 && wz + 1 < kWinZ && wr >= 0 && wr + 1 < kWinR) {
+                        double *wp = win + wz + wr * kWinZ;
+                        lds_add(wp, w[0]);
+                        lds_add(wp + kWinZ, w[1]);
+                        lds_add(wp + 1, w[2]);
+                        lds_add(wp + 1 + kWinZ, w[3]);
+                    } else {
+                        double *g0 = raw + (size_t)(NzL - 1) + (size_t)(NrL - 1) * st.nz;
+                        global_add(g0, w[0]);
+                        global_add(g0 + st.nz, w[1]);
+                        global_add(g0 + 1, w[2]);
+                        global_add(g0 + 1 + st.nz, w[3]);
+                    }
                 }
             }
 
@@ -290,8 +370,8 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
             bool selected = false;
             if ((MODE & M_MCC) && alive) {
                 if (!(MODE & (M_PUSH | M_ABSORB))) {
-                    p.R = F[1][i];
-                    p.Vz = F[2][i]; p.Vr = F[3][i]; p.Vt = F[4][i];
+                    if (!(MODE & M_DEPOSIT)) p.R = gp(F[1])[i];
+                    p.Vz = gp(F[2])[i]; p.Vr = gp(F[3])[i]; p.Vt = gp(F[4])[i];
                 }
                 const StateKey sk = state_key(p.Z, p.R, p.Vz, p.Vr, p.Vt);
                 for (int g = 0; g < st.ng; ++g) {
@@ -305,20 +385,26 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
                 const unsigned long long m = __ballot(selected);
                 if (m) {
                     const int leader = __ffsll((long long)m) - 1;
-                    unsigned long long qb = 0;
-                    if (lane == leader) qb = atomicAdd(&ctr->q_count, (unsigned long long)__popcll(m));
+                    // partition by tile and wave: consecutive tiles cycle through
+                    // every partition, so each fills evenly whatever the species mix
+                    const unsigned part = (unsigned)(t * (kBlock / 64) + wv) & (kQueueParts - 1);
+                    unsigned qb = 0;
+                    if (lane == leader)
+                        qb = atomicAdd(&st.q_part_count[part * kQueueStride], (unsigned)__popcll(m));
                     qb = __shfl(qb, leader);
                     if (selected) {
-                        const unsigned long long q = qb + __popcll(m & ((1ull << lane) - 1ull));
-                        if (q < (unsigned long long)st.q_cap) {
-                            if (!(MODE & (M_PUSH | M_ABSORB | M_DEPOSIT))) p.WQ = F[7][i];
+                        const unsigned long long qj = (unsigned long long)qb + __popcll(m & ((1ull << lane) - 1ull));
+                        const unsigned long long q = (unsigned long long)part * st.q_part_cap + qj;
+                        if (qj < (unsigned long long)st.q_part_cap) {
+                            if (!(MODE & (M_PUSH | M_ABSORB | M_DEPOSIT))) p.WQ = gp(F[7])[i];
                             if (!(MODE & (M_PUSH | M_ABSORB))) {
-                                p.Az = F[5][i]; p.Ar = F[6][i];
+                                p.Az = gp(F[5])[i]; p.Ar = gp(F[6])[i];
                             }
-                            st.q_f[0][q] = p.Z; st.q_f[1][q] = p.R; st.q_f[2][q] = p.Vz; st.q_f[3][q] = p.Vr;
-                            st.q_f[4][q] = p.Vt; st.q_f[5][q] = p.Az; st.q_f[6][q] = p.Ar; st.q_f[7][q] = p.WQ;
-                            st.q_slot[q] = i;
-                            st.q_species[q] = s;
+                            gp(st.q_f[0])[q] = p.Z; gp(st.q_f[1])[q] = p.R; gp(st.q_f[2])[q] = p.Vz;
+                            gp(st.q_f[3])[q] = p.Vr; gp(st.q_f[4])[q] = p.Vt; gp(st.q_f[5])[q] = p.Az;
+                            gp(st.q_f[6])[q] = p.Ar; gp(st.q_f[7])[q] = p.WQ;
+                            gp(st.q_slot)[q] = i;
+                            gp(st.q_species)[q] = s;
                         } else {
                             atomicOr(&ctr->overflow, 4);
                         }
@@ -332,13 +418,13 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
                     atomicAdd(&dead_blk[s], (unsigned long long)__popcll(m));
             }
             if (deleted) {
-                F[0][i] = kDead;
+                gp(F[0])[i] = kDead;
             } else if (alive) {
                 if (MODE & M_PUSH) {
-                    F[0][i] = p.Z; F[1][i] = p.R; F[2][i] = p.Vz; F[3][i] = p.Vr;
-                    F[4][i] = p.Vt; F[5][i] = p.Az; F[6][i] = p.Ar;
+                    gp(F[0])[i] = p.Z; gp(F[1])[i] = p.R; gp(F[2])[i] = p.Vz; gp(F[3])[i] = p.Vr;
+                    gp(F[4])[i] = p.Vt; gp(F[5])[i] = p.Az; gp(F[6])[i] = p.Ar;
                 } else if (MODE & M_ABSORB) {
-                    F[1][i] = p.R; F[3][i] = p.Vr; F[6][i] = p.Ar;
+                    gp(F[1])[i] = p.R; gp(F[3])[i] = p.Vr; gp(F[6])[i] = p.Ar;
                 }
             }
         }
@@ -349,7 +435,7 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
                 const double v = win[q];
                 if (v != 0.0) {
                     const int iz = zw0 + (q % kWinZ), ir = rw0 + (q / kWinZ);
-                    unsafeAtomicAdd(raw + (size_t)(iz - 1) + (size_t)(ir - 1) * st.nz, v);
+                    global_add(raw + (size_t)(iz - 1) + (size_t)(ir - 1) * st.nz, v);
                     win[q] = 0.0;
                 }
             }
@@ -376,12 +462,50 @@ __global__ void __launch_bounds__(kBlock)
 collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
 {
     const DevState &st = *stp;
-    const unsigned long long nq_all = ctr->q_count;
-    const unsigned long long nq = nq_all < (unsigned long long)st.q_cap ? nq_all : (unsigned long long)st.q_cap;
+    // exclusive scan of the partition fill levels, in LDS
+    __shared__ unsigned int pre[kQueueParts + 1];
+    __shared__ unsigned int wtot[kBlock / 64];
+    __shared__ int last_block;
+    {
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        constexpr int per = kQueueParts / kBlock;       // 2 partitions per thread
+        unsigned c[per];
+        unsigned tsum = 0;
+        for (int j = 0; j < per; ++j) {
+            const unsigned v = st.q_part_count[(threadIdx.x * per + j) * kQueueStride];
+            c[j] = v < (unsigned long long)st.q_part_cap ? v : (unsigned)st.q_part_cap;
+            tsum += c[j];
+        }
+        unsigned x = tsum;
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wtot[w] = x;
+        __syncthreads();
+        unsigned before = 0;
+        for (int k = 0; k < w; ++k) before += wtot[k];
+        unsigned run = before + x - tsum;
+        for (int j = 0; j < per; ++j) {
+            pre[threadIdx.x * per + j] = run;
+            run += c[j];
+        }
+        if (threadIdx.x == kBlock - 1) pre[kQueueParts] = run;
+        __syncthreads();
+    }
+    const unsigned long long nq = pre[kQueueParts];
     const uint32_t step = ctr->step;
     const unsigned long long seed = ctr->seed;
     const unsigned long long stride = (unsigned long long)gridDim.x * kBlock;
-    for (unsigned long long q = (unsigned long long)blockIdx.x * kBlock + threadIdx.x; q < nq; q += stride) {
+    for (unsigned long long e = (unsigned long long)blockIdx.x * kBlock + threadIdx.x; e < nq; e += stride) {
+        // partition holding flat entry e: last p with pre[p] <= e
+        int lo = 0, hi = kQueueParts;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (pre[mid] <= e) lo = mid;
+            else hi = mid;
+        }
+        const unsigned long long q = (unsigned long long)lo * st.q_part_cap + (e - pre[lo]);
         Part p;
         p.Z = st.q_f[0][q]; p.R = st.q_f[1][q]; p.Vz = st.q_f[2][q]; p.Vr = st.q_f[3][q];
         p.Vt = st.q_f[4][q]; p.Az = st.q_f[5][q]; p.Ar = st.q_f[6][q]; p.WQ = st.q_f[7][q];
@@ -399,13 +523,18 @@ collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
         F[2][i] = p.Vz; F[3][i] = p.Vr; F[4][i] = p.Vt; F[5][i] = p.Az; F[6][i] = p.Ar;
     }
     // The last block to finish folds this pass's products into n[], empties
-    // the queue and advances the random stream's step key.
+    // the queue partitions and advances the random stream's step key.
     __syncthreads();
     if (threadIdx.x == 0) {
         __threadfence();
-        unsigned prev = atomicAdd(&ctr->blocks_done, 1u);
-        if (prev == gridDim.x - 1) {
-            __threadfence();
+        const unsigned prev = atomicAdd(&ctr->blocks_done, 1u);
+        last_block = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (last_block) {
+        __threadfence();
+        for (int p = threadIdx.x; p < kQueueParts; p += kBlock) st.q_part_count[p * kQueueStride] = 0;
+        if (threadIdx.x == 0) {
             for (int s = 0; s < st.ns; ++s) {
                 unsigned long long add = atomicAdd(&ctr->n_new[s], 0ull);
                 unsigned long long n0 = ctr->n[s];
@@ -413,11 +542,10 @@ collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
                 ctr->n[s] = (n0 + add > cap) ? cap : n0 + add;
                 ctr->n_new[s] = 0;
             }
-            ctr->q_count = 0;
             ctr->step = ctr->step + 1;
             ctr->blocks_done = 0;
-            __threadfence();
         }
+        __threadfence();
     }
 }
 
@@ -643,6 +771,7 @@ struct ipm_ctx {
     long long *q_slot = nullptr;
     int *q_species = nullptr;
     long long q_cap = 0;
+    unsigned *q_part_count = nullptr;
     hipGraphExec_t graph = nullptr;
     int block = kBlock, blocks_per_cu = 4;
     // optional HIP-event timing of the kernels ipm_step enqueues
@@ -679,7 +808,9 @@ static int ensure_queue(ipm_ctx *c)
     if (c->q_slot) HIPCHK(hipFree(c->q_slot));
     if (c->q_species) HIPCHK(hipFree(c->q_species));
     c->qbuf = nullptr; c->q_slot = nullptr; c->q_species = nullptr; c->q_cap = 0;
-    const size_t cap = ((size_t)need + 31) & ~(size_t)31;
+    // every partition gets an equal share plus a margin for uneven filling
+    const size_t part_cap = (((size_t)need / kQueueParts) * 5 / 4 + 1024 + 31) & ~(size_t)31;
+    const size_t cap = part_cap * kQueueParts;
     HIPCHK(hipMalloc(&c->qbuf, sizeof(double) * 8 * cap));
     HIPCHK(hipMalloc(&c->q_slot, sizeof(long long) * cap));
     HIPCHK(hipMalloc(&c->q_species, sizeof(int) * cap));
@@ -688,6 +819,12 @@ static int ensure_queue(ipm_ctx *c)
     c->h.q_slot = c->q_slot;
     c->h.q_species = c->q_species;
     c->h.q_cap = c->q_cap;
+    c->h.q_part_cap = (long long)part_cap;
+    if (!c->q_part_count) {
+        HIPCHK(hipMalloc(&c->q_part_count, sizeof(unsigned) * kQueueParts * kQueueStride));
+        HIPCHK(hipMemset(c->q_part_count, 0, sizeof(unsigned) * kQueueParts * kQueueStride));
+    }
+    c->h.q_part_count = c->q_part_count;
     if (c->graph) {
         (void)hipGraphExecDestroy(c->graph);
         c->graph = nullptr;
@@ -828,7 +965,8 @@ int ipm_destroy(ipm_ctx *c)
     void *ptrs[] = {c->d_state, c->d_ctr, c->raw_all, c->rho_one, c->chi_one, c->rho_tot, c->chi_tot,
                     c->d_chifac, c->inv_vol, c->cell_material, c->hedge, c->vedge, c->mat_type,
                     c->metal_charge, c->sigma, c->Ez, c->Er, c->mcc_counts, c->hist, c->offs,
-                    c->cursor, c->partial, c->partial2, c->d_dst, c->qbuf, c->q_slot, c->q_species};
+                    c->cursor, c->partial, c->partial2, c->d_dst, c->qbuf, c->q_slot, c->q_species,
+                    c->q_part_count};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -82,7 +82,7 @@ def cpu_baseline(cfg, geom, tables, particles, seconds):
     species = [W.species_params(cfg, s) for s in range(cfg.ns)]
     setup = O.MccSetup(cfg.dx, cfg.dt, [q for q, _ in species], [m for _, m in species], tables)
     ez, er = W.fields(cfg, 0.0)
-    st = O.StepState(geom, setup, sample, ez, er, fast=True)
+    st = O.StepState(geom, setup, sample, ez, er, extra=1.0, fast=True)
     psteps = 0
     nsteps = 0
     t0 = time.perf_counter()
@@ -92,7 +92,7 @@ def cpu_baseline(cfg, geom, tables, particles, seconds):
         psteps += n
         nsteps += 1
         el = time.perf_counter() - t0
-        if el >= seconds or nsteps >= 200:
+        if el >= seconds or nsteps >= 100:
             break
     return dict(value=psteps / el, unit="particle-steps/s", cores=1, kind="port",
                 sample=f"every {stride}th particle of the same workload ({sum(P.shape[1] for P in sample)} "
