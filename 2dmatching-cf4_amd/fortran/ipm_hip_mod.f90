@@ -1,0 +1,257 @@
+! Fortran binding of the gfx950 particle hot path (include/ipm_hip.h).
+!
+! The explicit bind(C) interfaces replace the reference's per-step particle
+! calls; the wrappers at the bottom keep the reference's names and its error
+! behaviour (print and stop, R/src/pic/particle/ParticleBundle.f90:587-590):
+!
+!   call PB(k)%MoveES(FG)                     -> call ipm_MoveES(ctx, k)
+!   call MaterialAborptionParticle(..., PB(k)) -> call ipm_MaterialAborptionParticle(ctx, k, MT charge, sigma)
+!   call FO(k)%P2CES(PB(k), Geom)             -> call ipm_P2CES(ctx, k, FO(k)%RhoOne, FO(k)%ChiOne)
+!   call FT%Sum(FO)                           -> call ipm_FieldSourceSum(ctx, FT%Rho, FT%Chi)
+!   call MCC(Ns, Ng, PB, SO, GO, MCCB)        -> call ipm_MCCStep(ctx)
+!
+! Species indices follow the reference (0 = electrons .. Ns); grid arrays are
+! passed with the reference's own bounds and ghost layers (column-major).
+module ipm_hip
+    use iso_c_binding
+    implicit none
+
+    integer(c_int), parameter :: IPM_OK = 0, IPM_EINVAL = -1, IPM_EHIP = -2, IPM_ECAPACITY = -3, IPM_ENODEV = -4
+    integer(c_int), parameter :: IPM_LAYOUT_AOS = 0, IPM_LAYOUT_SOA = 1
+
+    interface
+        integer(c_int) function ipm_create(ctx, nz, nr, dx, dt, ns, ng) bind(C, name="ipm_create")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), intent(out) :: ctx
+            integer(c_int), value :: nz, nr, ns, ng
+            real(c_double), value :: dx, dt
+        end function
+        integer(c_int) function ipm_destroy(ctx) bind(C, name="ipm_destroy")
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+        end function
+        type(c_ptr) function ipm_last_error() bind(C, name="ipm_last_error")
+            import :: c_ptr
+        end function
+        integer(c_int) function ipm_set_species(ctx, s, charge, mass, capacity) bind(C, name="ipm_set_species")
+            import :: c_ptr, c_int, c_double, c_int64_t
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: s
+            real(c_double), value :: charge, mass
+            integer(c_int64_t), value :: capacity
+        end function
+        integer(c_int) function ipm_set_geometry(ctx, inv_volume, cell_material, hedge_type, vedge_type, &
+                                                 ndiel, nmetal, material_type) bind(C, name="ipm_set_geometry")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(in) :: inv_volume(*)
+            integer(c_int), intent(in) :: cell_material(*), hedge_type(*), vedge_type(*), material_type(*)
+            integer(c_int), value :: ndiel, nmetal
+        end function
+        integer(c_int) function ipm_upload_species(ctx, s, n, po, layout) bind(C, name="ipm_upload_species")
+            import :: c_ptr, c_int, c_double, c_int64_t
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: s, layout
+            integer(c_int64_t), value :: n
+            real(c_double), intent(in) :: po(*)
+        end function
+        integer(c_int) function ipm_download_species(ctx, s, n, po, cap, layout) bind(C, name="ipm_download_species")
+            import :: c_ptr, c_int, c_double, c_int64_t
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: s, layout
+            integer(c_int64_t), intent(out) :: n
+            real(c_double), intent(inout) :: po(*)
+            integer(c_int64_t), value :: cap
+        end function
+        integer(c_int) function ipm_count(ctx, s, n) bind(C, name="ipm_count")
+            import :: c_ptr, c_int, c_int64_t
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: s
+            integer(c_int64_t), intent(out) :: n
+        end function
+        integer(c_int) function ipm_set_fields(ctx, Ez, Er) bind(C, name="ipm_set_fields")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(in) :: Ez(*), Er(*)
+        end function
+        integer(c_int) function ipm_move_es(ctx, s) bind(C, name="ipm_move_es")
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: s
+        end function
+        integer(c_int) function ipm_absorb(ctx, s) bind(C, name="ipm_absorb")
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: s
+        end function
+        integer(c_int) function ipm_move_absorb(ctx, s) bind(C, name="ipm_move_absorb")
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: s
+        end function
+        integer(c_int) function ipm_take_tallies(ctx, metal_charge, sigma) bind(C, name="ipm_take_tallies")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(inout) :: metal_charge(*), sigma(*)
+        end function
+        integer(c_int) function ipm_p2c(ctx, s, rho_one, chi_one) bind(C, name="ipm_p2c")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: s
+            real(c_double), intent(inout) :: rho_one(*), chi_one(*)
+        end function
+        integer(c_int) function ipm_field_sum(ctx, rho, chi) bind(C, name="ipm_field_sum")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(inout) :: rho(*), chi(*)
+        end function
+        integer(c_int) function ipm_set_mcc(ctx, s, g, model, nreaction, nsigma, emin, einterval, emax, &
+                                            collision_ratio, rtype, resultant, threshold, prob, gas_mass, &
+                                            gas_temperature, beta_max) bind(C, name="ipm_set_mcc")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: s, g, model, nreaction, nsigma
+            real(c_double), value :: emin, einterval, emax, collision_ratio, gas_mass, gas_temperature, beta_max
+            integer(c_int), intent(in) :: rtype(*), resultant(*)
+            real(c_double), intent(in) :: threshold(*), prob(*)
+        end function
+        integer(c_int) function ipm_set_rng(ctx, seed, step) bind(C, name="ipm_set_rng")
+            import :: c_ptr, c_int, c_int64_t, c_int32_t
+            type(c_ptr), value :: ctx
+            integer(c_int64_t), value :: seed
+            integer(c_int32_t), value :: step
+        end function
+        integer(c_int) function ipm_mcc(ctx) bind(C, name="ipm_mcc")
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function ipm_set_event_counting(ctx, on) bind(C, name="ipm_set_event_counting")
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: on
+        end function
+        integer(c_int) function ipm_take_mcc_counts(ctx, counts) bind(C, name="ipm_take_mcc_counts")
+            import :: c_ptr, c_int, c_int64_t
+            type(c_ptr), value :: ctx
+            integer(c_int64_t), intent(inout) :: counts(*)
+        end function
+        integer(c_int) function ipm_step(ctx) bind(C, name="ipm_step")
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function ipm_step_graph(ctx) bind(C, name="ipm_step_graph")
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function ipm_set_timing(ctx, on) bind(C, name="ipm_set_timing")
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: on
+        end function
+        integer(c_int) function ipm_take_timing(ctx, ms, launches) bind(C, name="ipm_take_timing")
+            import :: c_ptr, c_int, c_double, c_int64_t
+            type(c_ptr), value :: ctx
+            real(c_double), intent(inout) :: ms(*)
+            integer(c_int64_t), intent(inout) :: launches(*)
+        end function
+        integer(c_int) function ipm_sort(ctx, s) bind(C, name="ipm_sort")
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: s
+        end function
+        integer(c_int) function ipm_set_launch(ctx, block, blocks_per_cu) bind(C, name="ipm_set_launch")
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: block, blocks_per_cu
+        end function
+        integer(c_int) function ipm_synchronize(ctx) bind(C, name="ipm_synchronize")
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function ipm_species_device_ptrs(ctx, s, fields8) bind(C, name="ipm_species_device_ptrs")
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: s
+            type(c_ptr), intent(inout) :: fields8(8)
+        end function
+        integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
+            import :: c_ptr, c_size_t
+            type(c_ptr), value :: s
+        end function
+    end interface
+
+contains
+
+    ! Text of the library's last error.
+    function ipm_error_text() result(txt)
+        character(len=:), allocatable :: txt
+        character(kind=c_char), pointer :: chars(:)
+        type(c_ptr) :: p
+        integer :: n, i
+        p = ipm_last_error()
+        n = int(c_strlen(p))
+        call c_f_pointer(p, chars, [n])
+        allocate(character(len=n) :: txt)
+        do i = 1, n
+            txt(i:i) = chars(i)
+        end do
+    end function
+
+    ! The reference prints and stops on every hot-path error; so does this.
+    subroutine ipm_check(rc, where)
+        integer(c_int), intent(in) :: rc
+        character(*), intent(in) :: where
+        if (rc /= IPM_OK) then
+            write(*, '(a, a, a, i0, a, a)') "ipm: ", where, " failed (", rc, "): ", ipm_error_text()
+            stop 1
+        end if
+    end subroutine
+
+    ! PB(k)%MoveES(FG): R/src/pic/particle/ParticleBundle.f90:649
+    subroutine ipm_MoveES(ctx, k)
+        type(c_ptr), intent(in) :: ctx
+        integer, intent(in) :: k
+        call ipm_check(ipm_move_es(ctx, int(k, c_int)), "MoveES")
+    end subroutine
+
+    ! MaterialAborptionParticle(CF, Geom, MT, PB(k)):
+    ! R/src/pic/particleboundary/ParticleMaterial.F90:10.  charge_one_step(1:nmetal)
+    ! and sigma_one_step(1:Nz, 1:Nr) are accumulated, as the reference does.
+    subroutine ipm_MaterialAborptionParticle(ctx, k, charge_one_step, sigma_one_step)
+        type(c_ptr), intent(in) :: ctx
+        integer, intent(in) :: k
+        real(c_double), intent(inout) :: charge_one_step(:), sigma_one_step(:, :)
+        call ipm_check(ipm_absorb(ctx, int(k, c_int)), "MaterialAborptionParticle")
+        call ipm_check(ipm_take_tallies(ctx, charge_one_step, sigma_one_step), "MaterialAborptionParticle")
+    end subroutine
+
+    ! FO(k)%P2CES(PB(k), Geom): R/src/pic/field/FieldOne.f90:132
+    subroutine ipm_P2CES(ctx, k, RhoOne, ChiOne)
+        type(c_ptr), intent(in) :: ctx
+        integer, intent(in) :: k
+        real(c_double), intent(inout) :: RhoOne(:, :), ChiOne(:, :)
+        call ipm_check(ipm_p2c(ctx, int(k, c_int), RhoOne, ChiOne), "P2CES")
+    end subroutine
+
+    ! FT%Sum(FO): R/src/pic/field/FieldSource.f90:72.  Rho(1:Nz, 1:Nr),
+    ! Chi(0:Nz+1, 0:Nr+1) as the reference allocates them.
+    subroutine ipm_FieldSourceSum(ctx, Rho, Chi)
+        type(c_ptr), intent(in) :: ctx
+        real(c_double), intent(inout) :: Rho(:, :), Chi(:, :)
+        call ipm_check(ipm_field_sum(ctx, Rho, Chi), "FieldSource%Sum")
+    end subroutine
+
+    ! MCC(Ns, Ng, PB, SO, GO, MCCB): R/src/mc/MCCPublic.F90:38
+    subroutine ipm_MCCStep(ctx)
+        type(c_ptr), intent(in) :: ctx
+        call ipm_check(ipm_mcc(ctx), "MCC")
+    end subroutine
+
+    ! PB(k)%NPar
+    integer(c_int64_t) function ipm_NPar(ctx, k)
+        type(c_ptr), intent(in) :: ctx
+        integer, intent(in) :: k
+        call ipm_check(ipm_count(ctx, int(k, c_int), ipm_NPar), "NPar")
+    end function
+
+end module ipm_hip

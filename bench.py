@@ -110,6 +110,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the graph and plain-launch comparisons")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="for rocprofv3 --pmc runs: a 1 GiB device copy (counter calibration), then 4 steps")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -120,6 +122,17 @@ def main():
     import numpy as np
     import ipm2d
     from ipm2d import workload as W
+
+    if args.traffic_probe:
+        import torch
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+        a.fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)                      # reads and writes exactly 2**30 bytes
+        torch.cuda.synchronize()
+        del a, b
+        args.steps, args.warmup, args.no_cpu_baseline, args.no_extra = 4, 0, True, True
 
     cfg = W.make_config(args.config)
     sort_interval = cfg.sort_interval if args.sort_interval < 0 else args.sort_interval
