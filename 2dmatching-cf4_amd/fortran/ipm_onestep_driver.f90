@@ -32,7 +32,7 @@ program ipm_onestep_driver
     real(8), allocatable :: vfac(:, :), vol(:, :), inv_vol(:, :)
     real(8), allocatable :: PO(:, :), phi(:, :), Ez(:, :), Er(:, :)
     real(8), allocatable :: RhoOne(:, :), ChiOne(:, :), Rho(:, :), Chi(:, :), sigma(:, :)
-    real(8) :: metal_charge(2), wq, r, z, u1, u2, vth, vf, q_particles, q_grid
+    real(8) :: metal_charge(2), wq, r, z, u1, u2, vth, vf, q_particles, q_grid, q_abs
     integer(c_int64_t) :: nget, npar
     integer(c_int) :: rt(1), res(1)
     real(8) :: th(1), emin, eint, energy, v, pmax, ngas, sigma0, ratio
@@ -206,17 +206,19 @@ program ipm_onestep_driver
         ! all of it: no particle sits in the zero-volume end layers)
         q_grid = sum(Rho * vol)
         q_particles = 0.d0
+        q_abs = 0.d0
         do s = 0, Ns - 1
             npar = ipm_NPar(ctx, s)
             allocate(PO(8, max(npar, 1_c_int64_t)))
             call ipm_check(ipm_download_species(ctx, s, nget, PO, max(npar, 1_c_int64_t), IPM_LAYOUT_AOS), "download")
             q_particles = q_particles + charge(s) * sum(PO(8, 1:nget))
+            q_abs = q_abs + abs(charge(s)) * sum(PO(8, 1:nget))
             deallocate(PO)
         end do
         call ipm_MCCStep(ctx)
-        write(*, '(a, i3, a, i8, a, i8, a, es14.6, a, es14.6, a, 2es12.4)') "step", it, &
+        write(*, '(a, i3, a, i8, a, i8, a, es24.16, a, es24.16, a, es24.16, a, 2es12.4)') "step", it, &
             " NPar(e)", ipm_NPar(ctx, 0), " NPar(CF3+)", ipm_NPar(ctx, 1), &
-            " q_grid", q_grid, " q_particles", q_particles, " metal", metal_charge
+            " q_grid", q_grid, " q_particles", q_particles, " q_abs", q_abs, " metal", metal_charge
     end do
     call ipm_check(ipm_destroy(ctx), "destroy")
     write(*, '(a)') "driver done"

@@ -44,4 +44,5 @@ def test_fortran_driver_onestep(oracle, tmp_path):
     for l in steps:
         qg = float(re.search(r"q_grid\s+(\S+)", l).group(1))
         qp = float(re.search(r"q_particles\s+(\S+)", l).group(1))
-        assert abs(qg - qp) <= 1e-12 * abs(qp) + 1e-30, l
+        qa = float(re.search(r"q_abs\s+(\S+)", l).group(1))
+        assert qa > 0 and abs(qg - qp) <= 1e-12 * qa, l
