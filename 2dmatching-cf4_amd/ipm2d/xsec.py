@@ -129,13 +129,13 @@ def write_sigma_file(model, reactions):
     if model == 3:
         lines.append("SR%Reaction= ")
         for r in reactions:
-            lines.append(f"{r.reactant},{r.rtype},{r.resultant},{r.threshold!r}")
+            lines.append(f"{r.reactant},{r.rtype},{r.resultant},{float(r.threshold)!r}")
     else:
         for k, r in enumerate(reactions, 1):
-            lines.append(f"SR({k})%Reaction = {r.reactant},{r.rtype},{r.resultant},{r.threshold!r},")
+            lines.append(f"SR({k})%Reaction = {r.reactant},{r.rtype},{r.resultant},{float(r.threshold)!r},")
             lines.append(f"SR({k})%EnergySigma =")
             for e, s in zip(r.energy, r.sigma):
-                lines.append(f" {e!r}\t{s!r}")
+                lines.append(f" {float(e)!r}\t{float(s)!r}")
     lines.append("/")
     return "\n".join(lines) + "\n"
 
