@@ -295,26 +295,35 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
         // Wave w of the block takes kItems consecutive runs of 64 slots, so a
         // wave walks a contiguous, cell-sorted stretch of the tile.
         const int wv = threadIdx.x >> 6;
+        // Software pipeline: the loads of item k+1 are issued before item k
+        // is processed, so two sets of eight loads are in flight per wave.
+        auto load_item = [&](long long i, Part &q) {
+            q.Z = gp(F[0])[i];
+            if (MODE & (M_PUSH | M_ABSORB | M_DEPOSIT)) {
+                q.R = gp(F[1])[i];
+                q.WQ = gp(F[7])[i];
+            }
+            if (MODE & (M_PUSH | M_ABSORB)) {
+                q.Vz = gp(F[2])[i]; q.Vr = gp(F[3])[i]; q.Vt = gp(F[4])[i];
+                q.Az = gp(F[5])[i]; q.Ar = gp(F[6])[i];
+            }
+        };
+        Part nxt;
+        {
+            const long long i0 = base + (long long)(wv * kItems) * 64 + lane;
+            if (i0 < n_s) load_item(i0, nxt);
+        }
 #pragma unroll 1
         for (int k = 0; k < kItems; ++k) {
             const long long i = base + (long long)(wv * kItems + k) * 64 + lane;
-            bool valid = i < n_s;
-            Part p;
-            bool live = false, deleted = false;
-            if (valid) {
-                // all fields are requested before the liveness test so that the
-                // eight loads are in flight together
-                p.Z = gp(F[0])[i];
-                if (MODE & (M_PUSH | M_ABSORB | M_DEPOSIT)) {
-                    p.R = gp(F[1])[i];
-                    p.WQ = gp(F[7])[i];
-                }
-                if (MODE & (M_PUSH | M_ABSORB)) {
-                    p.Vz = gp(F[2])[i]; p.Vr = gp(F[3])[i]; p.Vt = gp(F[4])[i];
-                    p.Az = gp(F[5])[i]; p.Ar = gp(F[6])[i];
-                }
-                live = !is_dead(p.Z);
+            const bool valid = i < n_s;
+            Part p = nxt;
+            if (k + 1 < kItems) {
+                const long long i1 = i + 64;
+                if (i1 < n_s) load_item(i1, nxt);
             }
+            bool live = false, deleted = false;
+            if (valid) live = !is_dead(p.Z);
             if (live) {
                 if (MODE & M_PUSH) push_one(st, efactor, p);
                 if (MODE & M_ABSORB) deleted = absorb_one(st, charge, p);
@@ -773,7 +782,7 @@ struct ipm_ctx {
     long long q_cap = 0;
     unsigned *q_part_count = nullptr;
     hipGraphExec_t graph = nullptr;
-    int block = kBlock, blocks_per_cu = 4;
+    int block = kBlock, blocks_per_cu = 0;
     // optional HIP-event timing of the kernels ipm_step enqueues
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -853,10 +862,20 @@ static int check_ready(ipm_ctx *c, bool need_fields, bool need_geom)
     return IPM_OK;
 }
 
+// blocks_per_cu > 0: persistent grid of that many blocks per CU looping over
+// tiles.  blocks_per_cu == 0: flat grid of one block per tile, sized from the
+// species capacities so that a captured graph covers any population; blocks
+// past the live tiles return at once.
 static int grid_blocks(ipm_ctx *c)
 {
-    int g = c->num_cu * c->blocks_per_cu;
-    return g < 8 ? 8 : g;
+    if (c->blocks_per_cu > 0) {
+        int g = c->num_cu * c->blocks_per_cu;
+        return g < 8 ? 8 : g;
+    }
+    long long tmax = 0;
+    for (int s = 0; s < c->ns; ++s) tmax += (c->h.cap[s] + kTile - 1) / kTile;
+    tmax = ((tmax + 7) / 8) * 8;
+    return (int)std::max<long long>(8, std::min<long long>(tmax, 0x7fffffffLL));
 }
 
 template <int MODE>
@@ -1434,7 +1453,7 @@ int ipm_set_launch(ipm_ctx *c, int block, int blocks_per_cu)
     if (!c) return fail(IPM_EINVAL, "null context");
     if (block != 0 && block != kBlock) return fail(IPM_EINVAL, "block size is fixed at %d", kBlock);
     if (blocks_per_cu < 0 || blocks_per_cu > 64) return fail(IPM_EINVAL, "invalid blocks_per_cu %d", blocks_per_cu);
-    c->blocks_per_cu = blocks_per_cu == 0 ? 4 : blocks_per_cu;
+    c->blocks_per_cu = blocks_per_cu;
     if (c->graph) {
         (void)hipGraphExecDestroy(c->graph);
         c->graph = nullptr;

@@ -1,0 +1,91 @@
+"""Online autotuner for the batched step: launch shape and cell-sort interval.
+
+The tuned knobs change only how the work is scheduled, never what is
+computed (the GPU parity tests show the step is invariant to the launch
+shape, to sorting and to storage order), so the tuner measures them on the
+live simulation: every candidate advances the real state by a few steps.
+
+* ``blocks_per_cu``: 0 for one block per tile, else persistent blocks per
+  compute unit of the fused particle kernel; chosen by timing a few steps each.
+* ``sort_interval``: steps between cell sorts.  A sort costs about one
+  particle pass but keeps the deposit's LDS window and wave reduction
+  effective; the interval minimising the mean step time over a full
+  interval (sort included) is kept.
+
+Results can be cached in a JSON file keyed by grid, species count and
+particle count (rounded), so a production run tunes once.
+"""
+import json
+import os
+import time
+
+
+def _timed(hp, steps, sort_interval=0, offset=0):
+    hp.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        hp.one_step()
+        if sort_interval and (offset + k + 1) % sort_interval == 0:
+            hp.sort(-1)
+    hp.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def tune_blocks_per_cu(hp, candidates=(0, 2, 4, 6, 8), steps=6, timer=None):
+    """Return (best, {candidate: seconds per step})."""
+    timer = timer or (lambda: _timed(hp, steps))
+    res = {}
+    hp.sort(-1)
+    for b in candidates:
+        hp.set_launch(b)
+        hp.one_step()                     # first launch at a new shape is not timed
+        res[b] = timer()
+    best = min(res, key=res.get)
+    hp.set_launch(best)
+    return best, res
+
+
+def tune_sort_interval(hp, candidates=(5, 10, 20, 40), timer=None):
+    """Mean step time over one full interval (its sort included) per candidate;
+    returns (best, {interval: seconds per step})."""
+    res = {}
+    for si in candidates:
+        hp.sort(-1)
+        if timer is not None:
+            res[si] = timer(si)
+        else:
+            res[si] = _timed(hp, si, sort_interval=si)
+    best = min(res, key=res.get)
+    return best, res
+
+
+def cache_key(hp, nparticles):
+    return f"{hp.nz}x{hp.nr}_ns{hp.ns}_n{int(round(nparticles, -5))}"
+
+
+def autotune(hp, nparticles, cache_path=None, bpc_candidates=(0, 2, 4, 6, 8),
+             sort_candidates=(5, 10, 20, 40), steps=6):
+    """Tune (or load from cache) and apply; returns the chosen settings."""
+    key = cache_key(hp, nparticles)
+    if cache_path and os.path.exists(cache_path):
+        with open(cache_path) as f:
+            cache = json.load(f)
+        if key in cache:
+            hp.set_launch(cache[key]["blocks_per_cu"])
+            return cache[key]
+    bpc, tb = tune_blocks_per_cu(hp, bpc_candidates, steps)
+    si, ts = tune_sort_interval(hp, sort_candidates)
+    out = {"blocks_per_cu": bpc, "sort_interval": si,
+           "step_seconds_by_blocks_per_cu": {str(k): v for k, v in tb.items()},
+           "step_seconds_by_sort_interval": {str(k): v for k, v in ts.items()}}
+    if cache_path:
+        cache = {}
+        if os.path.exists(cache_path):
+            with open(cache_path) as f:
+                cache = json.load(f)
+        cache[key] = out
+        tmp = cache_path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(cache, f, indent=1)
+        os.replace(tmp, cache_path)
+    return out

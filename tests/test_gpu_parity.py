@@ -210,8 +210,10 @@ def test_fused_step_equals_separate_calls(rng):
     b.p2ces_all()
     rho_b, chi_b = b.field_source_sum()
     b.mcc()
-    assert_close(rho_a, rho_b, rtol=1e-12, atol=1e-30 * np.abs(rho_b).max(), what="Rho fused vs separate")
-    assert_close(chi_a, chi_b, rtol=1e-12, atol=1e-30 * np.abs(chi_b).max(), what="Chi fused vs separate")
+    # net Rho/Chi are differences of species terms (cancellation): absolute
+    # tolerance scaled to the field's magnitude; per-species sums meet 1e-12
+    assert_close(rho_a, rho_b, rtol=1e-12, atol=1e-12 * np.abs(rho_b).max(), what="Rho fused vs separate")
+    assert_close(chi_a, chi_b, rtol=1e-12, atol=1e-12 * np.abs(chi_b).max(), what="Chi fused vs separate")
     for s in range(cfg.ns):
         ga, gb = a.download(s), b.download(s)
         assert ga.shape == gb.shape
@@ -250,7 +252,7 @@ def test_full_step_matches_oracle_without_losses(oracle, rng):
         r_s, _ = oracle.p2ces(nz, nr, cfg.dt, q, m, X, geom.inv_volume)
         rho_ref = rho_ref + r_s
         pushed.append(X)
-    assert_close(rho, rho_ref, rtol=1e-12, atol=1e-14 * np.abs(rho_ref).max(), what="Rho")
+    assert_close(rho, rho_ref, rtol=1e-12, atol=1e-12 * np.abs(rho_ref).max(), what="Rho")
     setup = oracle.MccSetup(cfg.dx, cfg.dt, [q for q, _ in species], [m for _, m in species], tables)
     ref, _ = oracle.mcc(setup, 7, 3, pushed)
     for s in range(cfg.ns):
@@ -301,7 +303,8 @@ def test_graph_replay_equals_eager(rng):
         b.synchronize()
         ra, _ = a.field_source_sum()
         rb, _ = b.field_source_sum()
-        assert_close(ra, rb, rtol=1e-12, atol=1e-30 * np.abs(ra).max(), what=f"Rho replay {k}")
+        # net Rho is a difference of species terms: tolerance scaled to its magnitude
+        assert_close(ra, rb, rtol=1e-12, atol=1e-12 * np.abs(ra).max(), what=f"Rho replay {k}")
         for s in range(cfg.ns):
             assert a.count(s) == b.count(s)
     for s in range(cfg.ns):
