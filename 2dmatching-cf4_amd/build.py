@@ -40,9 +40,9 @@ def run(cmd, cwd=None):
 
 def build_hip():
     os.makedirs(LIBDIR, exist_ok=True)
-    src = os.path.join(CSRC, "ipm_hip.hip")
+    srcs = [os.path.join(CSRC, f) for f in ("ipm_hip.hip", "ipm_window.hip")]
     tmp = LIB + ".tmp"
-    run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC, src, "-o", tmp])
+    run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC, *srcs, "-o", tmp])
     os.replace(tmp, LIB)
     return LIB
 

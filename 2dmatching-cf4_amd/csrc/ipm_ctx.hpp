@@ -1,0 +1,106 @@
+// Host-side context of the particle hot path, shared by the translation
+// units that implement the C ABI of include/ipm_hip.h.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <utility>
+#include <vector>
+#include "ipm_device.hpp"
+
+using namespace ipm;
+
+struct ipm_window;
+// Frees the differentiable-window buffers of a context (ipm_window.hip).
+int window_release(struct ipm_ctx *c);
+
+inline thread_local char g_err[512] = "";
+
+inline int fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(x)                                                                      \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess)                                                          \
+            return fail(IPM_EHIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_),  \
+                        __FILE__, __LINE__);                                           \
+    } while (0)
+
+struct ipm_ctx {
+    int device = 0;
+    int num_cu = 256;
+    hipStream_t stream = nullptr;
+    int nz = 0, nr = 0, ns = 0, ng = 0;
+    double dx = 0, dt = 0;
+    DevState h{};
+    DevState *d_state = nullptr;
+    DevCounters *d_ctr = nullptr;
+    double *buf[2][IPM_NS_MAX] = {};
+    int cur[IPM_NS_MAX] = {};
+    bool species_set[IPM_NS_MAX] = {};
+    double chifac[IPM_NS_MAX] = {};
+    double *d_chifac = nullptr;
+    double *raw_all = nullptr, *rho_one = nullptr, *chi_one = nullptr;
+    double *rho_tot = nullptr, *chi_tot = nullptr;
+    double *inv_vol = nullptr;
+    int *cell_material = nullptr, *hedge = nullptr, *vedge = nullptr, *mat_type = nullptr;
+    double *metal_charge = nullptr, *sigma = nullptr;
+    double *Ez = nullptr, *Er = nullptr;
+    unsigned long long *mcc_counts = nullptr;
+    std::vector<void *> tab_mem[IPM_NS_MAX * IPM_NG_MAX];
+    unsigned long long *hist = nullptr, *offs = nullptr, *cursor = nullptr, *partial = nullptr,
+                       *partial2 = nullptr;
+    double **d_dst = nullptr;
+    double *qbuf = nullptr;
+    long long *q_slot = nullptr;
+    int *q_species = nullptr;
+    long long q_cap = 0;
+    unsigned *q_part_count = nullptr;
+    double *mom = nullptr;                 // moment sums, 5 * nz * nr (ipm_moments)
+    hipGraphExec_t graph = nullptr;
+    int block = 256, blocks_per_cu = 0;
+    // optional HIP-event timing of the kernels ipm_step enqueues
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, int>> ev_pairs[3];   // (start, stop) per kernel class
+    size_t ev_used = 0;
+    bool geometry_set = false, fields_set = false;
+    // differentiable-window state (ipm_window_* entry points), or null
+    ipm_window *win = nullptr;
+};
+
+inline int sync_state(ipm_ctx *c)
+{
+    HIPCHK(hipMemcpyAsync(c->d_state, &c->h, sizeof(DevState), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return IPM_OK;
+}
+
+
+inline int read_counters(ipm_ctx *c, DevCounters *out)
+{
+    HIPCHK(hipMemcpyAsync(out, c->d_ctr, sizeof(DevCounters), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (out->overflow & 1)
+        return fail(IPM_ECAPACITY, "The size of particle bundle reaches the maximum (MCC products dropped).");
+    if (out->overflow & 2) return fail(IPM_EINVAL, "MCC product with an invalid species index.");
+    if (out->overflow & 4) return fail(IPM_ECAPACITY, "collision queue overflow (selections were not collided)");
+    return IPM_OK;
+}
+
+inline int check_ready(ipm_ctx *c, bool need_fields, bool need_geom)
+{
+    if (!c) return fail(IPM_EINVAL, "null context");
+    for (int s = 0; s < c->ns; ++s)
+        if (!c->species_set[s]) return fail(IPM_EINVAL, "species %d has not been set", s);
+    if (need_fields && !c->fields_set) return fail(IPM_EINVAL, "fields have not been set");
+    if (need_geom && !c->geometry_set) return fail(IPM_EINVAL, "geometry has not been set");
+    return IPM_OK;
+}

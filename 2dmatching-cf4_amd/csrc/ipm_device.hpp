@@ -171,39 +171,77 @@ struct Part {
     double Z, R, Vz, Vr, Vt, Az, Ar, WQ;
 };
 
-// MoveESIm for one particle, R/src/pic/particle/ParticleBundle.f90:685-745.
-__device__ __forceinline__ void push_one(const DevState &st, double EFactor, Part &p)
+// Field sources for the gather of the push.  GridField reads one Ez/Er pair;
+// PairField reads interleaved (E0, E1) pairs and returns E0 + c * E1,
+// rounded as c * E1 first and the sum second (the same two roundings as
+// forming the combined grid on the host and gathering from it).
+struct GridField {
+    const double *Ez, *Er;
+    __device__ __forceinline__ double ez(size_t k) const { return gp(Ez)[k]; }
+    __device__ __forceinline__ double er(size_t k) const { return gp(Er)[k]; }
+};
+struct alignas(16) Pair2 {
+    double x, y;
+};
+// one 16-byte global load of pair k
+__device__ __forceinline__ Pair2 ld_pair(const Pair2 *p, size_t k)
 {
-    const int nz = st.nz, nr = st.nr;
+    const __attribute__((address_space(1))) Pair2 *q = gp(p) + k;
+    Pair2 v;
+    v.x = q->x;
+    v.y = q->y;
+    return v;
+}
+struct PairField {
+    const Pair2 *Ez, *Er;
+    double c;
+    __device__ __forceinline__ double ez(size_t k) const { const Pair2 v = ld_pair(Ez, k); return v.x + c * v.y; }
+    __device__ __forceinline__ double er(size_t k) const { const Pair2 v = ld_pair(Er, k); return v.x + c * v.y; }
+};
+
+// Gather indices and weights of MoveESIm, R/src/pic/particle/ParticleBundle.f90:685-700.
+// Index guards keep every read inside the arrays for any input; for the
+// positions absorption leaves behind (0 < Z < Nz-1, 0 <= R < Nr-1) they
+// are no-ops except R == 0 exactly, whose N2 row carries weight R1 = 0.
+struct Gather {
+    double Z1, Z2, Z1H, Z2H, R1, R2, R1H, R2H;
+    size_t kr, kz0, kz1, ldr;
+};
+__device__ __forceinline__ Gather gather_at(int nz, int nr, double z, double r)
+{
+    Gather g;
     const size_t ldz = (size_t)nz + 1;   // Ez(0:nz, 1:nr)
-    const size_t ldr = (size_t)nz;       // Er(1:nz, 0:nr)
-    const double *__restrict__ Ez = st.Ez;
-    const double *__restrict__ Er = st.Er;
-
-    const double z = p.Z, r = p.R;
+    g.ldr = (size_t)nz;                  // Er(1:nz, 0:nr)
     int N1 = (int)ceil(z);
-    double Z1 = (double)N1 - z, Z2 = 1.0 - Z1;
+    g.Z1 = (double)N1 - z; g.Z2 = 1.0 - g.Z1;
     int N1H = (int)(z + 0.5);
-    double Z1H = (double)N1H + 0.5 - z, Z2H = 1.0 - Z1H;
+    g.Z1H = (double)N1H + 0.5 - z; g.Z2H = 1.0 - g.Z1H;
     int N2 = (int)ceil(r);
-    double R1 = (double)N2 - r, R2 = 1.0 - R1;
+    g.R1 = (double)N2 - r; g.R2 = 1.0 - g.R1;
     int N2H = (int)(r + 0.5);
-    double R1H = (double)N2H + 0.5 - r, R2H = 1.0 - R1H;
-
-    // Index guards keep every read inside the arrays for any input; for the
-    // positions absorption leaves behind (0 < Z < Nz-1, 0 <= R < Nr-1) they
-    // are no-ops except R == 0 exactly, whose N2 row carries weight R1 = 0.
+    g.R1H = (double)N2H + 0.5 - r; g.R2H = 1.0 - g.R1H;
     int i1 = min(max(N1, 1), nz - 1);
     int i1h = min(max(N1H, 0), nz - 1);
     int j2 = min(max(N2, 1), nr - 1);
     int j2b = min(max(N2, 0), nr - 1) + 1;
     int j2h = min(max(N2H, 0), nr - 1);
+    g.kr = (size_t)(i1 - 1) + (size_t)j2h * g.ldr;
+    g.kz0 = (size_t)i1h + (size_t)(j2 - 1) * ldz;
+    g.kz1 = (size_t)i1h + (size_t)(j2b - 1) * ldz;
+    return g;
+}
 
-    const auto *er0 = gp(Er + (size_t)(i1 - 1) + (size_t)j2h * ldr);
-    const auto *ez0 = gp(Ez + (size_t)i1h + (size_t)(j2 - 1) * ldz);
-    const auto *ez1 = gp(Ez + (size_t)i1h + (size_t)(j2b - 1) * ldz);
-    double E1 = er0[0] * Z1 * R1H + er0[1] * Z2 * R1H + er0[ldr] * Z1 * R2H + er0[ldr + 1] * Z2 * R2H;
-    double E2 = ez0[0] * Z1H * R1 + ez0[1] * Z2H * R1 + ez1[0] * Z1H * R2 + ez1[1] * Z2H * R2;
+// MoveESIm for one particle, R/src/pic/particle/ParticleBundle.f90:685-745.
+template <class FS>
+__device__ __forceinline__ void push_core(const FS &fs, int nz, int nr, double EFactor, Part &p)
+{
+    const double z = p.Z, r = p.R;
+    const Gather g = gather_at(nz, nr, z, r);
+    const size_t kr = g.kr, kz0 = g.kz0, kz1 = g.kz1, ldr = g.ldr;
+    const double Z1 = g.Z1, Z2 = g.Z2, Z1H = g.Z1H, Z2H = g.Z2H, R1 = g.R1, R2 = g.R2, R1H = g.R1H, R2H = g.R2H;
+    double E1 = fs.er(kr) * Z1 * R1H + fs.er(kr + 1) * Z2 * R1H + fs.er(kr + ldr) * Z1 * R2H +
+                fs.er(kr + ldr + 1) * Z2 * R2H;
+    double E2 = fs.ez(kz0) * Z1H * R1 + fs.ez(kz0 + 1) * Z2H * R1 + fs.ez(kz1) * Z1H * R2 + fs.ez(kz1 + 1) * Z2H * R2;
     E1 = E1 * EFactor;
     E2 = E2 * EFactor;
 
@@ -235,6 +273,11 @@ __device__ __forceinline__ void push_one(const DevState &st, double EFactor, Par
         ar = ar * CosA;
     }
     p.Z = zz; p.R = P_R; p.Vz = vz; p.Vr = vr; p.Vt = vt; p.Az = az; p.Ar = ar;
+}
+
+__device__ __forceinline__ void push_one(const DevState &st, double EFactor, Part &p)
+{
+    push_core(GridField{st.Ez, st.Er}, st.nz, st.nr, EFactor, p);
 }
 
 // ---------------------------------------------------------- absorption

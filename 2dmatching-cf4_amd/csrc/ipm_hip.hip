@@ -597,6 +597,76 @@ __global__ void grid_kernel(const DevState *__restrict__ stp, const double *__re
     }
 }
 
+// ------------------------------------------------------- moment diagnostics
+// WeightingParticleField for one species, R/src/diag/diagParticleField.F90:196-257:
+// node sums, at the P2CES node weights, of WQ, WQ*(Vr^2+Vt^2+Vz^2), 1 (the
+// macro-particle count), WQ*Vz and WQ*Vr.  With implicit_j the currents use
+// the velocity of a copy pushed once more in the present field
+// (MoveParticleOneElectrostaticImplicit, R/src/pic/particle/ParticleBundle.f90:782-858),
+// as the reference does for its implicit scheme (diagParticleField.F90:225-236).
+__global__ void moments_kernel(const DevState *__restrict__ stp, const DevCounters *__restrict__ ctr, int s,
+                               int implicit_j, double *__restrict__ acc)
+{
+    const DevState &st = *stp;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)ctr->n[s]) return;
+    double *const *F = st.f[s];
+    Part p;
+    p.Z = gp(F[0])[i];
+    if (is_dead(p.Z)) return;
+    p.R = gp(F[1])[i]; p.Vz = gp(F[2])[i]; p.Vr = gp(F[3])[i]; p.Vt = gp(F[4])[i];
+    p.Az = gp(F[5])[i]; p.Ar = gp(F[6])[i]; p.WQ = gp(F[7])[i];
+    int NzL = (int)ceil(p.Z);
+    const double ZL = (double)NzL - p.Z, ZU = 1.0 - ZL;
+    int NrL = (int)ceil(p.R);
+    const double RL = (double)NrL - p.R, RU = 1.0 - RL;
+    NzL = min(max(NzL, 1), st.nz - 1);
+    const bool row0 = NrL < 1;
+    NrL = min(max(NrL, 1), st.nr - 1);
+    const double energy = p.Vr * p.Vr + p.Vt * p.Vt + p.Vz * p.Vz;
+    double vz = p.Vz, vr = p.Vr;
+    if (implicit_j) {
+        Part t = p;
+        push_one(st, st.efactor[s], t);
+        vz = t.Vz;
+        vr = t.Vr;
+    }
+    double w[4] = {ZL * RL, ZL * RU, ZU * RL, ZU * RU};
+    if (row0) { w[0] = w[1]; w[2] = w[3]; w[1] = 0.0; w[3] = 0.0; }
+    const size_t nn = (size_t)st.nz * st.nr;
+    const size_t k0 = (size_t)(NzL - 1) + (size_t)(NrL - 1) * st.nz;
+    const size_t node[4] = {k0, k0 + st.nz, k0 + 1, k0 + 1 + st.nz};
+    for (int q = 0; q < 4; ++q) {
+        if (w[q] == 0.0) continue;
+        const double ww = w[q] * p.WQ;
+        global_add(acc + node[q], ww);
+        global_add(acc + nn + node[q], ww * energy);
+        global_add(acc + 2 * nn + node[q], w[q]);
+        global_add(acc + 3 * nn + node[q], ww * vz);
+        global_add(acc + 4 * nn + node[q], ww * vr);
+    }
+}
+
+// Scaling of the sums, R/src/diag/diagParticleField.F90:300-318: densities
+// times point_inv_volume, the energy density times 0.5 m VFactor^2 and
+// divided by JtoeV and the density where that is positive (mean energy in
+// eV), currents times point_inv_volume * Charge * VFactor.
+__global__ void moments_scale_kernel(double *__restrict__ acc, const double *__restrict__ inv_vol, size_t nn,
+                                     double energy_factor, double charge, double vfactor)
+{
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nn) return;
+    const double iv = inv_vol[k];
+    const double rho = acc[k] * iv;
+    double e = acc[nn + k] * iv;
+    e = e * energy_factor;
+    if (rho > 0.0) e = e / kJTOEV / rho;
+    acc[k] = rho;
+    acc[nn + k] = e;
+    acc[3 * nn + k] = acc[3 * nn + k] * iv * charge * vfactor;
+    acc[4 * nn + k] = acc[4 * nn + k] * iv * charge * vfactor;
+}
+
 __global__ void clear_kernel(double *a, size_t n)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -732,71 +802,7 @@ __global__ void set_count_kernel(DevCounters *ctr, int s, const unsigned long lo
 // ====================================================================
 //                              host side
 // ====================================================================
-static thread_local char g_err[512] = "";
-
-static int fail(int code, const char *fmt, ...)
-{
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(g_err, sizeof g_err, fmt, ap);
-    va_end(ap);
-    return code;
-}
-
-#define HIPCHK(x)                                                                      \
-    do {                                                                               \
-        hipError_t e_ = (x);                                                           \
-        if (e_ != hipSuccess)                                                          \
-            return fail(IPM_EHIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_),  \
-                        __FILE__, __LINE__);                                           \
-    } while (0)
-
-struct ipm_ctx {
-    int device = 0;
-    int num_cu = 256;
-    hipStream_t stream = nullptr;
-    int nz = 0, nr = 0, ns = 0, ng = 0;
-    double dx = 0, dt = 0;
-    DevState h{};
-    DevState *d_state = nullptr;
-    DevCounters *d_ctr = nullptr;
-    double *buf[2][IPM_NS_MAX] = {};
-    int cur[IPM_NS_MAX] = {};
-    bool species_set[IPM_NS_MAX] = {};
-    double chifac[IPM_NS_MAX] = {};
-    double *d_chifac = nullptr;
-    double *raw_all = nullptr, *rho_one = nullptr, *chi_one = nullptr;
-    double *rho_tot = nullptr, *chi_tot = nullptr;
-    double *inv_vol = nullptr;
-    int *cell_material = nullptr, *hedge = nullptr, *vedge = nullptr, *mat_type = nullptr;
-    double *metal_charge = nullptr, *sigma = nullptr;
-    double *Ez = nullptr, *Er = nullptr;
-    unsigned long long *mcc_counts = nullptr;
-    std::vector<void *> tab_mem[IPM_NS_MAX * IPM_NG_MAX];
-    unsigned long long *hist = nullptr, *offs = nullptr, *cursor = nullptr, *partial = nullptr,
-                       *partial2 = nullptr;
-    double **d_dst = nullptr;
-    double *qbuf = nullptr;
-    long long *q_slot = nullptr;
-    int *q_species = nullptr;
-    long long q_cap = 0;
-    unsigned *q_part_count = nullptr;
-    hipGraphExec_t graph = nullptr;
-    int block = kBlock, blocks_per_cu = 0;
-    // optional HIP-event timing of the kernels ipm_step enqueues
-    bool timing = false;
-    std::vector<hipEvent_t> ev_pool;
-    std::vector<std::pair<int, int>> ev_pairs[3];   // (start, stop) per kernel class
-    size_t ev_used = 0;
-    bool geometry_set = false, fields_set = false;
-};
-
-static int sync_state(ipm_ctx *c)
-{
-    HIPCHK(hipMemcpyAsync(c->d_state, &c->h, sizeof(DevState), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    return IPM_OK;
-}
+#include "ipm_ctx.hpp"
 
 // Collision queue sized for the expected selections with a wide margin:
 // capacity_s * min(1, 2 * sum_g CollisionRatio) + 64k entries.
@@ -838,27 +844,6 @@ static int ensure_queue(ipm_ctx *c)
         (void)hipGraphExecDestroy(c->graph);
         c->graph = nullptr;
     }
-    return IPM_OK;
-}
-
-static int read_counters(ipm_ctx *c, DevCounters *out)
-{
-    HIPCHK(hipMemcpyAsync(out, c->d_ctr, sizeof(DevCounters), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    if (out->overflow & 1)
-        return fail(IPM_ECAPACITY, "The size of particle bundle reaches the maximum (MCC products dropped).");
-    if (out->overflow & 2) return fail(IPM_EINVAL, "MCC product with an invalid species index.");
-    if (out->overflow & 4) return fail(IPM_ECAPACITY, "collision queue overflow (selections were not collided)");
-    return IPM_OK;
-}
-
-static int check_ready(ipm_ctx *c, bool need_fields, bool need_geom)
-{
-    if (!c) return fail(IPM_EINVAL, "null context");
-    for (int s = 0; s < c->ns; ++s)
-        if (!c->species_set[s]) return fail(IPM_EINVAL, "species %d has not been set", s);
-    if (need_fields && !c->fields_set) return fail(IPM_EINVAL, "fields have not been set");
-    if (need_geom && !c->geometry_set) return fail(IPM_EINVAL, "geometry has not been set");
     return IPM_OK;
 }
 
@@ -974,6 +959,7 @@ int ipm_destroy(ipm_ctx *c)
 {
     if (!c) return IPM_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    (void)window_release(c);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     for (int b = 0; b < 2; ++b)
@@ -985,7 +971,7 @@ int ipm_destroy(ipm_ctx *c)
                     c->d_chifac, c->inv_vol, c->cell_material, c->hedge, c->vedge, c->mat_type,
                     c->metal_charge, c->sigma, c->Ez, c->Er, c->mcc_counts, c->hist, c->offs,
                     c->cursor, c->partial, c->partial2, c->d_dst, c->qbuf, c->q_slot, c->q_species,
-                    c->q_part_count};
+                    c->q_part_count, c->mom};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1208,6 +1194,31 @@ int ipm_p2c(ipm_ctx *c, int s, double *rho_one, double *chi_one)
     const size_t nn = (size_t)c->nz * c->nr;
     if (rho_one) HIPCHK(hipMemcpy(rho_one, c->rho_one + (size_t)s0 * nn, sizeof(double) * nn * (s1 - s0), hipMemcpyDeviceToHost));
     if (chi_one) HIPCHK(hipMemcpy(chi_one, c->chi_one + (size_t)s0 * nn, sizeof(double) * nn * (s1 - s0), hipMemcpyDeviceToHost));
+    return IPM_OK;
+}
+
+int ipm_moments(ipm_ctx *c, int s, int implicit_j, double *rho, double *energy_ev, double *mp, double *jz,
+                double *jr)
+{
+    int rc = check_ready(c, implicit_j != 0, true);
+    if (rc) return rc;
+    if (s < 0 || s >= c->ns) return fail(IPM_EINVAL, "invalid species index %d", s);
+    const size_t nn = (size_t)c->nz * c->nr;
+    if (!c->mom) HIPCHK(hipMalloc(&c->mom, sizeof(double) * 5 * nn));
+    HIPCHK(hipMemsetAsync(c->mom, 0, sizeof(double) * 5 * nn, c->stream));
+    const long long cap = c->h.cap[s];
+    if (cap > 0)
+        hipLaunchKernelGGL(moments_kernel, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, c->stream, c->d_state,
+                           c->d_ctr, s, implicit_j ? 1 : 0, c->mom);
+    const double vf = c->h.vfactor;
+    const double ef = 0.5 * c->h.mass[s] * vf * vf;
+    hipLaunchKernelGGL(moments_scale_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream, c->mom,
+                       c->inv_vol, nn, ef, c->h.charge[s], vf);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double *outs[5] = {rho, energy_ev, mp, jz, jr};
+    for (int q = 0; q < 5; ++q)
+        if (outs[q]) HIPCHK(hipMemcpy(outs[q], c->mom + (size_t)q * nn, sizeof(double) * nn, hipMemcpyDeviceToHost));
     return IPM_OK;
 }
 

@@ -67,6 +67,14 @@ SIGNATURES = {
     "ipm_set_launch": [_vp, C.c_int, C.c_int],
     "ipm_synchronize": [_vp],
     "ipm_species_device_ptrs": [_vp, C.c_int, C.POINTER(_dp)],
+    "ipm_moments": [_vp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp],
+    "ipm_window_create": [_vp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp],
+    "ipm_window_forward": [_vp, _dp, C.c_int, _dp],
+    "ipm_window_backward": [_vp, _dp, _dp, _dp],
+    "ipm_window_info": [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_float),
+                        C.POINTER(C.c_float)],
+    "ipm_window_download": [_vp, C.c_int, _dp, _ip],
+    "ipm_window_destroy": [_vp],
 }
 
 _lib = None

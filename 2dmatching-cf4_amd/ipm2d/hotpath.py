@@ -9,6 +9,7 @@ the reference's names and argument meaning:
     field_source_sum() -> Rho, Chi FT%Sum(FO)                  FieldSource.f90:72
     mcc()                          MCC(Ns, Ng, PB, ...)        MCCPublic.F90:38
     one_step()                     OneStep's particle calls    onestep.F90:141-177
+    moments(s)                     WeightingParticleField      diagParticleField.F90:173
 
 Errors raise ``IPMError`` (``CapacityError`` where the reference would
 ``stop`` on a full particle bundle).  Grids are returned as numpy arrays
@@ -181,6 +182,14 @@ class HotPath:
         chi = np.zeros((self.nz + 2) * (self.nr + 2))
         N.call("ipm_field_sum", self._h, _d(rho), _d(chi))
         return rho.reshape(self.nr, self.nz).T, chi.reshape(self.nr + 2, self.nz + 2).T
+
+    def moments(self, s, implicit_j=True):
+        """WeightingParticleField for species s: dict of (nz, nr) arrays
+        rho (RhoOne), energy (TOne, eV), mp (MPOne), jz, jr (JzOne, JrOne)."""
+        out = {k: np.zeros(self.nz * self.nr) for k in ("rho", "energy", "mp", "jz", "jr")}
+        N.call("ipm_moments", self._h, s, 1 if implicit_j else 0, *(_d(out[k]) for k in
+                                                                    ("rho", "energy", "mp", "jz", "jr")))
+        return {k: v.reshape(self.nr, self.nz).T for k, v in out.items()}
 
     def mcc(self):
         N.call("ipm_mcc", self._h)

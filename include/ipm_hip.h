@@ -146,6 +146,56 @@ int ipm_synchronize(ipm_ctx *ctx);
  * of the raw per-species deposit grid. */
 int ipm_species_device_ptrs(ipm_ctx *ctx, int s, double **fields8);
 
+/* Moment diagnostics of species s: WeightingParticleField
+ * (R/src/diag/diagParticleField.F90:173-318).  Each output is Nz*Nr (or
+ * NULL to skip): rho = RhoOne (number density), energy_ev = TOne (mean
+ * energy in eV where rho > 0), mp = MPOne (macro-particles per node),
+ * jz/jr = JzOne/JrOne (A/m^2).  implicit_j != 0 takes the current from a
+ * copy pushed once more in the present field, as the reference's implicit
+ * scheme does (diagParticleField.F90:225-236; needs ipm_set_fields);
+ * implicit_j == 0 uses the particle's own velocity. */
+int ipm_moments(ipm_ctx *ctx, int s, int implicit_j, double *rho, double *energy_ev, double *mp, double *jz,
+                double *jr);
+
+/* ---- Differentiable collisionless window ----------------------------------
+ * No counterpart in the reference (which has no gradients).  Advances a
+ * snapshot of the resident particles through nsteps steps of MoveESIm
+ * (R/src/pic/particle/ParticleBundle.f90:671-748) and the absorption rule of
+ * MaterialAborptionParticle (R/src/pic/particleboundary/ParticleMaterial.F90:22-41)
+ * in the field E = E0 + coef[n] * E1 of step n, with a loss per step
+ *   loss[n] = sum_i alpha[s_i] * WQ_i * Vz_i * M(Z_i, R_i),
+ *   M = P2CES-weighted interpolation (R/src/pic/field/FieldOne.f90:151-166)
+ *       of the node field mu (Nz*Nr),
+ * taken after the push and absorption of step n.  With mu = mask *
+ * point_inv_volume and alpha[s] = dt * w_s * Charge_s * VFactor, loss[n] =
+ * dt * sum_s w_s * sum_nodes mask * JzOne_s (explicit current, see
+ * ipm_moments).  No collisions happen inside the window.
+ *
+ * ipm_window_create snapshots every resident slot of every species (sort
+ * first: the adjoint skips whole wavefronts whose particles never reach the
+ * support of mu, which cell order makes common) and keeps a checkpoint of
+ * the state every `interval` steps (1 <= interval <= 64).  E0/E1 use the
+ * Ez/Er layouts above; alpha has ns entries. */
+int ipm_window_create(ipm_ctx *ctx, int nsteps, int interval, const double *Ez0, const double *Er0,
+                      const double *Ez1, const double *Er1, const double *mu, const double *alpha);
+/* Forward over the window from the snapshot; loss receives nsteps values.
+ * record != 0: particles are deleted by the reference's absorption rule and
+ * each one's deletion step is stored; record == 0 replays the stored
+ * deletion steps (events frozen), so the loss is a smooth function of coef. */
+int ipm_window_forward(ipm_ctx *ctx, const double *coef, int record, double *loss);
+/* Adjoint: dcoef[n] = d(sum_m gloss[m] * loss[m]) / d coef[n] for the
+ * replayed window at coef (checkpoints are rebuilt first when the last
+ * forward used other coefficients). */
+int ipm_window_backward(ipm_ctx *ctx, const double *coef, const double *gloss, double *dcoef);
+/* Particles in the window, checkpoint segments, and HIP-event milliseconds
+ * of the last forward and the last adjoint. */
+int ipm_window_info(ipm_ctx *ctx, int64_t *nparticles, int *nsegments, float *forward_ms, float *adjoint_ms);
+/* Checkpoint k (0 = snapshot, nsegments = end of window) as 7 arrays of
+ * nparticles doubles (Z R Vz Vr Vt Az Ar), and each particle's deletion step
+ * (nsteps = never, -1 = empty slot).  Either pointer may be NULL. */
+int ipm_window_download(ipm_ctx *ctx, int checkpoint, double *state, int *absorbed_step);
+int ipm_window_destroy(ipm_ctx *ctx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,6 +16,9 @@
  *   ora_deposit_*         charge conservation, on-node known answers,
  *                         uniform density vs analytic node volumes
  *   ora_node_volumes      closed-form cylindrical shell volumes
+ *   ora_moments           charge conservation, macro-particle count, uniform
+ *                         velocity and speed known answers, implicit current
+ *                         equal to the explicit one in a zero field
  *   ora_mcc               selection ratio 1-exp(-nu dt), channel fractions
  *                         n sigma v, Vahedi/isotropic cos-chi CDFs,
  *                         Maxwellian mean energy, energy bookkeeping of
@@ -277,6 +280,76 @@ void ora_deposit_finish(int nz, int nr, double dt, double charge, double mass,
         v = v * charge;
         rho[k] = v;
         chi[k] = v * ChiFactor;
+    }
+}
+
+/* WeightingParticleField for one species, R/src/diag/diagParticleField.F90:196-257
+ * (node sums) and 300-318 (scaling).  P holds n particles (no deleted
+ * slots).  implicit_j: the currents use a copy pushed once more in (Ez, Er)
+ * (MoveParticleOneElectrostaticImplicit, ParticleBundle.f90:782-858); else
+ * the particle's own Vz and Vr (the reference's explicit branch reads Vr of a
+ * temporary it never assigns, diagParticleField.F90:249-252). */
+void ora_moments(int nz, int nr, double dx, double dt, double charge, double mass, const double *Ez,
+                 const double *Er, int implicit_j, int64_t n, double *const P[8], const double *inv_vol,
+                 double *rho, double *energy, double *mp, double *jz, double *jr)
+{
+    const size_t nn = (size_t)nz * nr;
+    const double VFactor = dx / dt;     /* ParticleBundle.f90:346 */
+    for (size_t k = 0; k < nn; ++k) rho[k] = energy[k] = mp[k] = jz[k] = jr[k] = 0.0;
+#define AT(a, i, j) a[((size_t)(i) - 1) + ((size_t)(j) - 1) * nz]
+    for (int64_t p = 0; p < n; ++p) {
+        const double Z = P[0][p], R = P[1][p], WQ = P[7][p];
+        int NzL = (int)ceil(Z);
+        int NzU = NzL + 1;
+        double ZL = (double)NzL - Z;
+        double ZU = 1.0 - ZL;
+        int NrL = (int)ceil(R);
+        int NrU = NrL + 1;
+        double RL = (double)NrL - R;
+        double RU = 1.0 - RL;
+        double Energy = P[3][p] * P[3][p] + P[4][p] * P[4][p] + P[2][p] * P[2][p];
+        double tz = P[2][p], tr = P[3][p];
+        if (implicit_j) {
+            double z1 = Z, r1 = R, vz = P[2][p], vr = P[3][p], vt = P[4][p], az = P[5][p], ar = P[6][p];
+            ora_push(nz, nr, dx, dt, charge, mass, Ez, Er, 1, &z1, &r1, &vz, &vr, &vt, &az, &ar);
+            tz = vz;
+            tr = vr;
+        }
+        /* R == 0 exactly: row NrL = 0 lies outside the arrays and its weight RL is 0 */
+        if (NrL >= 1) {
+            AT(rho, NzL, NrL) = AT(rho, NzL, NrL) + ZL * RL * WQ;
+            AT(energy, NzL, NrL) = AT(energy, NzL, NrL) + ZL * RL * WQ * Energy;
+            AT(mp, NzL, NrL) = AT(mp, NzL, NrL) + ZL * RL;
+            AT(jz, NzL, NrL) = AT(jz, NzL, NrL) + ZL * RL * WQ * tz;
+            AT(jr, NzL, NrL) = AT(jr, NzL, NrL) + ZL * RL * WQ * tr;
+        }
+        AT(rho, NzL, NrU) = AT(rho, NzL, NrU) + ZL * RU * WQ;
+        AT(energy, NzL, NrU) = AT(energy, NzL, NrU) + ZL * RU * WQ * Energy;
+        AT(mp, NzL, NrU) = AT(mp, NzL, NrU) + ZL * RU;
+        AT(jz, NzL, NrU) = AT(jz, NzL, NrU) + ZL * RU * WQ * tz;
+        AT(jr, NzL, NrU) = AT(jr, NzL, NrU) + ZL * RU * WQ * tr;
+        if (NrL >= 1) {
+            AT(rho, NzU, NrL) = AT(rho, NzU, NrL) + ZU * RL * WQ;
+            AT(energy, NzU, NrL) = AT(energy, NzU, NrL) + ZU * RL * WQ * Energy;
+            AT(mp, NzU, NrL) = AT(mp, NzU, NrL) + ZU * RL;
+            AT(jz, NzU, NrL) = AT(jz, NzU, NrL) + ZU * RL * WQ * tz;
+            AT(jr, NzU, NrL) = AT(jr, NzU, NrL) + ZU * RL * WQ * tr;
+        }
+        AT(rho, NzU, NrU) = AT(rho, NzU, NrU) + ZU * RU * WQ;
+        AT(energy, NzU, NrU) = AT(energy, NzU, NrU) + ZU * RU * WQ * Energy;
+        AT(mp, NzU, NrU) = AT(mp, NzU, NrU) + ZU * RU;
+        AT(jz, NzU, NrU) = AT(jz, NzU, NrU) + ZU * RU * WQ * tz;
+        AT(jr, NzU, NrU) = AT(jr, NzU, NrU) + ZU * RU * WQ * tr;
+    }
+#undef AT
+    const double EnergyFactor = 0.5 * mass * VFactor * VFactor;
+    for (size_t k = 0; k < nn; ++k) {
+        rho[k] = rho[k] * inv_vol[k];
+        energy[k] = energy[k] * inv_vol[k];
+        energy[k] = energy[k] * EnergyFactor;
+        jz[k] = jz[k] * inv_vol[k] * charge * VFactor;
+        jr[k] = jr[k] * inv_vol[k] * charge * VFactor;
+        if (rho[k] > 0.0) energy[k] = energy[k] / C_JTOEV / rho[k];
     }
 }
 

@@ -68,6 +68,12 @@ void ora_deposit_finish(int nz, int nr, double dt, double charge, double mass,
                         const double *rho_raw, const double *inv_vol,
                         double *rho, double *chi);
 
+/* WeightingParticleField for one species (diagParticleField.F90:196-318):
+ * rho, energy (mean eV where rho > 0), mp, jz, jr, each nz*nr. */
+void ora_moments(int nz, int nr, double dx, double dt, double charge, double mass, const double *Ez,
+                 const double *Er, int implicit_j, int64_t n, double *const P[8], const double *inv_vol,
+                 double *rho, double *energy, double *mp, double *jz, double *jr);
+
 /* ---- MCC ---------------------------------------------------------------- */
 typedef struct {
     int model;            /* 0 none, 1 electron, 2 non-reactive ion, 3 reactive ion */

@@ -58,6 +58,8 @@ def lib(fast=False):
     L.ora_absorb.argtypes = [C.POINTER(OraGeom), C.c_double, _i64p, C.POINTER(_dp), _dp, _dp]
     L.ora_deposit_raw.argtypes = [C.c_int, C.c_int, C.c_int64, _dp, _dp, _dp, _dp]
     L.ora_deposit_finish.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _dp]
+    L.ora_moments.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp,
+                              C.c_int, C.c_int64, C.POINTER(_dp)] + [_dp] * 6
     L.ora_mcc.restype = C.c_int
     L.ora_mcc.argtypes = [C.POINTER(OraSetup), C.c_uint64, C.c_uint32, _i64p, _i64p, C.POINTER(_dp * 8), _i64p]
     L.ora_vel_maxwell.restype = C.c_uint32
@@ -185,6 +187,50 @@ def deposit_finish(nz, nr, dt, charge, mass, raw, inv_vol):
 
 def p2ces(nz, nr, dt, charge, mass, P, inv_vol):
     return deposit_finish(nz, nr, dt, charge, mass, deposit_raw(nz, nr, P), inv_vol)
+
+
+def moments(nz, nr, dx, dt, charge, mass, ez_flat, er_flat, P, inv_vol, implicit_j=True):
+    """WeightingParticleField of one species: dict of (nz, nr) arrays
+    rho, energy (eV), mp, jz, jr."""
+    P = np.array(P, dtype=np.float64, copy=True, order="C")
+    ez = np.ascontiguousarray(ez_flat, dtype=np.float64)
+    er = np.ascontiguousarray(er_flat, dtype=np.float64)
+    iv = F(inv_vol)
+    outs = {k: np.zeros(nz * nr) for k in ("rho", "energy", "mp", "jz", "jr")}
+    ptrs = (_dp * 8)(*[_d(P[f]) for f in range(8)])
+    lib().ora_moments(nz, nr, dx, dt, charge, mass, _d(ez), _d(er), 1 if implicit_j else 0, P.shape[1], ptrs,
+                      _d(iv), *[_d(outs[k]) for k in ("rho", "energy", "mp", "jz", "jr")])
+    return {k: unF(v, nz, nr) for k, v in outs.items()}
+
+
+def window_plain(geom, species, e0, e1, coef, mask, weights, dt, metal=1):
+    """The collisionless window by the plain method: for each step n the
+    field E0 + coef[n] E1 (formed on the host), MoveESIm and
+    MaterialAborptionParticle per species, then the loss
+    l[n] = dt * sum_s weights[s] * sum_nodes mask * JzOne_s (explicit current).
+    species: list of (charge, mass, P).  Returns (l, electrode charge
+    collected per step by metal `metal`, final particle arrays)."""
+    nz, nr, dx = geom.nz, geom.nr, geom.dx
+    Ps = [np.array(P, dtype=np.float64, copy=True) for _, _, P in species]
+    ez0, er0 = (np.asarray(x, dtype=np.float64).ravel() for x in e0)
+    ez1, er1 = (np.asarray(x, dtype=np.float64).ravel() for x in e1)
+    loss, qel = [], []
+    for n, c in enumerate(coef):
+        ez = ez0 + c * ez1
+        er = er0 + c * er1
+        ln, qn = 0.0, 0.0
+        for s, (q, m, _) in enumerate(species):
+            P = push(nz, nr, dx, dt, q, m, ez, er, Ps[s])
+            P, mc, _ = absorb(geom, q, P)
+            Ps[s] = P
+            if geom.nmetal >= metal:
+                qn += mc[metal - 1]
+            if weights[s] != 0.0:
+                mo = moments(nz, nr, dx, dt, q, m, ez, er, P, geom.inv_volume, implicit_j=False)
+                ln += weights[s] * dt * float(np.sum(mask * mo["jz"]))
+        loss.append(ln)
+        qel.append(qn)
+    return np.array(loss), np.array(qel), Ps
 
 
 class MccSetup:
