@@ -35,6 +35,7 @@ namespace {
 constexpr int kWBlock = 256;
 constexpr int kKMax = 64;     // largest checkpoint interval (per-step LDS sums)
 constexpr int kNS = 7;        // Z R Vz Vr Vt Az Ar (WQ is constant)
+constexpr int kTape = 5;      // Z R Vr Vt Ar: what the adjoint of a push reads
 
 struct WinArgs {
     int nz, nr, ns, T, K, nseg;
@@ -46,11 +47,12 @@ struct WinArgs {
     const Pair2 *Ezp, *Erp;
     const double *mu;
     const int *cell_material, *mat_type;
-    int ndiel, pad;
+    int ndiel;
+    int prefilter;                 // adjoint: skip wavefronts that never reach the support
     double *ck;                    // (nseg + 1) checkpoints of kNS arrays of N
     const double *wq;              // N
     int *death;                    // N: absorption step; T = survives; -1 = empty slot
-    double *tape;                  // K * kNS arrays of N
+    double *tape;                  // K * kTape arrays of N
     double *lam;                   // kNS arrays of N
     const double *c;               // T
     double *acc;                   // T: loss (forward) or dl/dc (adjoint)
@@ -138,8 +140,11 @@ __device__ __forceinline__ void lds_add_d(double *a, double v)
 
 // Adjoint of push_core with the PairField gather: on entry l holds the
 // adjoint of the pushed state, on exit that of the state q before the push;
-// returns the derivative with respect to the field coefficient c.
-__device__ __forceinline__ double push_adjoint(const WinArgs &a, double F, double c, const Part &q, Adj &l)
+// returns the derivative with respect to the field coefficient c and, in
+// E2out, the scaled Ez of the push.  Only q.Z, q.R, q.Vr, q.Vt, q.Ar are
+// read: the push is affine in Vz and Az.
+__device__ __forceinline__ double push_adjoint(const WinArgs &a, double F, double c, const Part &q, Adj &l,
+                                               double &E2out)
 {
     const Gather g = gather_at(a.nz, a.nr, q.Z, q.R);
     const Pair2 a00 = ld_pair(a.Erp, g.kr), a10 = ld_pair(a.Erp, g.kr + 1);
@@ -151,6 +156,7 @@ __device__ __forceinline__ double push_adjoint(const WinArgs &a, double F, doubl
     const double Z1 = g.Z1, Z2 = g.Z2, Z1H = g.Z1H, Z2H = g.Z2H, R1 = g.R1, R2 = g.R2, R1H = g.R1H, R2H = g.R2H;
 
     const double E1 = (A00 * Z1 * R1H + A10 * Z2 * R1H + A01 * Z1 * R2H + A11 * Z2 * R2H) * F;
+    E2out = (B00 * Z1H * R1 + B10 * Z2H * R1 + B01 * Z1H * R2 + B11 * Z2H * R2) * F;
     // d/dZ of Z1, Z1H is -1 and of Z2, Z2H +1; likewise in R
     const double dE1dz = ((A10 - A00) * R1H + (A11 - A01) * R2H) * F;
     const double dE1dr = ((A01 * Z1 + A11 * Z2) - (A00 * Z1 + A10 * Z2)) * F;
@@ -275,6 +281,21 @@ __global__ void __launch_bounds__(kWBlock) win_forward_kernel(WinArgs a, int j, 
 // Adjoint of checkpoint segment j.  lam holds, on entry, the adjoint of the
 // state at the segment end (ignored for the last segment) and, on exit, that
 // of the state at the segment start.  a.acc[n] += dl/dc[n].
+//
+// The recompute keeps Z R Vr Vt Ar of every pre-push state on the tape; the
+// reverse sweep walks the tape backwards holding the post-push state of the
+// step it differentiates (the next tape entry), and carries Vz and Az back
+// through the affine z-update: Az[n] = 2 Az[n+1] - E2[n],
+// Vz[n] = Vz[n+1] - Az[n+1] - E2[n].
+__device__ __forceinline__ void tape_store(double *t, long long N, long long i, const Part &p)
+{
+    gp(t)[i] = p.Z; gp(t + N)[i] = p.R; gp(t + 2 * N)[i] = p.Vr; gp(t + 3 * N)[i] = p.Vt; gp(t + 4 * N)[i] = p.Ar;
+}
+__device__ __forceinline__ void tape_load(const double *t, long long N, long long i, Part &p)
+{
+    p.Z = gp(t)[i]; p.R = gp(t + N)[i]; p.Vr = gp(t + 2 * N)[i]; p.Vt = gp(t + 3 * N)[i]; p.Ar = gp(t + 4 * N)[i];
+}
+
 __global__ void __launch_bounds__(kWBlock) win_adjoint_kernel(WinArgs a, int j)
 {
     __shared__ double sg[kKMax];
@@ -306,58 +327,65 @@ __global__ void __launch_bounds__(kWBlock) win_adjoint_kernel(WinArgs a, int j)
             l.Vr = gp(a.lam + 3 * N)[i]; l.Vt = gp(a.lam + 4 * N)[i]; l.Az = gp(a.lam + 5 * N)[i];
             l.Ar = gp(a.lam + 6 * N)[i];
         }
-        const bool seeded = l.Z != 0.0 || l.R != 0.0 || l.Vz != 0.0 || l.Vr != 0.0 || l.Vt != 0.0 ||
-                            l.Az != 0.0 || l.Ar != 0.0;
-        // Pass 1 (no stores): does the trajectory reach the loss support?
         Part p0{};
         if (active) load_state(src, N, i, p0);
-        bool touched = false;
-        {
+        bool need = true;
+        if (a.prefilter) {
+            // Pass 1 (no stores): particles that never reach the support and
+            // carry no adjoint from later steps have a zero adjoint over the
+            // whole segment; a wavefront of only such particles is skipped.
+            const bool seeded = l.Z != 0.0 || l.R != 0.0 || l.Vz != 0.0 || l.Vr != 0.0 || l.Vt != 0.0 ||
+                                l.Az != 0.0 || l.Ar != 0.0;
+            bool touched = false;
             Part p = p0;
             for (int n = n0; n < nl; ++n) {
                 push_core(PairField{a.Ezp, a.Erp, a.c[n]}, a.nz, a.nr, F, p);
                 if (p.R <= 0.0) { p.R = fabs(p.R); p.Vr = fabs(p.Vr); p.Ar = fabs(p.Ar); }
                 touched |= in_support(a, p);
             }
+            need = __ballot(touched || seeded) != 0;
         }
-        // Particles that never reach the support and carry no adjoint from
-        // later steps have a zero adjoint over the whole segment.
-        if (__ballot(touched || seeded)) {
+        if (need) {
             // Pass 2: recompute with the tape of pre-push states.
-            {
-                Part p = p0;
-                for (int n = n0; n < nl; ++n) {
-                    store_state(a.tape + (size_t)(n - n0) * kNS * N, N, i, p);
-                    push_core(PairField{a.Ezp, a.Erp, a.c[n]}, a.nz, a.nr, F, p);
-                    if (p.R <= 0.0) { p.R = fabs(p.R); p.Vr = fabs(p.Vr); p.Ar = fabs(p.Ar); }
-                }
+            Part post = p0;
+            for (int n = n0; n < nl; ++n) {
+                tape_store(a.tape + (size_t)(n - n0) * kTape * N, N, i, post);
+                push_core(PairField{a.Ezp, a.Erp, a.c[n]}, a.nz, a.nr, F, post);
+                if (post.R <= 0.0) { post.R = fabs(post.R); post.Vr = fabs(post.Vr); post.Ar = fabs(post.Ar); }
             }
-            // Reverse sweep, wave-uniform over the steps of the segment.
+            // Reverse sweep, wave-uniform over the steps of the segment;
+            // post is the state after step n (push and reflection).
             for (int n = n1 - 1; n >= n0; --n) {
                 const bool live = n < nl;
                 const double cn = a.c[n];
                 double gc = 0.0;
                 if (live) {
                     Part q;
-                    load_state(a.tape + (size_t)(n - n0) * kNS * N, N, i, q);
-                    Part q2 = q;
-                    push_core(PairField{a.Ezp, a.Erp, cn}, a.nz, a.nr, F, q2);
-                    double sR = 1.0, sVr = 1.0, sAr = 1.0;
-                    if (q2.R <= 0.0) {
-                        sR = q2.R < 0.0 ? -1.0 : 1.0; sVr = q2.Vr < 0.0 ? -1.0 : 1.0; sAr = q2.Ar < 0.0 ? -1.0 : 1.0;
-                        q2.R = fabs(q2.R); q2.Vr = fabs(q2.Vr); q2.Ar = fabs(q2.Ar);
-                    }
+                    tape_load(a.tape + (size_t)(n - n0) * kTape * N, N, i, q);
                     // loss of step n on the state after push and reflection
-                    if (in_support(a, q2)) {
+                    if (in_support(a, post)) {
                         double dz, dr;
-                        const double m = mask_at(a, q2.Z, q2.R, dz, dr);
+                        const double m = mask_at(a, post.Z, post.R, dz, dr);
                         const double gl = a.g[n] * al * wq;
-                        l.Z += gl * q2.Vz * dz;
-                        l.R += gl * q2.Vz * dr;
+                        l.Z += gl * post.Vz * dz;
+                        l.R += gl * post.Vz * dr;
                         l.Vz += gl * m;
                     }
-                    l.R *= sR; l.Vr *= sVr; l.Ar *= sAr;
-                    gc = push_adjoint(a, F, cn, q, l);
+                    if (post.R <= 0.0) {
+                        // reflected (R' == 0 exactly): signs of the pushed Vr, Ar
+                        Part q2 = q;
+                        q2.Vz = 0.0; q2.Az = 0.0;
+                        push_core(PairField{a.Ezp, a.Erp, cn}, a.nz, a.nr, F, q2);
+                        if (q2.Vr < 0.0) l.Vr = -l.Vr;
+                        if (q2.Ar < 0.0) l.Ar = -l.Ar;
+                    }
+                    double E2;
+                    gc = push_adjoint(a, F, cn, q, l, E2);
+                    // step back to the state before step n
+                    const double az = post.Az;
+                    post.Az = 2.0 * az - E2;
+                    post.Vz = post.Vz - az - E2;
+                    post.Z = q.Z; post.R = q.R; post.Vr = q.Vr; post.Vt = q.Vt; post.Ar = q.Ar;
                 }
                 if (__ballot(gc != 0.0)) {
                     const double w = wave_sum(gc);
@@ -477,7 +505,7 @@ int ipm_window_create(ipm_ctx *c, int nsteps, int interval, const double *Ez0, c
     HIPCHK(hipMalloc(&w->ck, sizeof(double) * kNS * NN * (a.nseg + 1)));
     HIPCHK(hipMalloc(&w->wq, sizeof(double) * NN));
     HIPCHK(hipMalloc(&w->death, sizeof(int) * NN));
-    HIPCHK(hipMalloc(&w->tape, sizeof(double) * kNS * NN * interval));
+    HIPCHK(hipMalloc(&w->tape, sizeof(double) * kTape * NN * interval));
     HIPCHK(hipMalloc(&w->lam, sizeof(double) * kNS * NN));
     HIPCHK(hipMalloc(&w->ezp, sizeof(Pair2) * (size_t)(nz + 1) * nr));
     HIPCHK(hipMalloc(&w->erp, sizeof(Pair2) * (size_t)nz * (nr + 1)));
@@ -511,6 +539,13 @@ int ipm_window_create(ipm_ctx *c, int nsteps, int interval, const double *Ez0, c
         a.zhi = (imax >= nz - 1) ? INFINITY : (double)imax;
         a.rlo = (jmin <= 2) ? -INFINITY : (double)(jmin - 2);
         a.rhi = (jmax >= nr - 1) ? INFINITY : (double)jmax;
+    }
+    // The skip test costs one extra recompute; it pays when the support is a
+    // small part of the grid (a flux through a few planes), not otherwise.
+    {
+        const double fz = (std::min(a.zhi, (double)(nz - 1)) - std::max(a.zlo, 0.0)) / (double)(nz - 1);
+        const double fr = (std::min(a.rhi, (double)(nr - 1)) - std::max(a.rlo, 0.0)) / (double)(nr - 1);
+        a.prefilter = (imax > 0 && fz * fr < 0.5) ? 1 : 0;
     }
     a.Ezp = w->ezp; a.Erp = w->erp; a.mu = w->mu;
     a.cell_material = c->cell_material; a.mat_type = c->mat_type; a.ndiel = c->h.ndiel;
