@@ -99,6 +99,53 @@ def cpu_baseline(cfg, geom, tables, particles, seconds):
                        f"particles), {nsteps} steps of push+absorb+deposit+MCC in {el:.1f} s, -O3 oracle")
 
 
+def adjoint_leg(reps=5):
+    """The differentiable window on its own workload (129x257 nodes, ~4M
+    particles, 200 collisionless steps, checkpoint every 20): HIP-event time
+    of the forward replay and of the adjoint, and the adjoint gradient against
+    central differences (h = 1e-4 relative) for RF amplitude, phase, Cm1, Cm2.
+    Two losses: the ion flux to the powered electrode (4 node planes) and the
+    same flux weighted over the whole domain (every ion contributes)."""
+    import numpy as np
+    import ipm2d
+    from ipm2d import circuit as ck
+    from ipm2d import workload as W
+    from ipm2d.matching import FluxGradient, electrode_capacitance
+    cfg = W.make_config("c4")
+    geom = W.make_geometry(cfg)
+    rng = np.random.default_rng(cfg.seed)
+    parts = [W.load_species(cfg, geom, s, rng) for s in range(cfg.ns)]
+    hp, _, _, _ = ipm2d.from_config(cfg, geom=geom, particles=parts, capacity_factor=1.2)
+    imn = ck.IMN(Cm1=500e-12, Cm2=200e-12, Rs=50.0, amplitude=200.0, phase_deg=30.0,
+                 eqc=electrode_capacitance(cfg))
+    out = {}
+    for name, layers in (("electrode_flux", 4), ("whole_domain", cfg.nz - 2)):
+        fg = FluxGradient(cfg, hp, geom, parts, imn, T=cfg.steps, interval=20, layers=layers)
+        _, g = fg.gradient()
+        fd = fg.central_fd(rel=1e-4)
+        w, c = fg.window, fg.base_volts
+        f_ms, a_ms = [], []
+        for _ in range(reps):
+            w.forward(c)
+            f_ms.append(w.info()["forward_ms"])
+            w.backward(c, np.ones(cfg.steps))
+            a_ms.append(w.info()["adjoint_ms"])
+        fms, ams = float(np.median(f_ms)), float(np.median(a_ms))
+        npart = w.info()["particles"]
+        out[name] = {"forward_ms": fms, "adjoint_ms": ams, "adjoint_over_forward": ams / fms,
+                     "fd_max_rel": float(((g - fd).abs() / fd.abs()).max()),
+                     "forward_particle_steps_per_s": npart * cfg.steps / (fms * 1e-3),
+                     "plain_equals_window": float(np.abs(fg.base_loss - fg.record_loss).max() /
+                                                  np.abs(fg.base_loss).max())}
+        log(f"[bench] adjoint {name}: forward {fms:.3f} ms, adjoint {ams:.3f} ms, "
+            f"ratio {ams / fms:.2f}, fd rel {out[name]['fd_max_rel']:.2e}")
+        fg.close()
+    hp.close()
+    out["workload"] = (f"c4: {cfg.nz}x{cfg.nr} nodes, {npart} particles (e/CF3+/F-), {cfg.steps} steps, "
+                       f"checkpoint every 20, IMN Rs 50 Ohm Cm1 500 pF Cm2 200 pF, 200 V 13.56 MHz phase 30 deg")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +156,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the graph and plain-launch comparisons")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-window leg")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: a 1 GiB device copy (counter calibration), then 4 steps")
@@ -132,7 +180,7 @@ def main():
         b.copy_(a)                      # reads and writes exactly 2**30 bytes
         torch.cuda.synchronize()
         del a, b
-        args.steps, args.warmup, args.no_cpu_baseline, args.no_extra = 4, 0, True, True
+        args.steps, args.warmup, args.no_cpu_baseline, args.no_extra, args.no_adjoint = 4, 0, True, True, True
 
     cfg = W.make_config(args.config)
     sort_interval = cfg.sort_interval if args.sort_interval < 0 else args.sort_interval
@@ -190,6 +238,10 @@ def main():
         log(f"[bench] graph {extra['graph_value']:.4g}, plain {extra['plain_per_species_value']:.4g}")
     hp.close()
 
+    adj = None
+    if not args.no_adjoint:
+        adj = adjoint_leg()
+
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")
@@ -197,7 +249,7 @@ def main():
         log(f"[bench] cpu {cpu['value']:.4g} particle-steps/s")
 
     out = {
-        "metric": "particle-steps/s (push+deposit+MCC) & % HBM peak",
+        "metric": "particle-steps/s (push+deposit+MCC) & % HBM peak; adjoint/forward time",
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -231,6 +283,8 @@ def main():
         },
         "kernel_ms_per_step": {k: (v[0] / v[1] if v[1] else None) for k, v in tm.items()},
         "cpu_baseline": cpu,
+        "adjoint_over_forward": adj["electrode_flux"]["adjoint_over_forward"] if adj else None,
+        "adjoint": adj,
     }
     out.update(extra)
     print(json.dumps(out), flush=True)
