@@ -174,6 +174,37 @@ module ipm_hip
             integer(c_int), value :: s
             type(c_ptr), intent(inout) :: fields8(8)
         end function
+        integer(c_int) function ipm_moments(ctx, s, implicit_j, rho, energy_ev, mp, jz, jr) &
+                bind(C, name="ipm_moments")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: s, implicit_j
+            real(c_double), intent(inout) :: rho(*), energy_ev(*), mp(*), jz(*), jr(*)
+        end function
+        integer(c_int) function ipm_window_create(ctx, nsteps, interval, Ez0, Er0, Ez1, Er1, mu, alpha) &
+                bind(C, name="ipm_window_create")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nsteps, interval
+            real(c_double), intent(in) :: Ez0(*), Er0(*), Ez1(*), Er1(*), mu(*), alpha(*)
+        end function
+        integer(c_int) function ipm_window_forward(ctx, coef, record, loss) bind(C, name="ipm_window_forward")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(in) :: coef(*)
+            integer(c_int), value :: record
+            real(c_double), intent(inout) :: loss(*)
+        end function
+        integer(c_int) function ipm_window_backward(ctx, coef, gloss, dcoef) bind(C, name="ipm_window_backward")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(in) :: coef(*), gloss(*)
+            real(c_double), intent(inout) :: dcoef(*)
+        end function
+        integer(c_int) function ipm_window_destroy(ctx) bind(C, name="ipm_window_destroy")
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+        end function
         integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
             import :: c_ptr, c_size_t
             type(c_ptr), value :: s
@@ -245,6 +276,17 @@ contains
     subroutine ipm_MCCStep(ctx)
         type(c_ptr), intent(in) :: ctx
         call ipm_check(ipm_mcc(ctx), "MCC")
+    end subroutine
+
+    ! WeightingParticleField for species k: R/src/diag/diagParticleField.F90:173.
+    ! RhoOne, TOne (eV), MPOne, JzOne, JrOne of DiagOne(k), each (1:Nz, 1:Nr);
+    ! the current from a copy pushed once more, as for the implicit scheme.
+    subroutine ipm_WeightingParticleField(ctx, k, RhoOne, TOne, MPOne, JzOne, JrOne)
+        type(c_ptr), intent(in) :: ctx
+        integer, intent(in) :: k
+        real(c_double), intent(inout) :: RhoOne(:, :), TOne(:, :), MPOne(:, :), JzOne(:, :), JrOne(:, :)
+        call ipm_check(ipm_moments(ctx, int(k, c_int), 1_c_int, RhoOne, TOne, MPOne, JzOne, JrOne), &
+                       "WeightingParticleField")
     end subroutine
 
     ! PB(k)%NPar
