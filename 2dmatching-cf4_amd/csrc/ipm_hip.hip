@@ -25,7 +25,7 @@ constexpr int kBlock = 256;          // threads per block (4 wave64)
 constexpr int kItems = 4;            // particles per thread per tile
 constexpr int kTile = kBlock * kItems;
 constexpr int kWinZ = 64;            // LDS deposit window (nodes in z)
-constexpr int kWinR = 4;             // LDS deposit window (nodes in r)
+constexpr int kWinR = 8;             // LDS deposit window (nodes in r)
 
 enum : int { M_PUSH = 1, M_ABSORB = 2, M_DEPOSIT = 4, M_MCC = 8 };
 
@@ -233,6 +233,11 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
     __shared__ double win[kWinZ * kWinR];
     __shared__ int anchor[2];
     __shared__ unsigned long long dead_blk[IPM_NS_MAX];
+    // Tile-local indices of the tile's selected particles, for one queue
+    // append per tile; two buffers, so wave 0 appends tile k while the other
+    // waves already work on tile k+1 (or leave).
+    __shared__ unsigned short q_sidx[(MODE & M_MCC) ? 2 * kTile : 1];
+    __shared__ unsigned q_scount[2];
 
     const int ns = st.ns;
     long long T = 0;
@@ -242,7 +247,9 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
     if (MODE & M_DEPOSIT)
         for (int q = threadIdx.x; q < kWinZ * kWinR; q += kBlock) win[q] = 0.0;
     if (threadIdx.x < IPM_NS_MAX) dead_blk[threadIdx.x] = 0;
+    if (threadIdx.x < 2) q_scount[threadIdx.x] = 0;
     __syncthreads();
+    int qbuf = 0;
 
     const unsigned G = gridDim.x;
     long long t0, t1, tstride, tfirst;
@@ -284,7 +291,7 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
                     anchor[0] = -1000000; anchor[1] = -1000000;
                 } else {
                     anchor[0] = (int)ceil(z0) - 4;
-                    anchor[1] = (int)ceil(r0) - 1;
+                    anchor[1] = (int)ceil(r0) - 3;
                 }
             }
             __syncthreads();
@@ -295,8 +302,6 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
         // Wave w of the block takes kItems consecutive runs of 64 slots, so a
         // wave walks a contiguous, cell-sorted stretch of the tile.
         const int wv = threadIdx.x >> 6;
-        // Software pipeline: the loads of item k+1 are issued before item k
-        // is processed, so two sets of eight loads are in flight per wave.
         auto load_item = [&](long long i, Part &q) {
             q.Z = gp(F[0])[i];
             if (MODE & (M_PUSH | M_ABSORB | M_DEPOSIT)) {
@@ -308,20 +313,12 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
                 q.Az = gp(F[5])[i]; q.Ar = gp(F[6])[i];
             }
         };
-        Part nxt;
-        {
-            const long long i0 = base + (long long)(wv * kItems) * 64 + lane;
-            if (i0 < n_s) load_item(i0, nxt);
-        }
 #pragma unroll 1
         for (int k = 0; k < kItems; ++k) {
             const long long i = base + (long long)(wv * kItems + k) * 64 + lane;
             const bool valid = i < n_s;
-            Part p = nxt;
-            if (k + 1 < kItems) {
-                const long long i1 = i + 64;
-                if (i1 < n_s) load_item(i1, nxt);
-            }
+            Part p;
+            if (valid) load_item(i, p);
             bool live = false, deleted = false;
             if (valid) live = !is_dead(p.Z);
             if (live) {
@@ -391,33 +388,17 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
                 }
             }
             if (MODE & M_MCC) {
+                // note the tile-local index (one LDS atomic per wave); the
+                // tile's selections go to the queue together after the loop
                 const unsigned long long m = __ballot(selected);
                 if (m) {
                     const int leader = __ffsll((long long)m) - 1;
-                    // partition by tile and wave: consecutive tiles cycle through
-                    // every partition, so each fills evenly whatever the species mix
-                    const unsigned part = (unsigned)(t * (kBlock / 64) + wv) & (kQueueParts - 1);
-                    unsigned qb = 0;
-                    if (lane == leader)
-                        qb = atomicAdd(&st.q_part_count[part * kQueueStride], (unsigned)__popcll(m));
-                    qb = __shfl(qb, leader);
-                    if (selected) {
-                        const unsigned long long qj = (unsigned long long)qb + __popcll(m & ((1ull << lane) - 1ull));
-                        const unsigned long long q = (unsigned long long)part * st.q_part_cap + qj;
-                        if (qj < (unsigned long long)st.q_part_cap) {
-                            if (!(MODE & (M_PUSH | M_ABSORB | M_DEPOSIT))) p.WQ = gp(F[7])[i];
-                            if (!(MODE & (M_PUSH | M_ABSORB))) {
-                                p.Az = gp(F[5])[i]; p.Ar = gp(F[6])[i];
-                            }
-                            gp(st.q_f[0])[q] = p.Z; gp(st.q_f[1])[q] = p.R; gp(st.q_f[2])[q] = p.Vz;
-                            gp(st.q_f[3])[q] = p.Vr; gp(st.q_f[4])[q] = p.Vt; gp(st.q_f[5])[q] = p.Az;
-                            gp(st.q_f[6])[q] = p.Ar; gp(st.q_f[7])[q] = p.WQ;
-                            gp(st.q_slot)[q] = i;
-                            gp(st.q_species)[q] = s;
-                        } else {
-                            atomicOr(&ctr->overflow, 4);
-                        }
-                    }
+                    unsigned sb = 0;
+                    if (lane == leader) sb = atomicAdd(&q_scount[qbuf], (unsigned)__popcll(m));
+                    sb = __shfl(sb, leader);
+                    if (selected)
+                        q_sidx[qbuf * kTile + sb + (unsigned)__popcll(m & ((1ull << lane) - 1ull))] =
+                            (unsigned short)(i - base);
                 }
             }
 
@@ -450,10 +431,42 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
             }
             __syncthreads();
         }
+        if (MODE & M_MCC) {
+            // Append the tile's selections with their post-push state, read
+            // back from the particle arrays this block just wrote (L2-hot):
+            // one queue atomic per tile, then contiguous queue stores, all by
+            // wave 0 so the other waves do not wait for the round trips.
+            // Partition = tile index mod kQueueParts, so consecutive tiles
+            // cycle through every partition.
+            __syncthreads();
+            if (wv == 0) {
+                const unsigned n = q_scount[qbuf];
+                const unsigned part = (unsigned)t & (kQueueParts - 1);
+                unsigned qb = 0;
+                if (lane == 0 && n) qb = atomicAdd(&st.q_part_count[part * kQueueStride], n);
+                qb = __shfl(qb, 0);
+                for (unsigned k = lane; k < n; k += 64) {
+                    const unsigned long long qj = (unsigned long long)qb + k;
+                    const long long i = base + q_sidx[qbuf * kTile + k];
+                    if (qj < (unsigned long long)st.q_part_cap) {
+                        const unsigned long long q = (unsigned long long)part * st.q_part_cap + qj;
+                        for (int f = 0; f < 8; ++f) gp(st.q_f[f])[q] = gp(F[f])[i];
+                        gp(st.q_slot)[q] = i;
+                        gp(st.q_species)[q] = s;
+                    } else {
+                        atomicOr(&ctr->overflow, 4);
+                    }
+                }
+                if (lane == 0) q_scount[qbuf] = 0;   // reused two tiles later, after a barrier
+            }
+            qbuf ^= 1;
+        }
     }
 
     if (MODE & M_ABSORB) {
-        __syncthreads();
+        // The last tile's barrier (deposit flush or queue append) already
+        // ordered every wave's dead_blk update before this read.
+        if (!(MODE & (M_DEPOSIT | M_MCC))) __syncthreads();
         if (threadIdx.x < (unsigned)ns && dead_blk[threadIdx.x])
             atomicAdd(&ctr->n_dead[threadIdx.x], dead_blk[threadIdx.x]);
     }

@@ -96,7 +96,9 @@ def test_absorption_matches_oracle(oracle, rng):
         np.testing.assert_array_equal(sort_set(got), sort_set(ref))
     assert (np.abs(sg_ref) > 0).sum() > 10, "dielectric edges were not exercised"
     assert_close(mc, mc_ref, rtol=1e-12, what="metal charge")
-    assert_close(sg, sg_ref, rtol=1e-12, atol=1e-30, what="sigma")
+    # sigma and metal charge add both species' charges: opposite signs cancel,
+    # so the atomic summation order shows at 1e-12 of the magnitude, not of the net
+    assert_close(sg, sg_ref, rtol=1e-12, atol=1e-12 * np.abs(sg_ref).max(), what="sigma")
     # tallies are cleared after being taken
     mc2, sg2 = hp.take_tallies()
     assert not mc2.any() and not sg2.any()
@@ -220,8 +222,8 @@ def test_fused_step_equals_separate_calls(rng):
         assert_close(sort_set(ga), sort_set(gb), rtol=1e-14, atol=1e-300, what=f"species {s}")
     ma, sa = a.take_tallies()
     mb, sb = b.take_tallies()
-    assert_close(ma, mb, rtol=1e-12, what="metal charge")
-    assert_close(sa, sb, rtol=1e-12, atol=1e-40, what="sigma")
+    assert_close(ma, mb, rtol=1e-12, atol=1e-12 * np.abs(ma).max(), what="metal charge")
+    assert_close(sa, sb, rtol=1e-12, atol=1e-12 * np.abs(sa).max(), what="sigma")
 
 
 def test_full_step_matches_oracle_without_losses(oracle, rng):
