@@ -58,11 +58,9 @@ struct ipm_ctx {
     unsigned long long *hist = nullptr, *offs = nullptr, *cursor = nullptr, *partial = nullptr,
                        *partial2 = nullptr;
     double **d_dst = nullptr;
-    double *qbuf = nullptr;
-    long long *q_slot = nullptr;
-    int *q_species = nullptr;
-    long long q_cap = 0;
-    unsigned *q_part_count = nullptr;
+    double *qbuf = nullptr;                // q_cap records of 8 doubles
+    unsigned *q_tile_count = nullptr;      // q_ntiles
+    long long q_cap = 0, q_ntiles = 0;
     double *mom = nullptr;                 // moment sums, 5 * nz * nr (ipm_moments)
     hipGraphExec_t graph = nullptr;
     int block = 256, blocks_per_cu = 0;

@@ -55,22 +55,20 @@ struct DevState {
     unsigned long long *mcc_counts;    // optional: (s*ng+g)*128 + j
     int count_events;
     int pad;
-    // Queue of particles selected for a collision (null-collision pass),
-    // split into kQueueParts partitions of q_part_cap entries; partition p
-    // holds entries [p*q_part_cap, p*q_part_cap + count[p]).  A wave always
-    // appends to the partition of its global wave index, so the append
-    // atomics spread over kQueueParts counters (one 128-byte line each).
-    double *q_f[8];
-    long long *q_slot;
-    int *q_species;
-    long long q_cap;
-    long long q_part_cap;
-    unsigned int *q_part_count;        // kQueueParts * kQueueStride
+    // Queue of particles selected for a collision (null-collision pass).
+    // Species s owns q_maxtiles[s] = ceil(cap/1024) tile regions of
+    // q_tile_cap[s] entries; entry k of live tile lt of species s sits at
+    // q_region[s] + lt*q_tile_cap[s] + k and holds the particle's state and
+    // its index inside the tile.  q_tile_count[q_cbase[s] + lt] is that
+    // tile's number of entries in the current pass.  The layout needs no
+    // global counter: a tile appends with an LDS counter only.  Entries are
+    // 64-byte records (Z R Vz Vr Vt Az Ar, index in tile), written and read whole.
+    double *q_rec;
+    unsigned int *q_tile_count;
+    long long q_region[IPM_NS_MAX], q_cbase[IPM_NS_MAX], q_tile_cap[IPM_NS_MAX];
+    double q_weight[IPM_NS_MAX];       // expected entries per tile (collision-pass split)
     DevTable tab[IPM_NS_MAX * IPM_NG_MAX];
 };
-
-constexpr int kQueueParts = 512;
-constexpr int kQueueStride = 32;       // u32 per counter: one 128-byte line
 
 // Mutable per-step counters (device memory).
 struct DevCounters {

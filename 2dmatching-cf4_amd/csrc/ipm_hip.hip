@@ -29,6 +29,18 @@ constexpr int kWinR = 8;             // LDS deposit window (nodes in r)
 
 enum : int { M_PUSH = 1, M_ABSORB = 2, M_DEPOSIT = 4, M_MCC = 8 };
 
+// A collision-queue record: Z R Vz Vr Vt Az Ar and the particle's index in
+// its tile, 64 contiguous bytes moved as four 16-byte accesses (WQ is read
+// from the particle arrays when the record is used).
+typedef double rec2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ unsigned load_rec(const double *r, Part &p)
+{
+    const auto *v = (const __attribute__((address_space(1))) rec2 *)r;
+    const rec2 a = v[0], b = v[1], c = v[2], d = v[3];
+    p.Z = a.x; p.R = a.y; p.Vz = b.x; p.Vr = b.y; p.Vt = c.x; p.Az = c.y; p.Ar = d.x;
+    return (unsigned)__double_as_longlong(d.y);
+}
+
 __device__ __forceinline__ bool is_dead(double z) { return z < -1.0e299; }
 
 // ------------------------------------------------------------ products
@@ -233,11 +245,11 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
     __shared__ double win[kWinZ * kWinR];
     __shared__ int anchor[2];
     __shared__ unsigned long long dead_blk[IPM_NS_MAX];
-    // Tile-local indices of the tile's selected particles, for one queue
-    // append per tile; two buffers, so wave 0 appends tile k while the other
-    // waves already work on tile k+1 (or leave).
-    __shared__ unsigned short q_sidx[(MODE & M_MCC) ? 2 * kTile : 1];
+    // Entries of the tile in the collision queue so far (two counters so
+    // the next tile never waits for the reset), and each wave's staging
+    // buffer of up to 64 queue records, written out with full waves.
     __shared__ unsigned q_scount[2];
+    __shared__ double q_wst[(MODE & M_MCC) ? (kBlock / 64) * 64 * 8 : 1];
 
     const int ns = st.ns;
     long long T = 0;
@@ -312,6 +324,28 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
                 q.Vz = gp(F[2])[i]; q.Vr = gp(F[3])[i]; q.Vt = gp(F[4])[i];
                 q.Az = gp(F[5])[i]; q.Ar = gp(F[6])[i];
             }
+        };
+        // Queue records staged by this wave (wave-uniform count).  A flush
+        // reserves space in the tile's region with one LDS atomic and writes
+        // the records with every lane storing 16 contiguous bytes.
+        unsigned cw = 0;
+        auto flush_wave = [&]() {
+            unsigned ob = 0;
+            if (lane == 0) ob = atomicAdd(&q_scount[qbuf], cw);
+            ob = __shfl(ob, 0);
+            const long long R = st.q_tile_cap[s];
+            double *qt = st.q_rec + (size_t)(st.q_region[s] + (t - tbeg) * R) * 8;
+            const double *wst = q_wst + wv * 512;
+            for (unsigned j = lane; j < cw * 4u; j += 64u) {
+                const unsigned rec = j >> 2, part = j & 3u;
+                if ((long long)(ob + rec) < R) {
+                    const rec2 v = {wst[rec * 8 + part * 2], wst[rec * 8 + part * 2 + 1]};
+                    *(__attribute__((address_space(1))) rec2 *)(qt + (size_t)(ob + rec) * 8 + part * 2) = v;
+                } else if (part == 0u) {
+                    atomicOr(&ctr->overflow, 4);
+                }
+            }
+            cw = 0;
         };
 #pragma unroll 1
         for (int k = 0; k < kItems; ++k) {
@@ -388,17 +422,22 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
                 }
             }
             if (MODE & M_MCC) {
-                // note the tile-local index (one LDS atomic per wave); the
-                // tile's selections go to the queue together after the loop
+                // stage the wave's selections as queue records in LDS
                 const unsigned long long m = __ballot(selected);
-                if (m) {
-                    const int leader = __ffsll((long long)m) - 1;
-                    unsigned sb = 0;
-                    if (lane == leader) sb = atomicAdd(&q_scount[qbuf], (unsigned)__popcll(m));
-                    sb = __shfl(sb, leader);
-                    if (selected)
-                        q_sidx[qbuf * kTile + sb + (unsigned)__popcll(m & ((1ull << lane) - 1ull))] =
-                            (unsigned short)(i - base);
+                const unsigned nsel = (unsigned)__popcll(m);
+                if (nsel) {
+                    if (cw + nsel > 64u) flush_wave();
+                    if (selected) {
+                        if (!(MODE & (M_PUSH | M_ABSORB))) {
+                            p.Az = gp(F[5])[i]; p.Ar = gp(F[6])[i];
+                        }
+                        const unsigned r = cw + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+                        double *d = q_wst + wv * 512 + r * 8;
+                        d[0] = p.Z; d[1] = p.R; d[2] = p.Vz; d[3] = p.Vr;
+                        d[4] = p.Vt; d[5] = p.Az; d[6] = p.Ar;
+                        d[7] = __longlong_as_double((long long)(i - base));
+                    }
+                    cw += nsel;
                 }
             }
 
@@ -419,6 +458,9 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
             }
         }
 
+        if (MODE & M_MCC) {
+            if (cw) flush_wave();
+        }
         if (MODE & M_DEPOSIT) {
             __syncthreads();
             for (int q = threadIdx.x; q < kWinZ * kWinR; q += kBlock) {
@@ -432,32 +474,14 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
             __syncthreads();
         }
         if (MODE & M_MCC) {
-            // Append the tile's selections with their post-push state, read
-            // back from the particle arrays this block just wrote (L2-hot):
-            // one queue atomic per tile, then contiguous queue stores, all by
-            // wave 0 so the other waves do not wait for the round trips.
-            // Partition = tile index mod kQueueParts, so consecutive tiles
-            // cycle through every partition.
-            __syncthreads();
-            if (wv == 0) {
+            // publish the tile's entry count (the deposit flush above already
+            // ordered every wave's LDS counter update)
+            if (!(MODE & M_DEPOSIT)) __syncthreads();
+            if (threadIdx.x == 0) {
                 const unsigned n = q_scount[qbuf];
-                const unsigned part = (unsigned)t & (kQueueParts - 1);
-                unsigned qb = 0;
-                if (lane == 0 && n) qb = atomicAdd(&st.q_part_count[part * kQueueStride], n);
-                qb = __shfl(qb, 0);
-                for (unsigned k = lane; k < n; k += 64) {
-                    const unsigned long long qj = (unsigned long long)qb + k;
-                    const long long i = base + q_sidx[qbuf * kTile + k];
-                    if (qj < (unsigned long long)st.q_part_cap) {
-                        const unsigned long long q = (unsigned long long)part * st.q_part_cap + qj;
-                        for (int f = 0; f < 8; ++f) gp(st.q_f[f])[q] = gp(F[f])[i];
-                        gp(st.q_slot)[q] = i;
-                        gp(st.q_species)[q] = s;
-                    } else {
-                        atomicOr(&ctr->overflow, 4);
-                    }
-                }
-                if (lane == 0) q_scount[qbuf] = 0;   // reused two tiles later, after a barrier
+                const long long R = st.q_tile_cap[s];
+                st.q_tile_count[st.q_cbase[s] + (t - tbeg)] = (long long)n < R ? n : (unsigned)R;
+                q_scount[qbuf] = 0;     // reused two tiles later, after a barrier
             }
             qbuf ^= 1;
         }
@@ -478,74 +502,129 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
 // is repeated and, if it hits, the collision of MCCPublic.F90:61-97 follows
 // on the state the previous gas left.  Results go back to the particle's
 // slot; products are appended to their species (MCCAddNewParticle,
-// R/src/mc/MCCPublic.F90:209-226).  Launched with a fixed grid that loops
-// over the queue length read from device memory, so it can be captured.
-__global__ void __launch_bounds__(kBlock)
+// R/src/mc/MCCPublic.F90:209-226).  The grid is split over the species in
+// proportion to their expected work (every species with live tiles gets a
+// block); a block takes a contiguous range of its species' tiles, scans
+// their entry counts in LDS and processes the entries flat.  Launched with a
+// fixed grid, so it can be captured.
+constexpr int kChunk = 1024;   // tile counts scanned per LDS pass
+
+__global__ void __launch_bounds__(kBlock, 2)
 collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
 {
     const DevState &st = *stp;
-    // exclusive scan of the partition fill levels, in LDS
-    __shared__ unsigned int pre[kQueueParts + 1];
-    __shared__ unsigned int wtot[kBlock / 64];
+    __shared__ unsigned cpre[kChunk + 1];
+    __shared__ unsigned wtot[kBlock / 64];
     __shared__ int last_block;
+    const int ns = st.ns;
+    const unsigned G = gridDim.x;
+
+    // species of this block and its tile range
+    int sb = -1;
+    long long t0 = 0, t1 = 0;
     {
-        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-        constexpr int per = kQueueParts / kBlock;       // 2 partitions per thread
-        unsigned c[per];
-        unsigned tsum = 0;
-        for (int j = 0; j < per; ++j) {
-            const unsigned v = st.q_part_count[(threadIdx.x * per + j) * kQueueStride];
-            c[j] = v < (unsigned long long)st.q_part_cap ? v : (unsigned)st.q_part_cap;
-            tsum += c[j];
+        double W = 0.0;
+        int A = 0;
+        for (int s = 0; s < ns; ++s) {
+            const long long nt = ((long long)ctr->n[s] + kTile - 1) / kTile;
+            if (nt > 0 && st.q_weight[s] > 0.0) { W += (double)nt * st.q_weight[s]; ++A; }
         }
-        unsigned x = tsum;
-        for (int o = 1; o < 64; o <<= 1) {
-            const unsigned y = __shfl_up(x, o);
-            if (lane >= o) x += y;
+        if (A > 0 && G >= (unsigned)A) {
+            const double Grem = (double)(G - A);
+            double Wlt = 0.0;
+            long long B = 0;
+            for (int s = 0; s < ns; ++s) {
+                const long long nt = ((long long)ctr->n[s] + kTile - 1) / kTile;
+                if (!(nt > 0 && st.q_weight[s] > 0.0)) continue;
+                const double Wle = Wlt + (double)nt * st.q_weight[s];
+                const long long nb = 1 + (long long)floor(Grem * Wle / W) - (long long)floor(Grem * Wlt / W);
+                if ((long long)blockIdx.x >= B && (long long)blockIdx.x < B + nb) {
+                    const long long lb = (long long)blockIdx.x - B;
+                    sb = s;
+                    t0 = nt * lb / nb;
+                    t1 = nt * (lb + 1) / nb;
+                }
+                B += nb;
+                Wlt = Wle;
+            }
         }
-        if (lane == 63) wtot[w] = x;
-        __syncthreads();
-        unsigned before = 0;
-        for (int k = 0; k < w; ++k) before += wtot[k];
-        unsigned run = before + x - tsum;
-        for (int j = 0; j < per; ++j) {
-            pre[threadIdx.x * per + j] = run;
-            run += c[j];
-        }
-        if (threadIdx.x == kBlock - 1) pre[kQueueParts] = run;
-        __syncthreads();
     }
-    const unsigned long long nq = pre[kQueueParts];
     const uint32_t step = ctr->step;
     const unsigned long long seed = ctr->seed;
-    const unsigned long long stride = (unsigned long long)gridDim.x * kBlock;
-    for (unsigned long long e = (unsigned long long)blockIdx.x * kBlock + threadIdx.x; e < nq; e += stride) {
-        // partition holding flat entry e: last p with pre[p] <= e
-        int lo = 0, hi = kQueueParts;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (pre[mid] <= e) lo = mid;
-            else hi = mid;
-        }
-        const unsigned long long q = (unsigned long long)lo * st.q_part_cap + (e - pre[lo]);
-        Part p;
-        p.Z = st.q_f[0][q]; p.R = st.q_f[1][q]; p.Vz = st.q_f[2][q]; p.Vr = st.q_f[3][q];
-        p.Vt = st.q_f[4][q]; p.Az = st.q_f[5][q]; p.Ar = st.q_f[6][q]; p.WQ = st.q_f[7][q];
-        const long long i = st.q_slot[q];
-        const int s = st.q_species[q];
-        const StateKey sk = state_key(p.Z, p.R, p.Vz, p.Vr, p.Vt);
-        for (int g = 0; g < st.ng; ++g) {
-            const DevTable &tb = st.tab[s * st.ng + g];
-            if (tb.model == 0) continue;
-            Rng rg = make_rng(sk, seed, step, (uint32_t)s | ((uint32_t)g << 8));
-            if (!(rg.next() < tb.ratio)) continue;
-            mcc_collide(st, ctr, s, g, tb, p, rg);
-        }
+    if (sb >= 0) {
+        const int s = sb;
+        const long long R = st.q_tile_cap[s];
+        const unsigned *cnt = st.q_tile_count + st.q_cbase[s];
         double *const *F = st.f[s];
-        F[2][i] = p.Vz; F[3][i] = p.Vr; F[4][i] = p.Vt; F[5][i] = p.Az; F[6][i] = p.Ar;
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        for (long long tc = t0; tc < t1; tc += kChunk) {
+            const int m = (int)min((long long)kChunk, t1 - tc);
+            // exclusive scan of the chunk's tile counts
+            constexpr int per = kChunk / kBlock;
+            unsigned c[per];
+            unsigned tsum = 0;
+            for (int j = 0; j < per; ++j) {
+                const int tt = threadIdx.x * per + j;
+                c[j] = tt < m ? cnt[tc + tt] : 0u;
+                tsum += c[j];
+            }
+            unsigned x = tsum;
+            for (int o = 1; o < 64; o <<= 1) {
+                const unsigned y = __shfl_up(x, o);
+                if (lane >= o) x += y;
+            }
+            if (lane == 63) wtot[w] = x;
+            __syncthreads();
+            unsigned before = 0;
+            for (int k = 0; k < w; ++k) before += wtot[k];
+            unsigned run = before + x - tsum;
+            for (int j = 0; j < per; ++j) {
+                cpre[threadIdx.x * per + j] = run;
+                run += c[j];
+            }
+            if (threadIdx.x == kBlock - 1) cpre[kChunk] = run;
+            __syncthreads();
+            const unsigned total = cpre[kChunk];
+            for (unsigned e = threadIdx.x; e < total; e += kBlock) {
+                // tile of entry e: last j with cpre[j] <= e
+                int lo = 0, hi = m;
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (cpre[mid] <= e) lo = mid;
+                    else hi = mid;
+                }
+                const long long lt = tc + lo;
+                const long long q = st.q_region[s] + lt * R + (e - cpre[lo]);
+                Part p;
+                const long long i = lt * kTile + load_rec(st.q_rec + (size_t)q * 8, p);
+                p.WQ = F[7][i];
+                const StateKey sk = state_key(p.Z, p.R, p.Vz, p.Vr, p.Vt);
+                const Part p0 = p;
+                for (int g = 0; g < st.ng; ++g) {
+                    const DevTable &tb = st.tab[s * st.ng + g];
+                    if (tb.model == 0) continue;
+                    Rng rg = make_rng(sk, seed, step, (uint32_t)s | ((uint32_t)g << 8));
+                    if (!(rg.next() < tb.ratio)) continue;
+                    mcc_collide(st, ctr, s, g, tb, p, rg);
+                }
+                // write back only what the collisions changed (null events
+                // change nothing; Az/Ar change only in charge exchange): each
+                // field is a separate cache line of a separate array
+                if (__double_as_longlong(p.Vz) != __double_as_longlong(p0.Vz) ||
+                    __double_as_longlong(p.Vr) != __double_as_longlong(p0.Vr) ||
+                    __double_as_longlong(p.Vt) != __double_as_longlong(p0.Vt)) {
+                    F[2][i] = p.Vz; F[3][i] = p.Vr; F[4][i] = p.Vt;
+                }
+                if (__double_as_longlong(p.Az) != __double_as_longlong(p0.Az) ||
+                    __double_as_longlong(p.Ar) != __double_as_longlong(p0.Ar)) {
+                    F[5][i] = p.Az; F[6][i] = p.Ar;
+                }
+            }
+            __syncthreads();
+        }
     }
-    // The last block to finish folds this pass's products into n[], empties
-    // the queue partitions and advances the random stream's step key.
+    // The last block to finish folds this pass's products into n[] and
+    // advances the random stream's step key.
     __syncthreads();
     if (threadIdx.x == 0) {
         __threadfence();
@@ -555,7 +634,6 @@ collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
     __syncthreads();
     if (last_block) {
         __threadfence();
-        for (int p = threadIdx.x; p < kQueueParts; p += kBlock) st.q_part_count[p * kQueueStride] = 0;
         if (threadIdx.x == 0) {
             for (int s = 0; s < st.ns; ++s) {
                 unsigned long long add = atomicAdd(&ctr->n_new[s], 0ull);
@@ -817,46 +895,51 @@ __global__ void set_count_kernel(DevCounters *ctr, int s, const unsigned long lo
 // ====================================================================
 #include "ipm_ctx.hpp"
 
-// Collision queue sized for the expected selections with a wide margin:
-// capacity_s * min(1, 2 * sum_g CollisionRatio) + 64k entries.
+// Collision-queue regions: species s gets ceil(cap_s/1024) tile regions of
+// R_s = min(1024, ceil(1024 * min(1, 2 * sum_g CollisionRatio)) + 64)
+// entries, at least 4 standard deviations above the binomial mean of a
+// tile's selections for any ratio (a larger count stops the step with
+// IPM_ECAPACITY).  Species without MCC tables get none.
 static int ensure_queue(ipm_ctx *c)
 {
-    long long need = 65536;
+    long long R[IPM_NS_MAX] = {}, nt[IPM_NS_MAX] = {};
+    long long need = 0, ntiles = 0;
     for (int s = 0; s < c->ns; ++s) {
-        if (!c->species_set[s]) continue;
         double r = 0.0;
+        bool any = false;
         for (int g = 0; g < c->ng; ++g)
-            if (c->h.tab[s * c->ng + g].model != 0) r += c->h.tab[s * c->ng + g].ratio;
-        double f = std::min(1.0, 2.0 * r);
-        need += (long long)((double)c->h.cap[s] * f) + 1;
+            if (c->h.tab[s * c->ng + g].model != 0) { r += c->h.tab[s * c->ng + g].ratio; any = true; }
+        nt[s] = c->species_set[s] ? (c->h.cap[s] + kTile - 1) / kTile : 0;
+        if (any) {
+            const double f = std::min(1.0, 2.0 * r);
+            R[s] = std::min<long long>(kTile, (long long)ceil((double)kTile * f) + 64);
+        }
+        c->h.q_weight[s] = any ? (double)kTile * std::min(1.0, r) + 1e-9 : 0.0;
+        c->h.q_tile_cap[s] = R[s];
+        c->h.q_region[s] = need;
+        c->h.q_cbase[s] = ntiles;
+        need += nt[s] * R[s];
+        ntiles += nt[s];
     }
-    if (need <= c->q_cap) return IPM_OK;
-    HIPCHK(hipStreamSynchronize(c->stream));
-    if (c->qbuf) HIPCHK(hipFree(c->qbuf));
-    if (c->q_slot) HIPCHK(hipFree(c->q_slot));
-    if (c->q_species) HIPCHK(hipFree(c->q_species));
-    c->qbuf = nullptr; c->q_slot = nullptr; c->q_species = nullptr; c->q_cap = 0;
-    // every partition gets an equal share plus a margin for uneven filling
-    const size_t part_cap = (((size_t)need / kQueueParts) * 5 / 4 + 1024 + 31) & ~(size_t)31;
-    const size_t cap = part_cap * kQueueParts;
-    HIPCHK(hipMalloc(&c->qbuf, sizeof(double) * 8 * cap));
-    HIPCHK(hipMalloc(&c->q_slot, sizeof(long long) * cap));
-    HIPCHK(hipMalloc(&c->q_species, sizeof(int) * cap));
-    c->q_cap = (long long)cap;
-    for (int f = 0; f < 8; ++f) c->h.q_f[f] = c->qbuf + (size_t)f * cap;
-    c->h.q_slot = c->q_slot;
-    c->h.q_species = c->q_species;
-    c->h.q_cap = c->q_cap;
-    c->h.q_part_cap = (long long)part_cap;
-    if (!c->q_part_count) {
-        HIPCHK(hipMalloc(&c->q_part_count, sizeof(unsigned) * kQueueParts * kQueueStride));
-        HIPCHK(hipMemset(c->q_part_count, 0, sizeof(unsigned) * kQueueParts * kQueueStride));
+    need = std::max<long long>(need, 1);
+    ntiles = std::max<long long>(ntiles, 1);
+    if (need > c->q_cap || ntiles > c->q_ntiles) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (c->qbuf) HIPCHK(hipFree(c->qbuf));
+        if (c->q_tile_count) HIPCHK(hipFree(c->q_tile_count));
+        c->qbuf = nullptr; c->q_tile_count = nullptr; c->q_cap = 0; c->q_ntiles = 0;
+        HIPCHK(hipMalloc(&c->qbuf, sizeof(double) * 8 * (size_t)need));
+        HIPCHK(hipMalloc(&c->q_tile_count, sizeof(unsigned) * (size_t)ntiles));
+        HIPCHK(hipMemset(c->q_tile_count, 0, sizeof(unsigned) * (size_t)ntiles));
+        c->q_cap = need;
+        c->q_ntiles = ntiles;
+        if (c->graph) {
+            (void)hipGraphExecDestroy(c->graph);
+            c->graph = nullptr;
+        }
     }
-    c->h.q_part_count = c->q_part_count;
-    if (c->graph) {
-        (void)hipGraphExecDestroy(c->graph);
-        c->graph = nullptr;
-    }
+    c->h.q_rec = c->qbuf;
+    c->h.q_tile_count = c->q_tile_count;
     return IPM_OK;
 }
 
@@ -983,8 +1066,8 @@ int ipm_destroy(ipm_ctx *c)
     void *ptrs[] = {c->d_state, c->d_ctr, c->raw_all, c->rho_one, c->chi_one, c->rho_tot, c->chi_tot,
                     c->d_chifac, c->inv_vol, c->cell_material, c->hedge, c->vedge, c->mat_type,
                     c->metal_charge, c->sigma, c->Ez, c->Er, c->mcc_counts, c->hist, c->offs,
-                    c->cursor, c->partial, c->partial2, c->d_dst, c->qbuf, c->q_slot, c->q_species,
-                    c->q_part_count, c->mom};
+                    c->cursor, c->partial, c->partial2, c->d_dst, c->qbuf, c->q_tile_count,
+                    c->mom};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -53,13 +53,16 @@ def timed_steps(hp, cfg, steps, sort_interval, graph=False):
 
 
 def plain_steps(hp, cfg, steps):
-    """Per-species launches in the reference's order, each call synchronous."""
+    """Per-species launches in the reference's call order (OneStep,
+    R/src/onestep.F90:141-177: MoveES and MaterialAborptionParticle per
+    species, P2CES per species, FieldSource%Sum, MCC), each call synchronous."""
     hp.synchronize()
     n0 = live_total(hp, cfg.ns)
     t0 = time.perf_counter()
     for _ in range(steps):
         for s in range(cfg.ns):
-            hp.move_absorb(s)
+            hp.move_es(s)
+            hp.material_absorption(s)
         for s in range(cfg.ns):
             hp.p2ces(s, copy=False)
         hp.field_source_sum(copy=False)
