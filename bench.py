@@ -73,21 +73,17 @@ def plain_steps(hp, cfg, steps):
     return t1 - t0, 0.5 * (n0 + n1) * steps
 
 
-def cpu_baseline(cfg, geom, tables, particles, seconds):
-    """Oracle (statement-by-statement restatement of the reference routines)
-    on one host core over a bounded sample of the same workload."""
+_SHARE = None   # (cfg, geom, tables, particles, ...) for forked oracle workers
+
+
+def _oracle_run(cfg, geom, tables, sample, ez, er, seconds, max_steps):
+    """Advance `sample` with the oracle for `seconds` (or max_steps)."""
     from oracle import ipm_oracle as O
     from ipm2d import workload as W
-    total = sum(P.shape[1] for P in particles)
-    target = 2_000_000
-    stride = max(1, total // target)
-    sample = [P[:, ::stride].copy() for P in particles]
     species = [W.species_params(cfg, s) for s in range(cfg.ns)]
     setup = O.MccSetup(cfg.dx, cfg.dt, [q for q, _ in species], [m for _, m in species], tables)
-    ez, er = W.fields(cfg, 0.0)
     st = O.StepState(geom, setup, sample, ez, er, extra=1.0, fast=True)
-    psteps = 0
-    nsteps = 0
+    psteps, nsteps = 0, 0
     t0 = time.perf_counter()
     while True:
         n = int(st.n.sum())
@@ -95,11 +91,46 @@ def cpu_baseline(cfg, geom, tables, particles, seconds):
         psteps += n
         nsteps += 1
         el = time.perf_counter() - t0
-        if el >= seconds or nsteps >= 100:
-            break
-    return dict(value=psteps / el, unit="particle-steps/s", cores=1, kind="port",
-                sample=f"every {stride}th particle of the same workload ({sum(P.shape[1] for P in sample)} "
-                       f"particles), {nsteps} steps of push+absorb+deposit+MCC in {el:.1f} s, -O3 oracle")
+        if el >= seconds or nsteps >= max_steps:
+            return psteps, nsteps, el
+
+
+def _oracle_worker(w):
+    cfg, geom, tables, particles, ez, er, seconds, workers = _SHARE
+    return _oracle_run(cfg, geom, tables, [P[:, w::workers].copy() for P in particles], ez, er, seconds, 1000)
+
+
+def cpu_baseline(cfg, geom, tables, particles, seconds, workers):
+    """The oracle (statement-by-statement restatement of the reference
+    routines) on the host: one core on a ~2M-particle sample, then `workers`
+    processes each owning an interleaved 1/workers share of the whole
+    workload (the reference's MPI ranks each own their particles; the
+    per-step grid reduction between ranks is not included).  Runs before any
+    GPU initialisation, so the workers can be forked."""
+    global _SHARE
+    import multiprocessing as mp
+    from ipm2d import workload as W
+    total = sum(P.shape[1] for P in particles)
+    ez, er = W.fields(cfg, 0.0)
+    stride = max(1, total // 2_000_000)
+    sample = [P[:, ::stride].copy() for P in particles]
+    ps1, n1, el1 = _oracle_run(cfg, geom, tables, sample, ez, er, seconds, 100)
+    out = dict(value=ps1 / el1, unit="particle-steps/s", cores=1, kind="port",
+               sample=f"every {stride}th particle of the same workload ({sum(P.shape[1] for P in sample)} "
+                      f"particles), {n1} steps of push+absorb+deposit+MCC in {el1:.1f} s, -O3 oracle, one core")
+    if workers > 1:
+        _SHARE = (cfg, geom, tables, particles, ez, er, seconds, workers)
+        with mp.get_context("fork").Pool(workers) as pool:
+            res = pool.map(_oracle_worker, range(workers))
+        _SHARE = None
+        ps = sum(r[0] for r in res)
+        el = max(r[2] for r in res)
+        out = dict(value=ps / el, unit="particle-steps/s", cores=workers, kind="port",
+                   sample=f"the whole workload ({total} particles) split over {workers} processes, "
+                          f"{min(r[1] for r in res)}-{max(r[1] for r in res)} steps each of "
+                          f"push+absorb+deposit+MCC in {el:.1f} s, -O3 oracle",
+                   single_core_value=ps1 / el1, single_core_sample=out["sample"])
+    return out
 
 
 def adjoint_leg(reps=5):
@@ -156,7 +187,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default="c2")
     ap.add_argument("--sort-interval", type=int, default=-1)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-workers", type=int, default=0, help="0: min(16, cores)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the graph and plain-launch comparisons")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-window leg")
@@ -194,6 +226,12 @@ def main():
     particles = [W.load_species(cfg, geom, s, rng) for s in range(cfg.ns)]
     ntot = sum(P.shape[1] for P in particles)
     log(f"[bench] {cfg.name}: {cfg.nz}x{cfg.nr} nodes, {ntot} particles, host load {time.perf_counter() - t:.1f}s")
+    cpu = None
+    if not args.no_cpu_baseline:
+        workers = args.cpu_workers or min(16, os.cpu_count() or 1)
+        log(f"[bench] cpu baseline (1 core, then {workers} processes) ...")
+        cpu = cpu_baseline(cfg, geom, tables, particles, args.cpu_seconds, workers)
+        log(f"[bench] cpu {cpu['value']:.4g} particle-steps/s on {cpu['cores']} cores")
     hp, _, _, _ = ipm2d.from_config(cfg, geom=geom, tables=tables, particles=particles, capacity_factor=1.25)
     if args.blocks_per_cu:
         hp.set_launch(args.blocks_per_cu)
@@ -244,12 +282,6 @@ def main():
     adj = None
     if not args.no_adjoint:
         adj = adjoint_leg()
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        log("[bench] cpu baseline ...")
-        cpu = cpu_baseline(cfg, geom, tables, particles, args.cpu_seconds)
-        log(f"[bench] cpu {cpu['value']:.4g} particle-steps/s")
 
     out = {
         "metric": "particle-steps/s (push+deposit+MCC) & % HBM peak; adjoint/forward time",

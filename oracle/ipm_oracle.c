@@ -794,7 +794,7 @@ double ora_probability_reactive(int nreaction, int nsigma, double species_mass, 
  * rho_raw: ns*nz*nr (accumulated, caller clears). Returns ora_mcc's code. */
 int ora_step(const ora_geom *g, const ora_mcc_setup *st, const double *Ez, const double *Er,
              uint64_t seed, uint32_t step, int64_t *n, const int64_t *cap, double *(*P)[8],
-             double *rho_raw, double *metal_charge, double *sigma)
+             double *rho_raw, double *metal_charge, double *sigma, int64_t *counts)
 {
     const size_t nn = (size_t)g->nz * g->nr;
     for (int s = 0; s < st->ns; ++s) {
@@ -805,7 +805,7 @@ int ora_step(const ora_geom *g, const ora_mcc_setup *st, const double *Ez, const
     }
     for (int s = 0; s < st->ns; ++s)
         ora_deposit_raw(g->nz, g->nr, n[s], P[s][0], P[s][1], P[s][7], rho_raw + (size_t)s * nn);
-    return ora_mcc(st, seed, step, n, cap, P, NULL);
+    return ora_mcc(st, seed, step, n, cap, P, counts);
 }
 
 /* Exposes the scattering-angle samplers (IsotropicCosKai / CosKaiVahedi,

@@ -115,7 +115,7 @@ uint32_t ora_vel_maxwell(double mass, double temperature, uint64_t seed, uint32_
  * deposits (rho_raw: ns*nz*nr, accumulated), then MCC. */
 int ora_step(const ora_geom *g, const ora_mcc_setup *st, const double *Ez, const double *Er,
              uint64_t seed, uint32_t step, int64_t *n, const int64_t *cap, double *(*P)[8],
-             double *rho_raw, double *metal_charge, double *sigma);
+             double *rho_raw, double *metal_charge, double *sigma, int64_t *counts);
 
 /* Scattering-angle and secondary-energy samplers, one draw each. */
 double ora_cos_chi(int vahedi, double energy, uint64_t seed, uint32_t slot, uint32_t draw);

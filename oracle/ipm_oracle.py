@@ -67,7 +67,7 @@ def lib(fast=False):
                                   C.c_uint32, _dp, _dp, _dp]
     L.ora_step.restype = C.c_int
     L.ora_step.argtypes = [C.POINTER(OraGeom), C.POINTER(OraSetup), _dp, _dp, C.c_uint64, C.c_uint32, _i64p,
-                           _i64p, C.c_void_p, _dp, _dp, _dp]
+                           _i64p, C.c_void_p, _dp, _dp, _dp, _i64p]
     L.ora_cos_chi.restype = C.c_double
     L.ora_cos_chi.argtypes = [C.c_int, C.c_double, C.c_uint64, C.c_uint32, C.c_uint32]
     L.ora_secondary_energy.restype = C.c_double
@@ -368,11 +368,17 @@ class StepState:
         self.mc = np.zeros(max(geom.nmetal, 1))
         self.sg = np.zeros(geom.nz * geom.nr)
 
-    def step(self, seed, step):
+    def step(self, seed, step, counts=None):
+        """One step; counts (int64, ns*ng*128) accumulates MCC events if given."""
         self.rho[:] = 0.0
+        cp = counts.ctypes.data_as(_i64p) if counts is not None else None
         rc = lib(self.fast).ora_step(C.byref(self.geom.s), C.byref(self.setup.s), _d(self.ez), _d(self.er), seed,
                                      step, self.n.ctypes.data_as(_i64p), self.cap.ctypes.data_as(_i64p),
-                                     C.cast(self.ptrs, C.c_void_p), _d(self.rho), _d(self.mc), _d(self.sg))
+                                     C.cast(self.ptrs, C.c_void_p), _d(self.rho), _d(self.mc), _d(self.sg), cp)
         if rc != 0:
             raise RuntimeError("oracle step capacity overflow")
         return int(self.n.sum())
+
+    def species(self, s):
+        """Live particles of species s as (8, n) (a view)."""
+        return self.bufs[s][:, :self.n[s]]
