@@ -62,6 +62,7 @@ struct ipm_ctx {
     unsigned *q_tile_count = nullptr;      // q_ntiles
     long long q_cap = 0, q_ntiles = 0;
     double *mom = nullptr;                 // moment sums, 5 * nz * nr (ipm_moments)
+    double *phi = nullptr;                 // potential (ipm_set_potential), nz * nr
     hipGraphExec_t graph = nullptr;
     int block = 256, blocks_per_cu = 0;
     // optional HIP-event timing of the kernels ipm_step enqueues

@@ -695,46 +695,69 @@ __global__ void grid_kernel(const DevState *__restrict__ stp, const double *__re
 // the velocity of a copy pushed once more in the present field
 // (MoveParticleOneElectrostaticImplicit, R/src/pic/particle/ParticleBundle.f90:782-858),
 // as the reference does for its implicit scheme (diagParticleField.F90:225-236).
-__global__ void moments_kernel(const DevState *__restrict__ stp, const DevCounters *__restrict__ ctr, int s,
-                               int implicit_j, double *__restrict__ acc)
+__global__ void __launch_bounds__(kBlock)
+moments_kernel(const DevState *__restrict__ stp, const DevCounters *__restrict__ ctr, int s, int implicit_j,
+               double *__restrict__ acc)
 {
     const DevState &st = *stp;
+    const long long n = (long long)ctr->n[s];
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long long)ctr->n[s]) return;
+    if ((long long)blockIdx.x * blockDim.x >= n) return;          // whole block past the live slots
+    const int lane = threadIdx.x & 63;
     double *const *F = st.f[s];
+    // Every lane takes part in the wave sums: lanes without a particle carry
+    // zero weights and a key of their own.
+    int key = -1 - lane;
+    double wr[4] = {0, 0, 0, 0}, we[4] = {0, 0, 0, 0}, wm[4] = {0, 0, 0, 0}, wz[4] = {0, 0, 0, 0},
+           wrr[4] = {0, 0, 0, 0};
+    int NzL = 1, NrL = 1;
     Part p;
-    p.Z = gp(F[0])[i];
-    if (is_dead(p.Z)) return;
-    p.R = gp(F[1])[i]; p.Vz = gp(F[2])[i]; p.Vr = gp(F[3])[i]; p.Vt = gp(F[4])[i];
-    p.Az = gp(F[5])[i]; p.Ar = gp(F[6])[i]; p.WQ = gp(F[7])[i];
-    int NzL = (int)ceil(p.Z);
-    const double ZL = (double)NzL - p.Z, ZU = 1.0 - ZL;
-    int NrL = (int)ceil(p.R);
-    const double RL = (double)NrL - p.R, RU = 1.0 - RL;
-    NzL = min(max(NzL, 1), st.nz - 1);
-    const bool row0 = NrL < 1;
-    NrL = min(max(NrL, 1), st.nr - 1);
-    const double energy = p.Vr * p.Vr + p.Vt * p.Vt + p.Vz * p.Vz;
-    double vz = p.Vz, vr = p.Vr;
-    if (implicit_j) {
-        Part t = p;
-        push_one(st, st.efactor[s], t);
-        vz = t.Vz;
-        vr = t.Vr;
+    p.Z = i < n ? gp(F[0])[i] : kDead;
+    if (!is_dead(p.Z)) {
+        p.R = gp(F[1])[i]; p.Vz = gp(F[2])[i]; p.Vr = gp(F[3])[i]; p.Vt = gp(F[4])[i];
+        p.Az = gp(F[5])[i]; p.Ar = gp(F[6])[i]; p.WQ = gp(F[7])[i];
+        NzL = (int)ceil(p.Z);
+        const double ZL = (double)NzL - p.Z, ZU = 1.0 - ZL;
+        NrL = (int)ceil(p.R);
+        const double RL = (double)NrL - p.R, RU = 1.0 - RL;
+        NzL = min(max(NzL, 1), st.nz - 1);
+        const bool row0 = NrL < 1;
+        NrL = min(max(NrL, 1), st.nr - 1);
+        const double energy = p.Vr * p.Vr + p.Vt * p.Vt + p.Vz * p.Vz;
+        double vz = p.Vz, vr = p.Vr;
+        if (implicit_j) {
+            Part t = p;
+            push_one(st, st.efactor[s], t);
+            vz = t.Vz;
+            vr = t.Vr;
+        }
+        double w[4] = {ZL * RL, ZL * RU, ZU * RL, ZU * RU};
+        if (row0) { w[0] = w[1]; w[2] = w[3]; w[1] = 0.0; w[3] = 0.0; }
+        for (int q = 0; q < 4; ++q) {
+            const double ww = w[q] * p.WQ;
+            wr[q] = ww; we[q] = ww * energy; wm[q] = w[q]; wz[q] = ww * vz; wrr[q] = ww * vr;
+        }
+        key = (NzL - 1) + (NrL - 1) * (st.nz - 1);
     }
-    double w[4] = {ZL * RL, ZL * RU, ZU * RL, ZU * RU};
-    if (row0) { w[0] = w[1]; w[2] = w[3]; w[1] = 0.0; w[3] = 0.0; }
-    const size_t nn = (size_t)st.nz * st.nr;
-    const size_t k0 = (size_t)(NzL - 1) + (size_t)(NrL - 1) * st.nz;
-    const size_t node[4] = {k0, k0 + st.nz, k0 + 1, k0 + 1 + st.nz};
-    for (int q = 0; q < 4; ++q) {
-        if (w[q] == 0.0) continue;
-        const double ww = w[q] * p.WQ;
-        global_add(acc + node[q], ww);
-        global_add(acc + nn + node[q], ww * energy);
-        global_add(acc + 2 * nn + node[q], w[q]);
-        global_add(acc + 3 * nn + node[q], ww * vz);
-        global_add(acc + 4 * nn + node[q], ww * vr);
+    // runs of particles in one cell (cell-sorted storage) are summed across
+    // the wave first; only the run's last lane adds to memory
+    wave_segmented_sum(key, wr);
+    wave_segmented_sum(key, we);
+    wave_segmented_sum(key, wm);
+    wave_segmented_sum(key, wz);
+    const bool run_end = wave_segmented_sum(key, wrr);
+    if (run_end && key >= 0) {
+        const size_t nn = (size_t)st.nz * st.nr;
+        const size_t k0 = (size_t)(NzL - 1) + (size_t)(NrL - 1) * st.nz;
+        const size_t node[4] = {k0, k0 + st.nz, k0 + 1, k0 + 1 + st.nz};
+        for (int q = 0; q < 4; ++q) {
+            if (wm[q] == 0.0) continue;
+            global_add(acc + node[q], wr[q]);
+            global_add(acc + nn + node[q], we[q]);
+            global_add(acc + 2 * nn + node[q], wm[q]);
+            global_add(acc + 3 * nn + node[q], wz[q]);
+            global_add(acc + 4 * nn + node[q], wrr[q]);
+        }
     }
 }
 
@@ -756,6 +779,35 @@ __global__ void moments_scale_kernel(double *__restrict__ acc, const double *__r
     acc[nn + k] = e;
     acc[3 * nn + k] = acc[3 * nn + k] * iv * charge * vfactor;
     acc[4 * nn + k] = acc[4 * nn + k] * iv * charge * vfactor;
+}
+
+// FT%SetE(FG), R/src/pic/field/FieldSource.f90:95-120, on a potential whose
+// ghost layers are filled by linear extrapolation, the reference's default
+// on every side of a one-rank domain (R/src/pic/domain/PICCom.F90:40-43,
+// 516-578: ghost = 2*edge - next).  One thread per face of Ez then of Er.
+__global__ void set_e_kernel(const double *__restrict__ phi, double *__restrict__ Ez, double *__restrict__ Er,
+                             int nz, int nr, double dz)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long nez = (long long)(nz + 1) * nr, ner = (long long)nz * (nr + 1);
+    auto P = [&](int i, int j) -> double {       // potential with ghosts, i = 0..nz+1, j = 0..nr+1
+        auto in = [&](int a, int b) { return phi[(size_t)(a - 1) + (size_t)(b - 1) * nz]; };
+        if (i == 0) return 2.0 * in(1, j) - in(2, j);
+        if (i == nz + 1) return 2.0 * in(nz, j) - in(nz - 1, j);
+        if (j == 0) return 2.0 * in(i, 1) - in(i, 2);
+        if (j == nr + 1) return 2.0 * in(i, nr) - in(i, nr - 1);
+        return in(i, j);
+    };
+    if (t < nez) {
+        const int i = (int)(t % (nz + 1)), j = (int)(t / (nz + 1)) + 1;   // Ez(i, j), i = 0..nz
+        const double v = P(i, j) - P(i + 1, j);
+        Ez[t] = v / dz;
+    } else if (t < nez + ner) {
+        const long long u = t - nez;
+        const int i = (int)(u % nz) + 1, j = (int)(u / nz);             // Er(i, j), j = 0..nr
+        const double v = P(i, j) - P(i, j + 1);
+        Er[u] = v / dz;
+    }
 }
 
 __global__ void clear_kernel(double *a, size_t n)
@@ -1067,7 +1119,7 @@ int ipm_destroy(ipm_ctx *c)
                     c->d_chifac, c->inv_vol, c->cell_material, c->hedge, c->vedge, c->mat_type,
                     c->metal_charge, c->sigma, c->Ez, c->Er, c->mcc_counts, c->hist, c->offs,
                     c->cursor, c->partial, c->partial2, c->d_dst, c->qbuf, c->q_tile_count,
-                    c->mom};
+                    c->mom, c->phi};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1223,6 +1275,21 @@ int ipm_set_fields(ipm_ctx *c, const double *Ez, const double *Er)
     return IPM_OK;
 }
 
+int ipm_set_potential(ipm_ctx *c, const double *phi)
+{
+    if (!c || !phi) return fail(IPM_EINVAL, "null argument");
+    const size_t nn = (size_t)c->nz * c->nr;
+    if (!c->phi) HIPCHK(hipMalloc(&c->phi, sizeof(double) * nn));
+    HIPCHK(hipMemcpyAsync(c->phi, phi, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+    const long long tot = (long long)(c->nz + 1) * c->nr + (long long)c->nz * (c->nr + 1);
+    hipLaunchKernelGGL(set_e_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, c->phi, c->Ez,
+                       c->Er, c->nz, c->nr, c->dx);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->fields_set = true;
+    return IPM_OK;
+}
+
 static int check_sel(ipm_ctx *c, int s)
 {
     if (s < -1 || s >= c->ns) return fail(IPM_EINVAL, "invalid species index %d", s);
@@ -1304,8 +1371,8 @@ int ipm_moments(ipm_ctx *c, int s, int implicit_j, double *rho, double *energy_e
     HIPCHK(hipMemsetAsync(c->mom, 0, sizeof(double) * 5 * nn, c->stream));
     const long long cap = c->h.cap[s];
     if (cap > 0)
-        hipLaunchKernelGGL(moments_kernel, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, c->stream, c->d_state,
-                           c->d_ctr, s, implicit_j ? 1 : 0, c->mom);
+        hipLaunchKernelGGL(moments_kernel, dim3((unsigned)((cap + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream,
+                           c->d_state, c->d_ctr, s, implicit_j ? 1 : 0, c->mom);
     const double vf = c->h.vfactor;
     const double ef = 0.5 * c->h.mass[s] * vf * vf;
     hipLaunchKernelGGL(moments_scale_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream, c->mom,

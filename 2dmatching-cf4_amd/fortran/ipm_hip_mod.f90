@@ -174,6 +174,11 @@ module ipm_hip
             integer(c_int), value :: s
             type(c_ptr), intent(inout) :: fields8(8)
         end function
+        integer(c_int) function ipm_set_potential(ctx, phi) bind(C, name="ipm_set_potential")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(in) :: phi(*)
+        end function
         integer(c_int) function ipm_moments(ctx, s, implicit_j, rho, energy_ev, mp, jz, jr) &
                 bind(C, name="ipm_moments")
             import :: c_ptr, c_int, c_double
@@ -276,6 +281,13 @@ contains
     subroutine ipm_MCCStep(ctx)
         type(c_ptr), intent(in) :: ctx
         call ipm_check(ipm_mcc(ctx), "MCC")
+    end subroutine
+
+    ! FT%SetE(FG): R/src/pic/field/FieldSource.f90:95, from Phi(1:Nz, 1:Nr).
+    subroutine ipm_SetE(ctx, Phi)
+        type(c_ptr), intent(in) :: ctx
+        real(c_double), intent(in) :: Phi(:, :)
+        call ipm_check(ipm_set_potential(ctx, Phi), "SetE")
     end subroutine
 
     ! WeightingParticleField for species k: R/src/diag/diagParticleField.F90:173.

@@ -67,6 +67,7 @@ SIGNATURES = {
     "ipm_set_launch": [_vp, C.c_int, C.c_int],
     "ipm_synchronize": [_vp],
     "ipm_species_device_ptrs": [_vp, C.c_int, C.POINTER(_dp)],
+    "ipm_set_potential": [_vp, _dp],
     "ipm_moments": [_vp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp],
     "ipm_window_create": [_vp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp],
     "ipm_window_forward": [_vp, _dp, C.c_int, _dp],

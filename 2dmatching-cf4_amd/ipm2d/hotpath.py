@@ -142,6 +142,12 @@ class HotPath:
         assert ez.size == (self.nz + 1) * self.nr and er.size == self.nz * (self.nr + 1)
         N.call("ipm_set_fields", self._h, _d(ez), _d(er))
 
+    def set_potential(self, phi):
+        """FT%SetE(FG) on the device from a (nz, nr) node potential."""
+        ph = np.ascontiguousarray(fortran(np.asarray(phi, dtype=np.float64)))
+        assert ph.size == self.nz * self.nr
+        N.call("ipm_set_potential", self._h, _d(ph))
+
     # ------------------------------------------------- reference-named calls
     def move_es(self, s=-1):
         N.call("ipm_move_es", self._h, s)

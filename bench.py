@@ -192,7 +192,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the graph and plain-launch comparisons")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-window leg")
-    ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--blocks-per-cu", type=int, default=-1, help="-1: autotuned; 0: one block per tile")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the config's launch shape and sort interval")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: a 1 GiB device copy (counter calibration), then 4 steps")
     args = ap.parse_args()
@@ -233,7 +234,14 @@ def main():
         cpu = cpu_baseline(cfg, geom, tables, particles, args.cpu_seconds, workers)
         log(f"[bench] cpu {cpu['value']:.4g} particle-steps/s on {cpu['cores']} cores")
     hp, _, _, _ = ipm2d.from_config(cfg, geom=geom, tables=tables, particles=particles, capacity_factor=1.25)
-    if args.blocks_per_cu:
+    tuned = None
+    if not args.no_autotune and not args.traffic_probe and args.sort_interval < 0 and args.blocks_per_cu < 0:
+        # the tuner advances the live simulation, so it also warms up
+        from ipm2d import autotune
+        tuned = autotune.autotune(hp, ntot)
+        sort_interval = tuned["sort_interval"]
+        log(f"[bench] autotuned: blocks_per_cu {tuned['blocks_per_cu']}, sort every {sort_interval} steps")
+    elif args.blocks_per_cu > 0:
         hp.set_launch(args.blocks_per_cu)
     log(f"[bench] uploaded; warmup {args.warmup} steps")
     for k in range(args.warmup):
@@ -304,6 +312,9 @@ def main():
             "species": cfg.ns,
             "grid": [cfg.nz, cfg.nr],
             "sort_interval": sort_interval,
+            "launch": ("autotuned: " if tuned else "") + (
+                "one block per 1024-slot tile" if (tuned["blocks_per_cu"] if tuned else max(args.blocks_per_cu, 0)) == 0
+                else f"{tuned['blocks_per_cu'] if tuned else args.blocks_per_cu} persistent blocks per CU"),
         },
         "roofline": {
             "bound": "hbm",

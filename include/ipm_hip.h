@@ -146,6 +146,14 @@ int ipm_synchronize(ipm_ctx *ctx);
  * of the raw per-species deposit grid. */
 int ipm_species_device_ptrs(ipm_ctx *ctx, int s, double **fields8);
 
+/* FT%SetE(FG) on the device (R/src/pic/field/FieldSource.f90:95-120):
+ * phi(1:Nz, 1:Nr) is the node potential; its ghost layers are filled by
+ * linear extrapolation as the reference's field communicator does for every
+ * side of a one-rank domain (R/src/pic/domain/PICCom.F90:474-590), and Ez,
+ * Er are formed on the device.  Replaces ipm_set_fields when the host has
+ * the potential: one grid crosses PCIe instead of two. */
+int ipm_set_potential(ipm_ctx *ctx, const double *phi);
+
 /* Moment diagnostics of species s: WeightingParticleField
  * (R/src/diag/diagParticleField.F90:173-318).  Each output is Nz*Nr (or
  * NULL to skip): rho = RhoOne (number density), energy_ev = TOne (mean

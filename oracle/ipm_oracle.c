@@ -16,6 +16,9 @@
  *   ora_deposit_*         charge conservation, on-node known answers,
  *                         uniform density vs analytic node volumes
  *   ora_node_volumes      closed-form cylindrical shell volumes
+ *   ora_set_e             linear potentials give the uniform field exactly
+ *                         on every face including the ghost-fed ones;
+ *                         numpy restatement agrees bit for bit
  *   ora_moments           charge conservation, macro-particle count, uniform
  *                         velocity and speed known answers, implicit current
  *                         equal to the explicit one in a zero field
@@ -281,6 +284,40 @@ void ora_deposit_finish(int nz, int nr, double dt, double charge, double mass,
         rho[k] = v;
         chi[k] = v * ChiFactor;
     }
+}
+
+/* SetE: FT%SetE(FG), R/src/pic/field/FieldSource.f90:95-120, after the
+ * potential's ghost layers are filled by linear extrapolation, the
+ * reference's default for every side of a one-rank domain
+ * (R/src/pic/domain/PICCom.F90:40-43, 516-519, 536-539, 556-559, 575-578:
+ * ghost = 2*edge - next).  phi is (1:Nz, 1:Nr); Ez (0:Nz, 1:Nr), Er (1:Nz, 0:Nr). */
+void ora_set_e(int nz, int nr, double dx, const double *phi, double *Ez, double *Er)
+{
+    const size_t lz = (size_t)nz + 2;
+    double *p = (double *)calloc(lz * (size_t)(nr + 2), sizeof(double));
+#define PP(i, j) p[(size_t)(i) + (size_t)(j) * lz]        /* i = 0..nz+1, j = 0..nr+1 */
+    for (int j = 1; j <= nr; ++j)
+        for (int i = 1; i <= nz; ++i) PP(i, j) = phi[(size_t)(i - 1) + (size_t)(j - 1) * nz];
+    for (int j = 1; j <= nr; ++j) {
+        PP(0, j) = 2.0 * PP(1, j) - PP(2, j);
+        PP(nz + 1, j) = 2.0 * PP(nz, j) - PP(nz - 1, j);
+    }
+    for (int i = 1; i <= nz; ++i) {
+        PP(i, 0) = 2.0 * PP(i, 1) - PP(i, 2);
+        PP(i, nr + 1) = 2.0 * PP(i, nr) - PP(i, nr - 1);
+    }
+    for (int j = 1; j <= nr; ++j)
+        for (int i = 0; i <= nz; ++i) {
+            double v = PP(i, j) - PP(i + 1, j);
+            Ez[(size_t)i + (size_t)(j - 1) * (nz + 1)] = v / dx;
+        }
+    for (int j = 0; j <= nr; ++j)
+        for (int i = 1; i <= nz; ++i) {
+            double v = PP(i, j) - PP(i, j + 1);
+            Er[(size_t)(i - 1) + (size_t)j * nz] = v / dx;
+        }
+#undef PP
+    free(p);
 }
 
 /* WeightingParticleField for one species, R/src/diag/diagParticleField.F90:196-257

@@ -68,6 +68,10 @@ void ora_deposit_finish(int nz, int nr, double dt, double charge, double mass,
                         const double *rho_raw, const double *inv_vol,
                         double *rho, double *chi);
 
+/* SetE after linear ghost extrapolation (FieldSource.f90:95-120,
+ * PICCom.F90:516-578): phi (1:Nz,1:Nr) -> Ez (0:Nz,1:Nr), Er (1:Nz,0:Nr). */
+void ora_set_e(int nz, int nr, double dx, const double *phi, double *Ez, double *Er);
+
 /* WeightingParticleField for one species (diagParticleField.F90:196-318):
  * rho, energy (mean eV where rho > 0), mp, jz, jr, each nz*nr. */
 void ora_moments(int nz, int nr, double dx, double dt, double charge, double mass, const double *Ez,

@@ -58,6 +58,7 @@ def lib(fast=False):
     L.ora_absorb.argtypes = [C.POINTER(OraGeom), C.c_double, _i64p, C.POINTER(_dp), _dp, _dp]
     L.ora_deposit_raw.argtypes = [C.c_int, C.c_int, C.c_int64, _dp, _dp, _dp, _dp]
     L.ora_deposit_finish.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _dp]
+    L.ora_set_e.argtypes = [C.c_int, C.c_int, C.c_double, _dp, _dp, _dp]
     L.ora_moments.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp,
                               C.c_int, C.c_int64, C.POINTER(_dp)] + [_dp] * 6
     L.ora_mcc.restype = C.c_int
@@ -187,6 +188,16 @@ def deposit_finish(nz, nr, dt, charge, mass, raw, inv_vol):
 
 def p2ces(nz, nr, dt, charge, mass, P, inv_vol):
     return deposit_finish(nz, nr, dt, charge, mass, deposit_raw(nz, nr, P), inv_vol)
+
+
+def set_e(phi, dx):
+    """SetE from a (nz, nr) potential: Fortran-layout flat Ez, Er."""
+    nz, nr = phi.shape
+    ph = F(phi)
+    ez = np.zeros((nz + 1) * nr)
+    er = np.zeros(nz * (nr + 1))
+    lib().ora_set_e(nz, nr, dx, _d(ph), _d(ez), _d(er))
+    return ez, er
 
 
 def moments(nz, nr, dx, dt, charge, mass, ez_flat, er_flat, P, inv_vol, implicit_j=True):

@@ -340,3 +340,19 @@ def test_mcc_independent_of_storage_order(rng):
         hp.mcc()
     for s in range(cfg.ns):
         np.testing.assert_array_equal(sort_set(a.download(s)), sort_set(b.download(s)))
+
+
+def test_set_potential_matches_oracle_set_e(oracle, rng):
+    """ipm_set_potential (SetE on the device) feeds the push exactly as the
+    oracle's SetE restatement does."""
+    geom = small_geometry()
+    nz, nr = geom.nz, geom.nr
+    phi = rng.standard_normal((nz, nr)) * 30.0
+    hp = make_hp(geom)
+    hp.set_potential(phi)
+    P = random_particles(nz, nr, 5000, rng)
+    hp.upload(0, P)
+    hp.move_es(0)
+    ez, er = oracle.set_e(phi, DX)
+    ref = oracle.push(nz, nr, DX, DT, SPECIES[0][0], SPECIES[0][1], ez, er, P)
+    np.testing.assert_array_equal(hp.download(0), ref)
