@@ -609,15 +609,19 @@ collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
                 }
                 // write back only what the collisions changed (null events
                 // change nothing; Az/Ar change only in charge exchange): each
-                // field is a separate cache line of a separate array
+                // field is a separate cache line of a separate array, stored
+                // non-temporally so the scattered lines do not crowd L2
                 if (__double_as_longlong(p.Vz) != __double_as_longlong(p0.Vz) ||
                     __double_as_longlong(p.Vr) != __double_as_longlong(p0.Vr) ||
                     __double_as_longlong(p.Vt) != __double_as_longlong(p0.Vt)) {
-                    F[2][i] = p.Vz; F[3][i] = p.Vr; F[4][i] = p.Vt;
+                    __builtin_nontemporal_store(p.Vz, &F[2][i]);
+                    __builtin_nontemporal_store(p.Vr, &F[3][i]);
+                    __builtin_nontemporal_store(p.Vt, &F[4][i]);
                 }
                 if (__double_as_longlong(p.Az) != __double_as_longlong(p0.Az) ||
                     __double_as_longlong(p.Ar) != __double_as_longlong(p0.Ar)) {
-                    F[5][i] = p.Az; F[6][i] = p.Ar;
+                    __builtin_nontemporal_store(p.Az, &F[5][i]);
+                    __builtin_nontemporal_store(p.Ar, &F[6][i]);
                 }
             }
             __syncthreads();
