@@ -63,6 +63,7 @@ struct ipm_ctx {
     long long q_cap = 0, q_ntiles = 0;
     double *mom = nullptr;                 // moment sums, 5 * nz * nr (ipm_moments)
     double *phi = nullptr;                 // potential (ipm_set_potential), nz * nr
+    unsigned long long *rezone = nullptr;  // split/merge tallies (ipm_split_merge), 3
     hipGraphExec_t graph = nullptr;
     int block = 256, blocks_per_cu = 0;
     // optional HIP-event timing of the kernels ipm_step enqueues

@@ -15,6 +15,7 @@
 #include <stdarg.h>
 #include <vector>
 #include <algorithm>
+#include <cmath>
 #include "ipm_device.hpp"
 
 using namespace ipm;
@@ -820,6 +821,108 @@ __global__ void clear_kernel(double *a, size_t n)
     if (i < n) a[i] = 0.0;
 }
 
+// ------------------------------------------------------------ rezoning
+// Random-stream tag of the merge draw: apart from every (species, gas) tag
+// of the collision pass, which stay below 2^16.
+constexpr uint32_t kMergeTag = 0x4D520000u;
+
+// split_particle_weight followed by merge_particle_random,
+// R/src/base/Rezoning.F90:53-92, one thread per slot.  The reference weight
+// of a particle is CalRW, R/src/pic/particle/ParticleBundle.f90:393-402:
+// Weight * ((1 - alpha) * R / grid_r + alpha).
+//   split  (mode & 1): Ncr = int(WQ / ref); for Ncr > 1 the particle gives
+//          Ncr - 1 copies of weight ref, subtracting ref once per copy.
+//   merge  (mode & 2): a particle with WQ < merge_limit * ref survives with
+//          weight ref when its uniform draw is below WQ / ref and is deleted
+//          otherwise.  The copies of a split never qualify (ref < l * ref
+//          implies a draw below 1: always kept at weight ref), so only the
+//          particle itself is tested, after its split.
+// Copies are appended after the n slots in use (folded by rezone_fold_kernel).
+// Copy j (1-based) of a particle has the low bits of its Vt bit pattern
+// XORed with j: copies then key different random streams in the collision
+// and merge passes, where exact copies would draw alike forever.  A particle
+// whose ref is not positive (alpha = 0 on the axis) is left alone.
+__global__ void __launch_bounds__(kBlock)
+split_merge_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr, int s, int mode, double weight,
+                   double alpha, double grid_r, double merge_limit, unsigned long long *__restrict__ stats)
+{
+    const DevState &st = *stp;
+    const long long n = (long long)ctr->n[s];
+    if ((long long)blockIdx.x * blockDim.x >= n) return;
+    const long long cap = st.cap[s];
+    __shared__ unsigned long long tally[4];   // copies, kept, deleted, dead slots
+    if (threadIdx.x < 4) tally[threadIdx.x] = 0;
+    __syncthreads();
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double *const *F = st.f[s];
+    const double Z = i < n ? gp(F[0])[i] : kDead;
+    if (!is_dead(Z)) {
+        const double R = F[1][i];
+        const double wq0 = F[7][i];
+        double wq = wq0;
+        const double ref = weight * ((1.0 - alpha) * R / grid_r + alpha);
+        if (ref > 0.0) {
+            const double Vz = F[2][i], Vr = F[3][i], Vt = F[4][i];
+            if ((mode & 1) && wq / ref >= 2.0) {
+                const double ratio = wq / ref;
+                if (!(ratio - 1.0 <= (double)(cap - n))) {
+                    atomicOr(&ctr->overflow, 1);            // more copies than free slots
+                } else {
+                    const unsigned long long m = (unsigned long long)(long long)ratio - 1ull;
+                    const unsigned long long k0 = atomicAdd(&ctr->n_new[s], m);
+                    const unsigned long long base = (unsigned long long)n + k0;
+                    if (base + m > (unsigned long long)cap) {
+                        // the reservation ran past the capacity: nothing is
+                        // split, reserved slots inside it become dead slots
+                        atomicOr(&ctr->overflow, 1);
+                        unsigned long long nd = 0;
+                        for (unsigned long long q = base; q < (unsigned long long)cap && q < base + m; ++q, ++nd)
+                            F[0][q] = kDead;
+                        if (nd) atomicAdd(&tally[3], nd);
+                    } else {
+                        const double Az = F[5][i], Ar = F[6][i];
+                        const unsigned long long vtb = (unsigned long long)__double_as_longlong(Vt);
+                        for (unsigned long long j = 1; j <= m; ++j) {
+                            wq = wq - ref;
+                            const unsigned long long q = base + j - 1;
+                            F[0][q] = Z; F[1][q] = R; F[2][q] = Vz; F[3][q] = Vr;
+                            F[4][q] = __longlong_as_double((long long)(vtb ^ j));
+                            F[5][q] = Az; F[6][q] = Ar; F[7][q] = ref;
+                        }
+                        atomicAdd(&tally[0], m);
+                    }
+                }
+            }
+            if ((mode & 2) && wq < merge_limit * ref) {
+                const StateKey sk = state_key(Z, R, Vz, Vr, Vt);
+                Rng g = make_rng(sk, ctr->seed, ctr->step, kMergeTag | (uint32_t)s);
+                if (g.next() < wq / ref) {
+                    wq = ref;
+                    atomicAdd(&tally[1], 1ull);
+                } else {
+                    F[0][i] = kDead;
+                    atomicAdd(&tally[2], 1ull);
+                    atomicAdd(&tally[3], 1ull);
+                }
+            }
+            if (__double_as_longlong(wq) != __double_as_longlong(wq0)) F[7][i] = wq;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3 && tally[threadIdx.x]) atomicAdd(&stats[threadIdx.x], tally[threadIdx.x]);
+    if (threadIdx.x == 3 && tally[3]) atomicAdd(&ctr->n_dead[s], tally[3]);
+}
+
+// Folds the copies appended by split_merge_kernel into n[s].
+__global__ void rezone_fold_kernel(DevCounters *ctr, int s, long long cap)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const unsigned long long n = ctr->n[s] + ctr->n_new[s];
+        ctr->n[s] = n > (unsigned long long)cap ? (unsigned long long)cap : n;
+        ctr->n_new[s] = 0;
+    }
+}
+
 // ------------------------------------------------------------ cell sort
 __device__ __forceinline__ unsigned long long cell_of(const DevState &st, double z, double r)
 {
@@ -1123,7 +1226,7 @@ int ipm_destroy(ipm_ctx *c)
                     c->d_chifac, c->inv_vol, c->cell_material, c->hedge, c->vedge, c->mat_type,
                     c->metal_charge, c->sigma, c->Ez, c->Er, c->mcc_counts, c->hist, c->offs,
                     c->cursor, c->partial, c->partial2, c->d_dst, c->qbuf, c->q_tile_count,
-                    c->mom, c->phi};
+                    c->mom, c->phi, c->rezone};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1291,6 +1394,34 @@ int ipm_set_potential(ipm_ctx *c, const double *phi)
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     c->fields_set = true;
+    return IPM_OK;
+}
+
+int ipm_split_merge(ipm_ctx *c, int s, int mode, double weight, double alpha, double grid_r, double merge_limit,
+                    int64_t *stats)
+{
+    int rc = check_ready(c, false, false);
+    if (rc) return rc;
+    if (s < 0 || s >= c->ns) return fail(IPM_EINVAL, "invalid species index %d", s);
+    if (mode < 0 || mode > 3) return fail(IPM_EINVAL, "invalid split/merge mode %d", mode);
+    if (!(weight > 0.0) || !(grid_r > 0.0) || !std::isfinite(weight) || !std::isfinite(grid_r) ||
+        !std::isfinite(alpha) || !std::isfinite(merge_limit))
+        return fail(IPM_EINVAL, "split/merge needs finite weight > 0, grid_r > 0, alpha and merge_limit");
+    if (!c->rezone) HIPCHK(hipMalloc(&c->rezone, sizeof(unsigned long long) * 3));
+    HIPCHK(hipMemsetAsync(c->rezone, 0, sizeof(unsigned long long) * 3, c->stream));
+    const long long cap = c->h.cap[s];
+    if (cap > 0 && mode) {
+        hipLaunchKernelGGL(split_merge_kernel, dim3((unsigned)((cap + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                           c->stream, c->d_state, c->d_ctr, s, mode, weight, alpha, grid_r, merge_limit, c->rezone);
+        hipLaunchKernelGGL(rezone_fold_kernel, dim3(1), dim3(64), 0, c->stream, c->d_ctr, s, cap);
+        HIPCHK(hipGetLastError());
+    }
+    unsigned long long t[3];
+    HIPCHK(hipMemcpyAsync(t, c->rezone, sizeof t, hipMemcpyDeviceToHost, c->stream));
+    DevCounters ctr;
+    if ((rc = read_counters(c, &ctr))) return rc;
+    if (stats)
+        for (int q = 0; q < 3; ++q) stats[q] = (int64_t)t[q];
     return IPM_OK;
 }
 

@@ -186,6 +186,14 @@ module ipm_hip
             integer(c_int), value :: s, implicit_j
             real(c_double), intent(inout) :: rho(*), energy_ev(*), mp(*), jz(*), jr(*)
         end function
+        integer(c_int) function ipm_split_merge(ctx, s, mode, weight, alpha, grid_r, merge_limit, stats) &
+                bind(C, name="ipm_split_merge")
+            import :: c_ptr, c_int, c_double, c_int64_t
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: s, mode
+            real(c_double), value :: weight, alpha, grid_r, merge_limit
+            integer(c_int64_t), intent(inout) :: stats(*)
+        end function
         integer(c_int) function ipm_window_create(ctx, nsteps, interval, Ez0, Er0, Ez1, Er1, mu, alpha) &
                 bind(C, name="ipm_window_create")
             import :: c_ptr, c_int, c_double
@@ -299,6 +307,18 @@ contains
         real(c_double), intent(inout) :: RhoOne(:, :), TOne(:, :), MPOne(:, :), JzOne(:, :), JrOne(:, :)
         call ipm_check(ipm_moments(ctx, int(k, c_int), 1_c_int, RhoOne, TOne, MPOne, JzOne, JrOne), &
                        "WeightingParticleField")
+    end subroutine
+
+    ! split_particle_weight then merge_particle_random of ModuleParticleRezoning
+    ! (R/src/base/Rezoning.F90:53-92) for bundle k; Weight, alpha and grid_r are
+    ! PB(k)%Weight, PB(k)%alpha and PB(k)%grid_r, merge_limit the module's.
+    subroutine ipm_SplitMerge(ctx, k, Weight, alpha, grid_r, merge_limit)
+        type(c_ptr), intent(in) :: ctx
+        integer, intent(in) :: k
+        real(c_double), intent(in) :: Weight, alpha, grid_r, merge_limit
+        integer(c_int64_t) :: stats(3)
+        call ipm_check(ipm_split_merge(ctx, int(k, c_int), 3_c_int, Weight, alpha, grid_r, merge_limit, stats), &
+                       "SplitMerge")
     end subroutine
 
     ! PB(k)%NPar

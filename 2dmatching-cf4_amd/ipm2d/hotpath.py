@@ -10,6 +10,8 @@ the reference's names and argument meaning:
     mcc()                          MCC(Ns, Ng, PB, ...)        MCCPublic.F90:38
     one_step()                     OneStep's particle calls    onestep.F90:141-177
     moments(s)                     WeightingParticleField      diagParticleField.F90:173
+    split_merge(s, ...)            split_particle_weight +     Rezoning.F90:53-92
+                                   merge_particle_random
 
 Errors raise ``IPMError`` (``CapacityError`` where the reference would
 ``stop`` on a full particle bundle).  Grids are returned as numpy arrays
@@ -196,6 +198,17 @@ class HotPath:
         N.call("ipm_moments", self._h, s, 1 if implicit_j else 0, *(_d(out[k]) for k in
                                                                     ("rho", "energy", "mp", "jz", "jr")))
         return {k: v.reshape(self.nr, self.nz).T for k, v in out.items()}
+
+    def split_merge(self, s, weight, alpha, grid_r, merge_limit=0.34, split=True, merge=True):
+        """split_particle_weight then merge_particle_random on species s
+        (Rezoning.F90:53-92) with CalRW's reference weight
+        weight * ((1 - alpha) * R / grid_r + alpha).  Returns the counts
+        (copies appended, merge candidates kept, merge candidates deleted)."""
+        st = np.zeros(3, dtype=np.int64)
+        mode = (1 if split else 0) | (2 if merge else 0)
+        N.call("ipm_split_merge", self._h, s, mode, float(weight), float(alpha), float(grid_r), float(merge_limit),
+               st.ctypes.data_as(C.POINTER(C.c_int64)))
+        return tuple(int(x) for x in st)
 
     def mcc(self):
         N.call("ipm_mcc", self._h)

@@ -165,6 +165,26 @@ int ipm_set_potential(ipm_ctx *ctx, const double *phi);
 int ipm_moments(ipm_ctx *ctx, int s, int implicit_j, double *rho, double *energy_ev, double *mp, double *jz,
                 double *jr);
 
+/* Weight rezoning of species s: split_particle_weight then
+ * merge_particle_random (R/src/base/Rezoning.F90:53-92), the per-bundle
+ * calls of particle_split_merge_control (Rezoning.F90:17-50).  The reference
+ * weight of a particle at R is CalRW (R/src/pic/particle/ParticleBundle.f90:393-402)
+ *   ref = weight * ((1 - alpha) * R / grid_r + alpha),
+ * with weight = PB%Weight, alpha = PB%alpha, grid_r = PB%grid_r = Nr - 1.
+ * mode & 1 splits: Ncr = int(WQ / ref); Ncr > 1 appends Ncr - 1 copies of
+ * weight ref and subtracts ref from WQ once per copy.  mode & 2 merges: WQ <
+ * merge_limit * ref keeps the particle at weight ref when a uniform draw is
+ * below WQ / ref and deletes it otherwise.  Draws come from the particle's
+ * state-keyed stream under a tag of their own, so the result does not depend
+ * on storage order.  Copy j of a split differs from its parent only in the
+ * low bits of Vt's bit pattern (XOR j), so that copies draw independently
+ * afterwards.  stats (3 entries, may be NULL) receives copies appended,
+ * merge candidates kept and merge candidates deleted.  Returns
+ * IPM_ECAPACITY (nothing split for the particles concerned) when copies
+ * would exceed the species capacity; the reference grows its array. */
+int ipm_split_merge(ipm_ctx *ctx, int s, int mode, double weight, double alpha, double grid_r,
+                    double merge_limit, int64_t *stats);
+
 /* ---- Differentiable collisionless window ----------------------------------
  * No counterpart in the reference (which has no gradients).  Advances a
  * snapshot of the resident particles through nsteps steps of MoveESIm

@@ -27,6 +27,11 @@
  *                         Maxwellian mean energy, energy bookkeeping of
  *                         excitation/ionization, momentum of elastic events
  *   ora_sigma_* / ora_probability_*  closed forms on synthetic tables
+ *   ora_split_merge       hand-computed splits (copy counts, remainders,
+ *                         total weight), merge keep fraction = WQ/ref within
+ *                         binomial bounds, expected weight conserved, the
+ *                         copies' Vt differing from the parent's by the XOR
+ *                         tag only
  * Parity unpinned: the reference's own random stream (Fortran
  * random_number, unseeded) -- by construction no check can reproduce it.
  */
@@ -97,6 +102,83 @@ void ora_state_key(double Z, double R, double Vz, double Vr, double Vt, uint32_t
 
 typedef struct { uint64_t seed; uint32_t step, tag, c0, c1, c2, k; } rng_t;
 static double rnext(rng_t *r) { return ora_uniform(r->seed, r->step, r->tag, r->c0, r->c1, r->c2, r->k++); }
+
+/* ---------------------------------------------------------- rezoning */
+#define ORA_MERGE_TAG 0x4D520000u
+
+/* CalRW, R/src/pic/particle/ParticleBundle.f90:393-402 */
+static double cal_rw(double weight, double alpha, double grid_r, double pos_r)
+{
+    return weight * ((1.0 - alpha) * pos_r / grid_r + alpha);
+}
+
+/* split_particle_weight (R/src/base/Rezoning.F90:73-92) then
+ * merge_particle_random (Rezoning.F90:53-70) on one bundle of n particles in
+ * arrays of capacity cap.  The uniform of the merge is drawn from the
+ * particle's state-keyed stream, tag ORA_MERGE_TAG | s, draw 0, in place of
+ * Fortran's random_number.  Copy j of a split carries the parent's Vt bit
+ * pattern XOR j.  A particle whose reference weight is not positive is
+ * skipped (the reference divides by it).  Returns -1 when the copies would
+ * exceed cap (nothing is changed then), 0 otherwise.  stats: copies
+ * appended, merge candidates kept, merge candidates deleted. */
+int ora_split_merge(int64_t *n, double *P[8], int64_t cap, int mode, double weight, double alpha,
+                    double grid_r, double merge_limit, uint64_t seed, uint32_t step, int s, int64_t stats[3])
+{
+    double *Z = P[0], *R = P[1], *Vt = P[4], *WQ = P[7];
+    stats[0] = stats[1] = stats[2] = 0;
+    if (mode & 1) {
+        int64_t need = 0;
+        for (int64_t i = 0; i < *n; ++i) {
+            double ref = cal_rw(weight, alpha, grid_r, R[i]);
+            if (ref > 0.0 && WQ[i] / ref >= 2.0) {
+                double ratio = WQ[i] / ref;
+                if (!(ratio - 1.0 <= (double)(cap - *n))) return -1;
+                need += (int64_t)ratio - 1;
+            }
+        }
+        if (need > cap - *n) return -1;
+        const int64_t npar = *n;
+        for (int64_t i = 0; i < npar; ++i) {
+            double ref = cal_rw(weight, alpha, grid_r, R[i]);
+            if (!(ref > 0.0)) continue;
+            int64_t ncr = (int64_t)(WQ[i] / ref);
+            if (ncr > 1) {
+                for (int64_t j = 1; j <= ncr - 1; ++j) {
+                    WQ[i] = WQ[i] - ref;
+                    int64_t q = *n;
+                    for (int f = 0; f < 8; ++f) P[f][q] = P[f][i];
+                    uint64_t b;
+                    memcpy(&b, &Vt[i], 8);
+                    b ^= (uint64_t)j;
+                    memcpy(&Vt[q], &b, 8);
+                    WQ[q] = ref;
+                    *n += 1;
+                    stats[0] += 1;
+                }
+            }
+        }
+    }
+    if (mode & 2) {
+        for (int64_t i = *n - 1; i >= 0; --i) {
+            double ref = cal_rw(weight, alpha, grid_r, R[i]);
+            if (!(ref > 0.0)) continue;
+            if (WQ[i] < merge_limit * ref) {
+                uint32_t k[3];
+                ora_state_key(Z[i], R[i], P[2][i], P[3][i], Vt[i], k);
+                double u = ora_uniform(seed, step, ORA_MERGE_TAG | (uint32_t)s, k[0], k[1], k[2], 0);
+                if (u < WQ[i] / ref) {
+                    WQ[i] = ref;
+                    stats[1] += 1;
+                } else {
+                    for (int f = 0; f < 8; ++f) P[f][i] = P[f][*n - 1];
+                    *n -= 1;
+                    stats[2] += 1;
+                }
+            }
+        }
+    }
+    return 0;
+}
 
 /* ------------------------------------------------------- node volumes */
 void ora_node_volumes(int nz, int nr, double dx, const double *factor, double *vol, double *inv_vol)

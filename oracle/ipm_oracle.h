@@ -72,6 +72,11 @@ void ora_deposit_finish(int nz, int nr, double dt, double charge, double mass,
  * PICCom.F90:516-578): phi (1:Nz,1:Nr) -> Ez (0:Nz,1:Nr), Er (1:Nz,0:Nr). */
 void ora_set_e(int nz, int nr, double dx, const double *phi, double *Ez, double *Er);
 
+/* split_particle_weight + merge_particle_random (Rezoning.F90:53-92) on one
+ * bundle; particle arrays hold cap slots.  -1 when copies exceed cap. */
+int ora_split_merge(int64_t *n, double *P[8], int64_t cap, int mode, double weight, double alpha,
+                    double grid_r, double merge_limit, uint64_t seed, uint32_t step, int s, int64_t stats[3]);
+
 /* WeightingParticleField for one species (diagParticleField.F90:196-318):
  * rho, energy (mean eV where rho > 0), mp, jz, jr, each nz*nr. */
 void ora_moments(int nz, int nr, double dx, double dt, double charge, double mass, const double *Ez,

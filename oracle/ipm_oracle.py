@@ -59,6 +59,9 @@ def lib(fast=False):
     L.ora_deposit_raw.argtypes = [C.c_int, C.c_int, C.c_int64, _dp, _dp, _dp, _dp]
     L.ora_deposit_finish.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _dp]
     L.ora_set_e.argtypes = [C.c_int, C.c_int, C.c_double, _dp, _dp, _dp]
+    L.ora_split_merge.argtypes = [_i64p, C.POINTER(_dp), C.c_int64, C.c_int, C.c_double, C.c_double, C.c_double,
+                                  C.c_double, C.c_uint64, C.c_uint32, C.c_int, _i64p]
+    L.ora_split_merge.restype = C.c_int
     L.ora_moments.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp,
                               C.c_int, C.c_int64, C.POINTER(_dp)] + [_dp] * 6
     L.ora_mcc.restype = C.c_int
@@ -167,6 +170,30 @@ def absorb(geom, charge, P, fast=False):
     sg = np.zeros(geom.nz * geom.nr)
     lib(fast).ora_absorb(C.byref(g.s), charge, C.byref(n), ptrs, _d(mc), _d(sg))
     return P[:, :n.value].copy(), mc[:geom.nmetal], unF(sg, geom.nz, geom.nr)
+
+
+def split_merge(P, weight, alpha, grid_r, merge_limit, seed, step, s, mode=3, fast=False):
+    """split_particle_weight then merge_particle_random (Rezoning.F90:53-92)
+    on a copy of P.  Returns (P_out, stats[copies, kept, deleted])."""
+    P = np.asarray(P, dtype=np.float64)
+    n0 = P.shape[1]
+    extra = 0
+    if mode & 1 and n0:
+        ref = weight * ((1.0 - alpha) * P[1] / grid_r + alpha)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            ratio = np.where(ref > 0, P[7] / np.where(ref > 0, ref, 1.0), 0.0)
+        extra = int(np.floor(ratio[ratio >= 2.0]).sum() - np.count_nonzero(ratio >= 2.0)) + 1
+    cap = n0 + extra
+    Q = np.zeros((8, max(cap, 1)))
+    Q[:, :n0] = P
+    n = C.c_int64(n0)
+    st = np.zeros(3, dtype=np.int64)
+    ptrs = (_dp * 8)(*[_d(Q[f]) for f in range(8)])
+    rc = lib(fast).ora_split_merge(C.byref(n), ptrs, cap, mode, weight, alpha, grid_r, merge_limit, seed, step, s,
+                                   st.ctypes.data_as(_i64p))
+    if rc:
+        raise ValueError("split copies exceed the capacity")
+    return Q[:, :n.value].copy(), st
 
 
 def deposit_raw(nz, nr, P, fast=False):
