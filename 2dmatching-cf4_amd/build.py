@@ -40,7 +40,7 @@ def run(cmd, cwd=None):
 
 def build_hip():
     os.makedirs(LIBDIR, exist_ok=True)
-    srcs = [os.path.join(CSRC, f) for f in ("ipm_hip.hip", "ipm_window.hip")]
+    srcs = [os.path.join(CSRC, f) for f in ("ipm_hip.hip", "ipm_window.hip", "ipm_recombine.hip")]
     tmp = LIB + ".tmp"
     run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC, *srcs, "-o", tmp])
     os.replace(tmp, LIB)

@@ -11,8 +11,11 @@
 using namespace ipm;
 
 struct ipm_window;
+struct ipm_recomb;
 // Frees the differentiable-window buffers of a context (ipm_window.hip).
 int window_release(struct ipm_ctx *c);
+// Frees the recombination work buffers of a context (ipm_recombine.hip).
+int recomb_release(struct ipm_ctx *c);
 
 inline thread_local char g_err[512] = "";
 
@@ -74,6 +77,8 @@ struct ipm_ctx {
     bool geometry_set = false, fields_set = false;
     // differentiable-window state (ipm_window_* entry points), or null
     ipm_window *win = nullptr;
+    // recombination work buffers (ipm_recombine), or null
+    ipm_recomb *recomb = nullptr;
 };
 
 inline int sync_state(ipm_ctx *c)
@@ -92,6 +97,7 @@ inline int read_counters(ipm_ctx *c, DevCounters *out)
         return fail(IPM_ECAPACITY, "The size of particle bundle reaches the maximum (MCC products dropped).");
     if (out->overflow & 2) return fail(IPM_EINVAL, "MCC product with an invalid species index.");
     if (out->overflow & 4) return fail(IPM_ECAPACITY, "collision queue overflow (selections were not collided)");
+    if (out->overflow & 8) return fail(IPM_EINVAL, "dead-slot count disagrees with the particle arrays");
     return IPM_OK;
 }
 

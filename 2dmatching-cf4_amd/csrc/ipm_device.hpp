@@ -78,7 +78,7 @@ struct DevCounters {
     unsigned long long seed;
     unsigned int step;
     unsigned int blocks_done;
-    int overflow;                          // 1 capacity, 2 bad species, 4 queue
+    int overflow;                          // 1 capacity, 2 bad species, 4 queue, 8 dead-slot count
     int pad;
 };
 
@@ -478,6 +478,15 @@ __device__ __forceinline__ int select_probability_reactive(const Mc &m, const De
         return select_probability(m, t, rg);
     }
     return 1;
+}
+
+// Cell (ceiling(Z), ceiling(R)) of a particle, clamped into the grid's
+// cells, as a 0-based index (iz - 1) + (ir - 1) * (nz - 1).
+__device__ __forceinline__ unsigned long long cell_of(const DevState &st, double z, double r)
+{
+    int iz = min(max((int)ceil(z), 1), st.nz - 1);
+    int ir = min(max((int)ceil(r), 1), st.nr - 1);
+    return (unsigned long long)(iz - 1) + (unsigned long long)(ir - 1) * (st.nz - 1);
 }
 
 }  // namespace ipm

@@ -924,12 +924,6 @@ __global__ void rezone_fold_kernel(DevCounters *ctr, int s, long long cap)
 }
 
 // ------------------------------------------------------------ cell sort
-__device__ __forceinline__ unsigned long long cell_of(const DevState &st, double z, double r)
-{
-    int iz = min(max((int)ceil(z), 1), st.nz - 1);
-    int ir = min(max((int)ceil(r), 1), st.nr - 1);
-    return (unsigned long long)(iz - 1) + (unsigned long long)(ir - 1) * (st.nz - 1);
-}
 
 // Wave-aggregated increment: lanes with equal keys share one atomic.
 // Returns each lane's position among the increments made to its key.
@@ -1215,6 +1209,7 @@ int ipm_destroy(ipm_ctx *c)
     if (!c) return IPM_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     (void)window_release(c);
+    (void)recomb_release(c);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     for (int b = 0; b < 2; ++b)

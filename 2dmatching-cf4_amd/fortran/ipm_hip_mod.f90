@@ -186,6 +186,16 @@ module ipm_hip
             integer(c_int), value :: s, implicit_j
             real(c_double), intent(inout) :: rho(*), energy_ev(*), mp(*), jz(*), jr(*)
         end function
+        integer(c_int) function ipm_recombine(ctx, nreaction, reaction_list, rate, time_step, deleted) &
+                bind(C, name="ipm_recombine")
+            import :: c_ptr, c_int, c_double, c_int64_t
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nreaction
+            integer(c_int), intent(in) :: reaction_list(*)
+            real(c_double), intent(in) :: rate(*)
+            real(c_double), value :: time_step
+            integer(c_int64_t), intent(inout) :: deleted(*)
+        end function
         integer(c_int) function ipm_split_merge(ctx, s, mode, weight, alpha, grid_r, merge_limit, stats) &
                 bind(C, name="ipm_split_merge")
             import :: c_ptr, c_int, c_double, c_int64_t
@@ -319,6 +329,19 @@ contains
         integer(c_int64_t) :: stats(3)
         call ipm_check(ipm_split_merge(ctx, int(k, c_int), 3_c_int, Weight, alpha, grid_r, merge_limit, stats), &
                        "SplitMerge")
+    end subroutine
+
+    ! RecombinationLocal%recb(Ns, PB, time_step) (R/src/mc/Recombination.F90:164-243)
+    ! with Re2d%recb_reaction(reaction_num, 3) (0-based species, as stored) and
+    ! Re2d%recb_rate; time_step = recombination_time_step_number * dt.
+    subroutine ipm_Recombination(ctx, ns, recb_reaction, recb_rate, time_step)
+        type(c_ptr), intent(in) :: ctx
+        integer, intent(in) :: ns
+        integer(c_int), intent(in) :: recb_reaction(:, :)
+        real(c_double), intent(in) :: recb_rate(:), time_step
+        integer(c_int64_t) :: deleted(ns + 1)
+        call ipm_check(ipm_recombine(ctx, int(size(recb_reaction, 1), c_int), recb_reaction, recb_rate, time_step, &
+                                     deleted), "Recombination")
     end subroutine
 
     ! PB(k)%NPar

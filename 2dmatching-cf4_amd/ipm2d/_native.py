@@ -69,6 +69,7 @@ SIGNATURES = {
     "ipm_species_device_ptrs": [_vp, C.c_int, C.POINTER(_dp)],
     "ipm_set_potential": [_vp, _dp],
     "ipm_moments": [_vp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp],
+    "ipm_recombine": [_vp, C.c_int, _ip, _dp, C.c_double, _i64p],
     "ipm_split_merge": [_vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _i64p],
     "ipm_window_create": [_vp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp],
     "ipm_window_forward": [_vp, _dp, C.c_int, _dp],

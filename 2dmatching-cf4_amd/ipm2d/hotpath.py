@@ -12,6 +12,7 @@ the reference's names and argument meaning:
     moments(s)                     WeightingParticleField      diagParticleField.F90:173
     split_merge(s, ...)            split_particle_weight +     Rezoning.F90:53-92
                                    merge_particle_random
+    recombine(reactions, ...)      RecombinationLocal%recb     Recombination.F90:164-273
 
 Errors raise ``IPMError`` (``CapacityError`` where the reference would
 ``stop`` on a full particle bundle).  Grids are returned as numpy arrays
@@ -209,6 +210,22 @@ class HotPath:
         N.call("ipm_split_merge", self._h, s, mode, float(weight), float(alpha), float(grid_r), float(merge_limit),
                st.ctypes.data_as(C.POINTER(C.c_int64)))
         return tuple(int(x) for x in st)
+
+    def recombine(self, reactions, rates, time_step):
+        """executingRecombination2D (Recombination.F90:164-273).  reactions:
+        (first, second, type) triples with 0-based species, type 0 constant
+        rate, 1 temperature form; rates in m^3/s; time_step =
+        recombination_time_step_number * dt.  Returns the macro-particles
+        deleted per species."""
+        rl = np.asarray(reactions, dtype=np.int32).reshape(-1, 3)
+        flat = np.ascontiguousarray(rl.T.reshape(-1))       # recb_reaction(n, 3), Fortran order
+        rt = np.ascontiguousarray(np.asarray(rates, dtype=np.float64).reshape(-1))
+        if rt.size != rl.shape[0]:
+            raise ValueError("one rate per reaction")
+        out = np.zeros(self.ns, dtype=np.int64)
+        N.call("ipm_recombine", self._h, rl.shape[0], _i(flat), _d(rt), float(time_step),
+               out.ctypes.data_as(C.POINTER(C.c_int64)))
+        return out
 
     def mcc(self):
         N.call("ipm_mcc", self._h)

@@ -185,6 +185,23 @@ int ipm_moments(ipm_ctx *ctx, int s, int implicit_j, double *rho, double *energy
 int ipm_split_merge(ipm_ctx *ctx, int s, int mode, double weight, double alpha, double grid_r,
                     double merge_limit, int64_t *stats);
 
+/* Volume recombination: RecombinationLocal%recb(Ns, PB, time_step), i.e.
+ * executingRecombination2D + deletingParticles (R/src/mc/Recombination.F90:164-273),
+ * with the reactions initRecombination2D stores (Recombination.F90:43-140).
+ * reaction_list is recb_reaction(nreaction, 3) in Fortran order with 0-based
+ * species: first species of reaction i at [i], second at [nreaction + i],
+ * type at [2*nreaction + i] (0 constant rate, 1 temperature form); rate[i]
+ * is recb_rate(i); time_step = recombination_time_step_number * dt.  Cell
+ * densities (and inverse mean energies for type 1 species) are formed once,
+ * then each reaction removes its budget from both species' particles of
+ * every cell, visiting them from the highest storage slot down; the draw is
+ * the particle's state-keyed uniform.  Each involved bundle is first
+ * reordered by cell, keeping the slot order inside a cell and dropping dead
+ * slots; deleted particles become dead slots.  deleted (ns entries, may be
+ * NULL) receives the macro-particles deleted per species. */
+int ipm_recombine(ipm_ctx *ctx, int nreaction, const int *reaction_list, const double *rate, double time_step,
+                  int64_t *deleted);
+
 /* ---- Differentiable collisionless window ----------------------------------
  * No counterpart in the reference (which has no gradients).  Advances a
  * snapshot of the resident particles through nsteps steps of MoveESIm

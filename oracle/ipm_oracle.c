@@ -27,6 +27,13 @@
  *                         Maxwellian mean energy, energy bookkeeping of
  *                         excitation/ionization, momentum of elastic events
  *   ora_sigma_* / ora_probability_*  closed forms on synthetic tables
+ *   ora_recombine         hand-computed single-cell budgets (constant and
+ *                         temperature forms), the visiting order's two
+ *                         outcomes at their closed-form frequencies, cells
+ *                         independent of their interleaving, passes of
+ *                         later reactions skipping earlier deletions;
+ *                         ora_recomb_density: closed-form densities and
+ *                         mean energies
  *   ora_split_merge       hand-computed splits (copy counts, remainders,
  *                         total weight), merge keep fraction = WQ/ref within
  *                         binomial bounds, expected weight conserved, the
@@ -177,6 +184,150 @@ int ora_split_merge(int64_t *n, double *P[8], int64_t cap, int mode, double weig
             }
         }
     }
+    return 0;
+}
+
+/* ------------------------------------------------------ recombination */
+#define ORA_RECOMB_TAG 0x52430000u
+
+static int64_t recomb_cell(int nz, int nr, double z, double r)
+{
+    int iz = (int)ceil(z), ir = (int)ceil(r);
+    iz = iz < 1 ? 1 : (iz > nz - 1 ? nz - 1 : iz);
+    ir = ir < 1 ? 1 : (ir > nr - 1 ? nr - 1 : ir);
+    return (int64_t)(iz - 1) + (int64_t)(ir - 1) * (nz - 1);
+}
+
+/* Volume of radial cell ir, initRecombination2D (Recombination.F90:133-137). */
+static double recomb_volume(double dx, int ir)
+{
+    return C_PI * dx * dx * dx * (double)(ir * ir - (ir - 1) * (ir - 1));
+}
+
+/* Cell number density and, when want_t, inverse mean energy (eV) of one
+ * bundle: executingRecombination2D, Recombination.F90:175-206.  dens and
+ * inv_t hold (nz-1)*(nr-1) cells. */
+void ora_recomb_density(int nz, int nr, double dx, double dt, double mass, int want_t, int64_t n,
+                        double *const P[8], double *dens, double *inv_t)
+{
+    const int64_t ncell = (int64_t)(nz - 1) * (nr - 1);
+    const double vf = dx / dt;
+    for (int64_t c = 0; c < ncell; ++c) {
+        dens[c] = 0.0;
+        if (want_t) inv_t[c] = 0.0;
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t c = recomb_cell(nz, nr, P[0][i], P[1][i]);
+        dens[c] = dens[c] + P[7][i];
+        if (want_t) {
+            double energy = P[2][i] * P[2][i] + P[3][i] * P[3][i] + P[4][i] * P[4][i];
+            inv_t[c] = inv_t[c] + energy * P[7][i];
+        }
+    }
+    for (int64_t c = 0; c < ncell; ++c) {
+        int ir = (int)(c / (nz - 1)) + 1;
+        double inv_vol = 1.0 / recomb_volume(dx, ir);
+        dens[c] = dens[c] * inv_vol;
+        if (want_t) {
+            inv_t[c] = inv_t[c] * inv_vol * 0.5 * mass * vf * vf;
+            if (dens[c] > 0.0 && inv_t[c] > 0.0)
+                inv_t[c] = 1.0 / (inv_t[c] / C_JTOEV / dens[c]);
+            else
+                inv_t[c] = 0.0;
+        }
+    }
+}
+
+/* deletingParticles, Recombination.F90:246-273, visiting i = n-1 .. 0 and
+ * skipping particles marked by earlier passes of the same call; deletion is
+ * a mark (Z = -1e300). */
+static int64_t recomb_delete(int nz, int nr, int64_t n, double *const P[8], double *recb, uint64_t seed,
+                             uint32_t step, uint32_t tag)
+{
+    int64_t cnt = 0;
+    for (int64_t i = n - 1; i >= 0; --i) {
+        if (P[0][i] < -1.0e299) continue;
+        int64_t c = recomb_cell(nz, nr, P[0][i], P[1][i]);
+        if (recb[c] > P[7][i]) {
+            recb[c] = recb[c] - P[7][i];
+            P[0][i] = -1.0e300;
+            ++cnt;
+        } else if (recb[c] > 0.0) {
+            uint32_t k[3];
+            ora_state_key(P[0][i], P[1][i], P[2][i], P[3][i], P[4][i], k);
+            double u = ora_uniform(seed, step, tag, k[0], k[1], k[2], 0);
+            if (u < recb[c] / P[7][i]) {
+                recb[c] = 0.0;
+                P[0][i] = -1.0e300;
+                ++cnt;
+            }
+        }
+    }
+    return cnt;
+}
+
+/* executingRecombination2D, Recombination.F90:164-243, on ns bundles of
+ * n[s] particles (P holds 8 arrays per species, species-major).  reactions
+ * is recb_reaction(nreaction, 3) in Fortran order with 0-based species;
+ * type 0 uses a constant rate, type 1 the temperature form.  The draw of
+ * reaction i's pass over species x (role 0 = first, 1 = second species) is
+ * the particle's state-keyed uniform under tag ORA_RECOMB_TAG | i<<4 |
+ * role<<3 | x.  Survivors are compacted, order kept.  Returns -1 on a bad
+ * reaction. */
+int ora_recombine(int nz, int nr, double dx, double dt, int ns, int64_t *n, double *const *P, const double *mass,
+                  int nreaction, const int *reactions, const double *rate, double time_step, uint64_t seed,
+                  uint32_t step, int64_t *deleted)
+{
+    const int64_t ncell = (int64_t)(nz - 1) * (nr - 1);
+    int involved[16] = {0}, want_t[16] = {0};
+    if (ns > 16) return -1;
+    for (int i = 0; i < nreaction; ++i) {
+        int a = reactions[i], b = reactions[nreaction + i], t = reactions[2 * nreaction + i];
+        if (a < 0 || a >= ns || b < 0 || b >= ns || (t != 0 && t != 1)) return -1;
+        involved[a] = involved[b] = 1;
+        if (t == 1) want_t[a] = want_t[b] = 1;
+    }
+    double *dens = (double *)calloc((size_t)(ns * ncell), sizeof(double));
+    double *inv_t = (double *)calloc((size_t)(ns * ncell), sizeof(double));
+    double *recb_neg = (double *)malloc(sizeof(double) * (size_t)ncell);
+    double *recb_pos = (double *)malloc(sizeof(double) * (size_t)ncell);
+    for (int s = 0; s < ns; ++s) {
+        deleted[s] = 0;
+        if (involved[s])
+            ora_recomb_density(nz, nr, dx, dt, mass[s], want_t[s], n[s], P + 8 * s, dens + s * ncell,
+                               inv_t + s * ncell);
+    }
+    for (int i = 0; i < nreaction; ++i) {
+        int a = reactions[i], b = reactions[nreaction + i], t = reactions[2 * nreaction + i];
+        const double *da = dens + a * ncell, *db = dens + b * ncell, *ta = inv_t + a * ncell, *tb = inv_t + b * ncell;
+        for (int64_t c = 0; c < ncell; ++c) {
+            double vol = recomb_volume(dx, (int)(c / (nz - 1)) + 1);
+            if (t == 0)
+                recb_neg[c] = da[c] * db[c] * rate[i] * time_step * vol;
+            else
+                recb_neg[c] = da[c] * db[c] * time_step * vol * rate[i] * sqrt(ta[c]) * tb[c];
+            recb_pos[c] = recb_neg[c];
+        }
+        deleted[a] += recomb_delete(nz, nr, n[a], P + 8 * a, recb_neg, seed, step,
+                                    ORA_RECOMB_TAG | ((uint32_t)i << 4) | (uint32_t)a);
+        deleted[b] += recomb_delete(nz, nr, n[b], P + 8 * b, recb_pos, seed, step,
+                                    ORA_RECOMB_TAG | ((uint32_t)i << 4) | (1u << 3) | (uint32_t)b);
+    }
+    for (int s = 0; s < ns; ++s) {
+        if (!involved[s]) continue;
+        double *const *Q = P + 8 * s;
+        int64_t j = 0;
+        for (int64_t i = 0; i < n[s]; ++i) {
+            if (Q[0][i] < -1.0e299) continue;
+            for (int f = 0; f < 8; ++f) Q[f][j] = Q[f][i];
+            ++j;
+        }
+        n[s] = j;
+    }
+    free(dens);
+    free(inv_t);
+    free(recb_neg);
+    free(recb_pos);
     return 0;
 }
 

@@ -72,6 +72,14 @@ void ora_deposit_finish(int nz, int nr, double dt, double charge, double mass,
  * PICCom.F90:516-578): phi (1:Nz,1:Nr) -> Ez (0:Nz,1:Nr), Er (1:Nz,0:Nr). */
 void ora_set_e(int nz, int nr, double dx, const double *phi, double *Ez, double *Er);
 
+/* Recombination (Recombination.F90:164-273): cell densities of one bundle,
+ * then the whole call on ns bundles (P: 8 arrays per species). */
+void ora_recomb_density(int nz, int nr, double dx, double dt, double mass, int want_t, int64_t n,
+                        double *const P[8], double *dens, double *inv_t);
+int ora_recombine(int nz, int nr, double dx, double dt, int ns, int64_t *n, double *const *P, const double *mass,
+                  int nreaction, const int *reactions, const double *rate, double time_step, uint64_t seed,
+                  uint32_t step, int64_t *deleted);
+
 /* split_particle_weight + merge_particle_random (Rezoning.F90:53-92) on one
  * bundle; particle arrays hold cap slots.  -1 when copies exceed cap. */
 int ora_split_merge(int64_t *n, double *P[8], int64_t cap, int mode, double weight, double alpha,

@@ -62,6 +62,11 @@ def lib(fast=False):
     L.ora_split_merge.argtypes = [_i64p, C.POINTER(_dp), C.c_int64, C.c_int, C.c_double, C.c_double, C.c_double,
                                   C.c_double, C.c_uint64, C.c_uint32, C.c_int, _i64p]
     L.ora_split_merge.restype = C.c_int
+    L.ora_recomb_density.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int64,
+                                     C.POINTER(_dp), _dp, _dp]
+    L.ora_recombine.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _i64p, C.POINTER(_dp), _dp,
+                                C.c_int, _ip, _dp, C.c_double, C.c_uint64, C.c_uint32, _i64p]
+    L.ora_recombine.restype = C.c_int
     L.ora_moments.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp,
                               C.c_int, C.c_int64, C.POINTER(_dp)] + [_dp] * 6
     L.ora_mcc.restype = C.c_int
@@ -170,6 +175,38 @@ def absorb(geom, charge, P, fast=False):
     sg = np.zeros(geom.nz * geom.nr)
     lib(fast).ora_absorb(C.byref(g.s), charge, C.byref(n), ptrs, _d(mc), _d(sg))
     return P[:, :n.value].copy(), mc[:geom.nmetal], unF(sg, geom.nz, geom.nr)
+
+
+def recomb_density(nz, nr, dx, dt, mass, P, want_t=True, fast=False):
+    """Cell number density and inverse mean energy (1/eV) of one bundle
+    (Recombination.F90:175-206), each (nz-1, nr-1)."""
+    P = np.ascontiguousarray(P, dtype=np.float64)
+    ncell = (nz - 1) * (nr - 1)
+    d = np.zeros(ncell)
+    t = np.zeros(ncell)
+    ptrs = (_dp * 8)(*[_d(P[f]) for f in range(8)])
+    lib(fast).ora_recomb_density(nz, nr, dx, dt, mass, 1 if want_t else 0, P.shape[1], ptrs, _d(d), _d(t))
+    return d.reshape(nr - 1, nz - 1).T, t.reshape(nr - 1, nz - 1).T
+
+
+def recombine(nz, nr, dx, dt, species, masses, reactions, rates, time_step, seed, step, fast=False):
+    """executingRecombination2D (Recombination.F90:164-243) on copies of the
+    bundles.  reactions: (neg, pos, type) triples with 0-based species.
+    Returns (bundles_out, deleted per species)."""
+    ns = len(species)
+    Ps = [np.array(P, dtype=np.float64, copy=True, order="C") for P in species]
+    n = np.array([P.shape[1] for P in Ps], dtype=np.int64)
+    ptrs = (_dp * (8 * ns))(*[_d(Ps[s][f]) for s in range(ns) for f in range(8)])
+    rl = np.asfortranarray(np.asarray(reactions, dtype=np.int32).reshape(-1, 3))
+    flat = np.ascontiguousarray(rl.T.reshape(-1))          # Fortran (nreaction, 3) order
+    rt = np.ascontiguousarray(np.asarray(rates, dtype=np.float64))
+    ms = np.ascontiguousarray(np.asarray(masses, dtype=np.float64))
+    deleted = np.zeros(ns, dtype=np.int64)
+    rc = lib(fast).ora_recombine(nz, nr, dx, dt, ns, n.ctypes.data_as(_i64p), ptrs, _d(ms), rl.shape[0], _i(flat),
+                                 _d(rt), time_step, seed, step, deleted.ctypes.data_as(_i64p))
+    if rc:
+        raise ValueError("invalid recombination reaction")
+    return [Ps[s][:, :n[s]].copy() for s in range(ns)], deleted
 
 
 def split_merge(P, weight, alpha, grid_r, merge_limit, seed, step, s, mode=3, fast=False):
