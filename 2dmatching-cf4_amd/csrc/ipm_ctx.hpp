@@ -62,6 +62,11 @@ struct ipm_ctx {
                        *partial2 = nullptr;
     double **d_dst = nullptr;
     double *qbuf = nullptr;                // q_cap records of 8 doubles
+    double *cbuf = nullptr;                // collision records, kCollFields arrays of q_cap doubles
+    long long *c_base = nullptr;           // per collision block: record start, count,
+    unsigned *c_count = nullptr;           //   staged products and their meta / counts
+    double *p_stage = nullptr;
+    unsigned *p_meta = nullptr, *p_cnt = nullptr;
     unsigned *q_tile_count = nullptr;      // q_ntiles
     long long q_cap = 0, q_ntiles = 0;
     double *mom = nullptr;                 // moment sums, 5 * nz * nr (ipm_moments)

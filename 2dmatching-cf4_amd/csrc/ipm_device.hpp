@@ -62,11 +62,26 @@ struct DevState {
     // its index inside the tile.  q_tile_count[q_cbase[s] + lt] is that
     // tile's number of entries in the current pass.  The layout needs no
     // global counter: a tile appends with an LDS counter only.  Entries are
-    // 64-byte records (Z R Vz Vr Vt Az Ar, index in tile), written and read whole.
+    // 64-byte records (Z R Vz Vr Vt WQ, index in tile, pad), written and read whole.
     double *q_rec;
     unsigned int *q_tile_count;
     long long q_region[IPM_NS_MAX], q_cbase[IPM_NS_MAX], q_tile_cap[IPM_NS_MAX];
     double q_weight[IPM_NS_MAX];       // expected entries per tile (collision-pass split)
+    // Collisions (non-null reactions) found by the collision pass and
+    // finished by the finishing pass: kCollFields arrays of c_cap doubles
+    // (structure of arrays), one record per collision.
+    double *c_rec;
+    long long c_cap;
+    // Per collision block (c_grid blocks): its records' start and count,
+    // and the products it staged (kProdStage records of 8 doubles, each
+    // with species | rank << 8, and per-species counts) for the append pass.
+    long long *c_base;
+    unsigned *c_count;
+    double *p_stage;
+    unsigned *p_meta;
+    unsigned *p_cnt;
+    int c_grid;
+    int c_pad;
     DevTable tab[IPM_NS_MAX * IPM_NG_MAX];
 };
 

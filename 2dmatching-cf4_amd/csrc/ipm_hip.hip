@@ -30,22 +30,37 @@ constexpr int kWinR = 8;             // LDS deposit window (nodes in r)
 
 enum : int { M_PUSH = 1, M_ABSORB = 2, M_DEPOSIT = 4, M_MCC = 8 };
 
-// A collision-queue record: Z R Vz Vr Vt Az Ar and the particle's index in
-// its tile, 64 contiguous bytes moved as four 16-byte accesses (WQ is read
-// from the particle arrays when the record is used).
+// A collision-queue record: Z R Vz Vr Vt WQ, the particle's index in its
+// tile and a pad, 64 contiguous bytes moved as four 16-byte accesses.  It
+// holds everything the selection, the target sampling and the reaction
+// choice read, so the collision pass makes no second, dependent read of the
+// particle; Az and Ar (read back only for real reactions) stay out.
+constexpr int kRecD = 8;
+constexpr int kRecParts = kRecD / 2;
 typedef double rec2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ unsigned load_rec(const double *r, Part &p)
 {
     const auto *v = (const __attribute__((address_space(1))) rec2 *)r;
     const rec2 a = v[0], b = v[1], c = v[2], d = v[3];
-    p.Z = a.x; p.R = a.y; p.Vz = b.x; p.Vr = b.y; p.Vt = c.x; p.Az = c.y; p.Ar = d.x;
-    return (unsigned)__double_as_longlong(d.y);
+    p.Z = a.x; p.R = a.y; p.Vz = b.x; p.Vr = b.y; p.Vt = c.x; p.WQ = c.y; p.Az = 0.0; p.Ar = 0.0;
+    return (unsigned)__double_as_longlong(d.x);
 }
 
 __device__ __forceinline__ bool is_dead(double z) { return z < -1.0e299; }
 
 // ------------------------------------------------------------ products
-__device__ __forceinline__ void emit_product(const DevState &st, DevCounters *ctr, Part q, int d)
+// Products of one collision block, staged in LDS and appended by
+// collide_append_kernel without any contended atomic.
+constexpr int kProdStage = 256;
+struct ProdSink {
+    double *rec;        // kProdStage x 8 doubles
+    unsigned *meta;     // species | rank << 8
+    unsigned *count;    // staged so far
+    unsigned *spc;      // staged per species (IPM_NS_MAX)
+};
+
+__device__ __forceinline__ void emit_product(const DevState &st, DevCounters *ctr, Part q, int d,
+                                             const ProdSink *ps = nullptr)
 {
     if (d < 0 || d >= st.ns) {
         atomicOr(&ctr->overflow, 2);
@@ -54,6 +69,17 @@ __device__ __forceinline__ void emit_product(const DevState &st, DevCounters *ct
     q.Vz = q.Vz * st.inv_vfactor;
     q.Vr = q.Vr * st.inv_vfactor;
     q.Vt = q.Vt * st.inv_vfactor;
+    if (ps) {
+        const unsigned k = atomicAdd(ps->count, 1u);
+        if (k < (unsigned)kProdStage) {
+            const unsigned r = atomicAdd(&ps->spc[d], 1u);
+            double *o = ps->rec + (size_t)k * 8;
+            o[0] = q.Z; o[1] = q.R; o[2] = q.Vz; o[3] = q.Vr; o[4] = q.Vt; o[5] = q.Az; o[6] = q.Ar; o[7] = q.WQ;
+            ps->meta[k] = (unsigned)d | (r << 8);
+            return;
+        }
+    }
+    // no sink, or the block's stage is full: reserve a slot directly
     unsigned long long k = atomicAdd(&ctr->n_new[d], 1ull);
     unsigned long long pos = ctr->n[d] + k;
     if (pos >= (unsigned long long)st.cap[d]) {
@@ -79,11 +105,11 @@ __device__ __forceinline__ void ion_reactive(Mc &m, double th)
 
 // One selected particle of species s against gas table t: the body of the
 // MCC loop, R/src/mc/MCCPublic.F90:61-97, with SelectCollisionElectron
-// (R/src/mc/MCCElectron.F90:13-147) and SelectCollisionIon (R/src/mc/MCCIon.F90:6-51).
-__device__ __forceinline__ void mcc_collide(const DevState &st, DevCounters *ctr, int s, int g,
-                                         const DevTable &t, Part &p, Rng &rg)
+// (R/src/mc/MCCElectron.F90:13-147) and SelectCollisionIon (R/src/mc/MCCIon.F90:6-51),
+// in two halves.  mcc_begin: VelRes(VFactor), Updater and the reaction
+// choice; it returns the reaction index (0 = null event).
+__device__ __forceinline__ int mcc_begin(const DevState &st, int s, const DevTable &t, const Part &p, Mc &m, Rng &rg)
 {
-    Mc m;
     m.poi = p;
     m.poi.Vz = m.poi.Vz * st.vfactor;
     m.poi.Vr = m.poi.Vr * st.vfactor;
@@ -92,6 +118,14 @@ __device__ __forceinline__ void mcc_collide(const DevState &st, DevCounters *ctr
     int idx = 0;
     if (t.model == 1 || t.model == 2) idx = select_probability(m, t, rg);
     else if (t.model == 3) idx = select_probability_reactive(m, t, rg);
+    return idx;
+}
+
+// mcc_finish: the collision of reaction idx (none for idx 0), the event
+// count and VelRes(1/VFactor) back into p.
+__device__ __forceinline__ void mcc_finish(const DevState &st, DevCounters *ctr, int s, int g, const DevTable &t,
+                                           Mc &m, Rng &rg, int idx, Part &p, const ProdSink *ps = nullptr)
+{
     if (idx > 0) {
         const int ty = t.rtype[idx - 1];
         const int res = t.resultant[idx - 1];
@@ -122,19 +156,19 @@ __device__ __forceinline__ void mcc_collide(const DevState &st, DevCounters *ctr
                 solid.Vz = solid.Vz * VF2;
                 solid.Vr = solid.Vr * VF2;
                 solid.Vt = solid.Vt * VF2;
-                emit_product(st, ctr, solid, s);
-                emit_product(st, ctr, m.pot, res);
+                emit_product(st, ctr, solid, s, ps);
+                emit_product(st, ctr, m.pot, res, ps);
             } break;
             case 14:
-                emit_product(st, ctr, m.pot, res);
+                emit_product(st, ctr, m.pot, res, ps);
                 break;
             case 15: {
                 double VF = sqrt(1.0 - th / m.energy);
                 m.energy = m.energy - th;
                 mc_vel_update(m, VF);
                 post_collision(m, true, rg);
-                emit_product(st, ctr, m.pot, res % 10);
-                emit_product(st, ctr, m.pot, res / 10);
+                emit_product(st, ctr, m.pot, res % 10, ps);
+                emit_product(st, ctr, m.pot, res / 10, ps);
             } break;
             default:
                 break;
@@ -147,11 +181,11 @@ __device__ __forceinline__ void mcc_collide(const DevState &st, DevCounters *ctr
             case 111: ion_reactive(m, th); break;
             case 112:
                 ion_reactive(m, th);
-                emit_product(st, ctr, m.pot, res);
+                emit_product(st, ctr, m.pot, res, ps);
                 break;
             case 113:
                 ion_reactive(m, th);
-                emit_product(st, ctr, m.poi, res);
+                emit_product(st, ctr, m.poi, res, ps);
                 break;
             default: break;   // 100 and -1 change nothing
             }
@@ -163,6 +197,14 @@ __device__ __forceinline__ void mcc_collide(const DevState &st, DevCounters *ctr
     m.poi.Vr = m.poi.Vr * st.inv_vfactor;
     m.poi.Vt = m.poi.Vt * st.inv_vfactor;
     p = m.poi;
+}
+
+__device__ __forceinline__ void mcc_collide(const DevState &st, DevCounters *ctr, int s, int g,
+                                            const DevTable &t, Part &p, Rng &rg, const ProdSink *ps = nullptr)
+{
+    Mc m;
+    const int idx = mcc_begin(st, s, t, p, m, rg);
+    mcc_finish(st, ctr, s, g, t, m, rg, idx, p, ps);
 }
 
 __device__ __forceinline__ void lds_add(double *a, double v)
@@ -250,7 +292,7 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
     // the next tile never waits for the reset), and each wave's staging
     // buffer of up to 64 queue records, written out with full waves.
     __shared__ unsigned q_scount[2];
-    __shared__ double q_wst[(MODE & M_MCC) ? (kBlock / 64) * 64 * 8 : 1];
+    __shared__ double q_wst[(MODE & M_MCC) ? (kBlock / 64) * 64 * kRecD : 1];
 
     const int ns = st.ns;
     long long T = 0;
@@ -335,13 +377,13 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
             if (lane == 0) ob = atomicAdd(&q_scount[qbuf], cw);
             ob = __shfl(ob, 0);
             const long long R = st.q_tile_cap[s];
-            double *qt = st.q_rec + (size_t)(st.q_region[s] + (t - tbeg) * R) * 8;
-            const double *wst = q_wst + wv * 512;
-            for (unsigned j = lane; j < cw * 4u; j += 64u) {
-                const unsigned rec = j >> 2, part = j & 3u;
+            double *qt = st.q_rec + (size_t)(st.q_region[s] + (t - tbeg) * R) * kRecD;
+            const double *wst = q_wst + wv * 64 * kRecD;
+            for (unsigned j = lane; j < cw * (unsigned)kRecParts; j += 64u) {
+                const unsigned rec = j / (unsigned)kRecParts, part = j - rec * (unsigned)kRecParts;
                 if ((long long)(ob + rec) < R) {
-                    const rec2 v = {wst[rec * 8 + part * 2], wst[rec * 8 + part * 2 + 1]};
-                    *(__attribute__((address_space(1))) rec2 *)(qt + (size_t)(ob + rec) * 8 + part * 2) = v;
+                    const rec2 v = {wst[rec * kRecD + part * 2], wst[rec * kRecD + part * 2 + 1]};
+                    *(__attribute__((address_space(1))) rec2 *)(qt + (size_t)(ob + rec) * kRecD + part * 2) = v;
                 } else if (part == 0u) {
                     atomicOr(&ctr->overflow, 4);
                 }
@@ -429,14 +471,13 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
                 if (nsel) {
                     if (cw + nsel > 64u) flush_wave();
                     if (selected) {
-                        if (!(MODE & (M_PUSH | M_ABSORB))) {
-                            p.Az = gp(F[5])[i]; p.Ar = gp(F[6])[i];
-                        }
+                        if (!(MODE & (M_PUSH | M_ABSORB | M_DEPOSIT))) p.WQ = gp(F[7])[i];
                         const unsigned r = cw + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
-                        double *d = q_wst + wv * 512 + r * 8;
+                        double *d = q_wst + wv * 64 * kRecD + r * kRecD;
                         d[0] = p.Z; d[1] = p.R; d[2] = p.Vz; d[3] = p.Vr;
-                        d[4] = p.Vt; d[5] = p.Az; d[6] = p.Ar;
-                        d[7] = __longlong_as_double((long long)(i - base));
+                        d[4] = p.Vt; d[5] = p.WQ;
+                        d[6] = __longlong_as_double((long long)(i - base));
+                        d[7] = 0.0;
                     }
                     cw += nsel;
                 }
@@ -499,26 +540,66 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
 }
 
 // ------------------------------------------------------- collision pass
-// Finishes every queued particle: for each gas in order, the selection draw
-// is repeated and, if it hits, the collision of MCCPublic.F90:61-97 follows
-// on the state the previous gas left.  Results go back to the particle's
-// slot; products are appended to their species (MCCAddNewParticle,
-// R/src/mc/MCCPublic.F90:209-226).  The grid is split over the species in
+// Four kernels finish the queued particles, with no atomic that many
+// threads contend for (device-wide atomics on one address serialize, and
+// the reference's append of products, MCCAddNewParticle,
+// R/src/mc/MCCPublic.F90:209-226, would otherwise put one on every product).
+//   collide_kernel: for each gas in order, the selection draw is repeated
+//     and, if it hits, VelRes(VFactor), Updater and the reaction choice of
+//     MCCPublic.F90:61-97 follow.  A null event ends there (VelRes back,
+//     results to the particle's slot); a real reaction is written to a
+//     collision record in the block's own region (the region of its queue
+//     tiles) and the rest of the particle's work moves on.
+//   collide_finish_kernel: block b runs the records of collide_kernel's
+//     block b with full wavefronts (the reaction, VelRes back, the gases
+//     after it, the write-back) and stages the products in LDS, then in its
+//     own slice of a global stage.
+//   collide_append_kernel: block b appends its staged products after the
+//     products of blocks 0..b-1 of the same species.
+//   collide_fold_kernel: folds the step's products into n[] and advances the
+//     random stream's step key.
+// Most selections are null events, so the split also keeps the reaction
+// code (scattering angles, products) off the wavefronts that would mostly
+// idle in it.  collide_kernel splits its grid over the species in
 // proportion to their expected work (every species with live tiles gets a
 // block); a block takes a contiguous range of its species' tiles, scans
-// their entry counts in LDS and processes the entries flat.  Launched with a
-// fixed grid, so it can be captured.
+// their entry counts in LDS and processes the entries flat.  All four use
+// fixed grids, so they can be captured.
 constexpr int kChunk = 1024;   // tile counts scanned per LDS pass
+constexpr int kCollFields = 24;
+// Record fields: 0-5 poi, 6-11 pot (Z R Vz Vr Vt WQ), 12 miu, 13 mass_ratio,
+// 14 energy, 15-19 gx gy gz gper g, 20 stream counters c0|c1, 21 c2|draw,
+// 22 slot, 23 species | gas << 8 | reaction << 16.  Az and Ar are read back
+// from the particle (the queue record does not carry them) and pot's are
+// formed from them as Updater does.
 
-__global__ void __launch_bounds__(kBlock, 2)
+__device__ __forceinline__ void put_coll(const DevState &st, unsigned long long e, const Mc &m, const Rng &rg,
+                                         long long i, int s, int g, int idx)
+{
+    double *c = st.c_rec + e;
+    const long long C = st.c_cap;
+    const double v[kCollFields] = {
+        m.poi.Z, m.poi.R, m.poi.Vz, m.poi.Vr, m.poi.Vt, m.poi.WQ,
+        m.pot.Z, m.pot.R, m.pot.Vz, m.pot.Vr, m.pot.Vt, m.pot.WQ,
+        m.miu, m.mass_ratio, m.energy, m.gx, m.gy, m.gz, m.gper, m.g,
+        __longlong_as_double((long long)((unsigned long long)rg.c0 | ((unsigned long long)rg.c1 << 32))),
+        __longlong_as_double((long long)((unsigned long long)rg.c2 | ((unsigned long long)rg.k << 32))),
+        __longlong_as_double(i),
+        __longlong_as_double((long long)((unsigned)s | ((unsigned)g << 8) | ((unsigned)idx << 16)))};
+#pragma unroll
+    for (int f = 0; f < kCollFields; ++f) gp(c)[(size_t)f * C] = v[f];
+}
+
+__global__ void __launch_bounds__(kBlock, 3)
 collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
 {
     const DevState &st = *stp;
     __shared__ unsigned cpre[kChunk + 1];
     __shared__ unsigned wtot[kBlock / 64];
-    __shared__ int last_block;
+    __shared__ unsigned nrec;
     const int ns = st.ns;
     const unsigned G = gridDim.x;
+    if (threadIdx.x == 0) nrec = 0;
 
     // species of this block and its tile range
     int sb = -1;
@@ -552,6 +633,9 @@ collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
     }
     const uint32_t step = ctr->step;
     const unsigned long long seed = ctr->seed;
+    // this block's records go to the region of its queue tiles, which holds
+    // at least as many entries as the tiles can queue
+    const long long rbase = sb >= 0 ? st.q_region[sb] + t0 * st.q_tile_cap[sb] : 0;
     if (sb >= 0) {
         const int s = sb;
         const long long R = st.q_tile_cap[s];
@@ -597,60 +681,196 @@ collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
                 const long long lt = tc + lo;
                 const long long q = st.q_region[s] + lt * R + (e - cpre[lo]);
                 Part p;
-                const long long i = lt * kTile + load_rec(st.q_rec + (size_t)q * 8, p);
-                p.WQ = F[7][i];
+                const long long i = lt * kTile + load_rec(st.q_rec + (size_t)q * kRecD, p);
                 const StateKey sk = state_key(p.Z, p.R, p.Vz, p.Vr, p.Vt);
                 const Part p0 = p;
+                bool handed = false;
                 for (int g = 0; g < st.ng; ++g) {
                     const DevTable &tb = st.tab[s * st.ng + g];
                     if (tb.model == 0) continue;
                     Rng rg = make_rng(sk, seed, step, (uint32_t)s | ((uint32_t)g << 8));
-                    if (!(rg.next() < tb.ratio)) continue;
-                    mcc_collide(st, ctr, s, g, tb, p, rg);
+                    if (st.ng == 1) {
+                        rg.k = 1;   // the particle pass drew this same uniform and selected it
+                    } else if (!(rg.next() < tb.ratio)) {
+                        continue;
+                    }
+                    Mc mc;
+                    const int idx = mcc_begin(st, s, tb, p, mc, rg);
+                    if (idx == 0) {
+                        mcc_finish(st, ctr, s, g, tb, mc, rg, 0, p);
+                        continue;
+                    }
+                    // a real reaction: the rest of this particle goes to the
+                    // finishing pass (one LDS atomic per wavefront)
+                    const unsigned long long bal = __ballot(1);
+                    const int leader = __ffsll((long long)bal) - 1;
+                    unsigned off = 0;
+                    if (lane == leader) off = atomicAdd(&nrec, (unsigned)__popcll(bal));
+                    off = __shfl(off, leader);
+                    const unsigned e2 = off + (unsigned)__popcll(bal & ((1ull << lane) - 1ull));
+                    put_coll(st, (unsigned long long)(rbase + e2), mc, rg, i, s, g, idx);
+                    handed = true;
+                    break;
                 }
                 // write back only what the collisions changed (null events
-                // change nothing; Az/Ar change only in charge exchange): each
+                // change nothing but VelRes rounding; Az/Ar never): each
                 // field is a separate cache line of a separate array, stored
                 // non-temporally so the scattered lines do not crowd L2
-                if (__double_as_longlong(p.Vz) != __double_as_longlong(p0.Vz) ||
-                    __double_as_longlong(p.Vr) != __double_as_longlong(p0.Vr) ||
-                    __double_as_longlong(p.Vt) != __double_as_longlong(p0.Vt)) {
+                if (!handed && (__double_as_longlong(p.Vz) != __double_as_longlong(p0.Vz) ||
+                                __double_as_longlong(p.Vr) != __double_as_longlong(p0.Vr) ||
+                                __double_as_longlong(p.Vt) != __double_as_longlong(p0.Vt))) {
                     __builtin_nontemporal_store(p.Vz, &F[2][i]);
                     __builtin_nontemporal_store(p.Vr, &F[3][i]);
                     __builtin_nontemporal_store(p.Vt, &F[4][i]);
-                }
-                if (__double_as_longlong(p.Az) != __double_as_longlong(p0.Az) ||
-                    __double_as_longlong(p.Ar) != __double_as_longlong(p0.Ar)) {
-                    __builtin_nontemporal_store(p.Az, &F[5][i]);
-                    __builtin_nontemporal_store(p.Ar, &F[6][i]);
                 }
             }
             __syncthreads();
         }
     }
-    // The last block to finish folds this pass's products into n[] and
-    // advances the random stream's step key.
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned prev = atomicAdd(&ctr->blocks_done, 1u);
-        last_block = (prev == gridDim.x - 1);
+        st.c_base[blockIdx.x] = rbase;
+        st.c_count[blockIdx.x] = nrec;
+    }
+}
+
+// The records of collide_kernel's block b, one per thread; products are
+// staged in LDS and written to block b's slice of the product stage.
+__global__ void __launch_bounds__(kBlock, 2)
+collide_finish_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
+{
+    const DevState &st = *stp;
+    __shared__ double prec[kProdStage * 8];
+    __shared__ unsigned pmeta[kProdStage];
+    __shared__ unsigned pcount, pspc[IPM_NS_MAX];
+    if (threadIdx.x == 0) pcount = 0;
+    if (threadIdx.x < IPM_NS_MAX) pspc[threadIdx.x] = 0;
+    __syncthreads();
+    const ProdSink sink = {prec, pmeta, &pcount, pspc};
+    const long long rbase = st.c_base[blockIdx.x];
+    const unsigned nrec = st.c_count[blockIdx.x];
+    const uint32_t step = ctr->step;
+    const unsigned long long seed = ctr->seed;
+    const long long C = st.c_cap;
+    for (unsigned e = threadIdx.x; e < nrec; e += kBlock) {
+        const double *c = st.c_rec + rbase + e;
+        double v[kCollFields];
+#pragma unroll
+        for (int f = 0; f < kCollFields; ++f) v[f] = gp(c)[(size_t)f * C];
+        const long long i = __double_as_longlong(v[22]);
+        const unsigned tag = (unsigned)__double_as_longlong(v[23]);
+        const int s = (int)(tag & 0xffu), g = (int)((tag >> 8) & 0xffu), idx = (int)(tag >> 16);
+        double *const *F = st.f[s];
+        const double az0 = gp(F[5])[i], ar0 = gp(F[6])[i];
+        const DevTable &t0 = st.tab[s * st.ng + g];
+        Mc m;
+        m.poi.Z = v[0]; m.poi.R = v[1]; m.poi.Vz = v[2]; m.poi.Vr = v[3];
+        m.poi.Vt = v[4]; m.poi.WQ = v[5]; m.poi.Az = az0; m.poi.Ar = ar0;
+        m.pot.Z = v[6]; m.pot.R = v[7]; m.pot.Vz = v[8]; m.pot.Vr = v[9];
+        m.pot.Vt = v[10]; m.pot.WQ = v[11];
+        m.pot.Ar = ar0 * st.mass[s] / t0.gas_mass;   // Updater, R/src/mc/TypeMCC.f90:90-91
+        m.pot.Az = az0 * st.mass[s] / t0.gas_mass;
+        m.miu = v[12]; m.mass_ratio = v[13]; m.energy = v[14];
+        m.gx = v[15]; m.gy = v[16]; m.gz = v[17]; m.gper = v[18]; m.g = v[19];
+        const unsigned long long w0 = (unsigned long long)__double_as_longlong(v[20]);
+        const unsigned long long w1 = (unsigned long long)__double_as_longlong(v[21]);
+        StateKey sk;
+        sk.c0 = (uint32_t)w0; sk.c1 = (uint32_t)(w0 >> 32); sk.c2 = (uint32_t)w1;
+        Rng rg = make_rng(sk, seed, step, (uint32_t)s | ((uint32_t)g << 8));
+        rg.k = (uint32_t)(w1 >> 32);
+        Part p;
+        mcc_finish(st, ctr, s, g, t0, m, rg, idx, p, &sink);
+        for (int g2 = g + 1; g2 < st.ng; ++g2) {
+            const DevTable &tb = st.tab[s * st.ng + g2];
+            if (tb.model == 0) continue;
+            Rng r2 = make_rng(sk, seed, step, (uint32_t)s | ((uint32_t)g2 << 8));
+            if (!(r2.next() < tb.ratio)) continue;
+            mcc_collide(st, ctr, s, g2, tb, p, r2, &sink);
+        }
+        __builtin_nontemporal_store(p.Vz, &F[2][i]);
+        __builtin_nontemporal_store(p.Vr, &F[3][i]);
+        __builtin_nontemporal_store(p.Vt, &F[4][i]);
+        if (__double_as_longlong(p.Az) != __double_as_longlong(az0) ||
+            __double_as_longlong(p.Ar) != __double_as_longlong(ar0)) {
+            __builtin_nontemporal_store(p.Az, &F[5][i]);
+            __builtin_nontemporal_store(p.Ar, &F[6][i]);
+        }
     }
     __syncthreads();
-    if (last_block) {
-        __threadfence();
-        if (threadIdx.x == 0) {
-            for (int s = 0; s < st.ns; ++s) {
-                unsigned long long add = atomicAdd(&ctr->n_new[s], 0ull);
-                unsigned long long n0 = ctr->n[s];
-                unsigned long long cap = (unsigned long long)st.cap[s];
-                ctr->n[s] = (n0 + add > cap) ? cap : n0 + add;
-                ctr->n_new[s] = 0;
-            }
-            ctr->step = ctr->step + 1;
-            ctr->blocks_done = 0;
+    const unsigned np = min(pcount, (unsigned)kProdStage);
+    double *gs = st.p_stage + (size_t)blockIdx.x * kProdStage * 8;
+    for (unsigned k = threadIdx.x; k < np * 8u; k += kBlock) gs[k] = prec[k];
+    for (unsigned k = threadIdx.x; k < np; k += kBlock) st.p_meta[(size_t)blockIdx.x * kProdStage + k] = pmeta[k];
+    if (threadIdx.x < IPM_NS_MAX) st.p_cnt[(size_t)blockIdx.x * IPM_NS_MAX + threadIdx.x] = pspc[threadIdx.x];
+}
+
+// Block b appends its staged products of species d after n[d], the
+// directly reserved products (n_new[d]) and the staged products of blocks
+// 0..b-1.
+__global__ void __launch_bounds__(kBlock)
+collide_append_kernel(const DevState *__restrict__ stp, DevCounters *ctr)
+{
+    const DevState &st = *stp;
+    __shared__ unsigned long long off[IPM_NS_MAX];
+    __shared__ unsigned long long part[kBlock / 64][IPM_NS_MAX];
+    const int ns = st.ns;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned long long acc[IPM_NS_MAX];
+    for (int d = 0; d < IPM_NS_MAX; ++d) acc[d] = 0;
+    for (unsigned b = threadIdx.x; b < blockIdx.x; b += kBlock)
+        for (int d = 0; d < ns; ++d) acc[d] += st.p_cnt[(size_t)b * IPM_NS_MAX + d];
+    for (int d = 0; d < ns; ++d) {
+        unsigned long long x = acc[d];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o);
+        if (lane == 0) part[w][d] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < (unsigned)ns) {
+        unsigned long long t = 0;
+        for (int k = 0; k < kBlock / 64; ++k) t += part[k][threadIdx.x];
+        off[threadIdx.x] = ctr->n[threadIdx.x] + ctr->n_new[threadIdx.x] + t;
+    }
+    __syncthreads();
+    unsigned np = 0;
+    for (int d = 0; d < ns; ++d) np += st.p_cnt[(size_t)blockIdx.x * IPM_NS_MAX + d];
+    const double *gs = st.p_stage + (size_t)blockIdx.x * kProdStage * 8;
+    for (unsigned k = threadIdx.x; k < np; k += kBlock) {
+        const unsigned meta = st.p_meta[(size_t)blockIdx.x * kProdStage + k];
+        const int d = (int)(meta & 0xffu);
+        const unsigned long long pos = off[d] + (meta >> 8);
+        if (pos >= (unsigned long long)st.cap[d]) {
+            atomicOr(&ctr->overflow, 1);
+            continue;
         }
-        __threadfence();
+        double *const *F = st.f[d];
+#pragma unroll
+        for (int f = 0; f < 8; ++f) F[f][pos] = gs[(size_t)k * 8 + f];
+    }
+}
+
+// Folds the step's products into n[] (capped at the capacity; the append
+// pass flags what did not fit) and advances the random stream's step key.
+__global__ void collide_fold_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
+{
+    const DevState &st = *stp;
+    __shared__ unsigned long long tot[kBlock / 64][IPM_NS_MAX];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int d = 0; d < st.ns; ++d) {
+        unsigned long long x = 0;
+        for (int b = threadIdx.x; b < st.c_grid; b += blockDim.x) x += st.p_cnt[(size_t)b * IPM_NS_MAX + d];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o);
+        if (lane == 0) tot[w][d] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int d = 0; d < st.ns; ++d) {
+            unsigned long long add = ctr->n_new[d];
+            for (int k = 0; k < (int)(blockDim.x / 64); ++k) add += tot[k][d];
+            const unsigned long long n0 = ctr->n[d], cap = (unsigned long long)st.cap[d];
+            ctr->n[d] = (n0 + add > cap) ? cap : n0 + add;
+            ctr->n_new[d] = 0;
+        }
+        ctr->step = ctr->step + 1;
     }
 }
 
@@ -1079,9 +1299,12 @@ static int ensure_queue(ipm_ctx *c)
     if (need > c->q_cap || ntiles > c->q_ntiles) {
         HIPCHK(hipStreamSynchronize(c->stream));
         if (c->qbuf) HIPCHK(hipFree(c->qbuf));
+        if (c->cbuf) HIPCHK(hipFree(c->cbuf));
+        c->cbuf = nullptr;
         if (c->q_tile_count) HIPCHK(hipFree(c->q_tile_count));
         c->qbuf = nullptr; c->q_tile_count = nullptr; c->q_cap = 0; c->q_ntiles = 0;
-        HIPCHK(hipMalloc(&c->qbuf, sizeof(double) * 8 * (size_t)need));
+        HIPCHK(hipMalloc(&c->qbuf, sizeof(double) * kRecD * (size_t)need));
+        HIPCHK(hipMalloc(&c->cbuf, sizeof(double) * kCollFields * (size_t)need));
         HIPCHK(hipMalloc(&c->q_tile_count, sizeof(unsigned) * (size_t)ntiles));
         HIPCHK(hipMemset(c->q_tile_count, 0, sizeof(unsigned) * (size_t)ntiles));
         c->q_cap = need;
@@ -1091,7 +1314,25 @@ static int ensure_queue(ipm_ctx *c)
             c->graph = nullptr;
         }
     }
+    if (!c->c_base) {
+        const size_t G = (size_t)c->num_cu * 6;
+        HIPCHK(hipMalloc(&c->c_base, sizeof(long long) * G));
+        HIPCHK(hipMalloc(&c->c_count, sizeof(unsigned) * G));
+        HIPCHK(hipMalloc(&c->p_stage, sizeof(double) * 8 * kProdStage * G));
+        HIPCHK(hipMalloc(&c->p_meta, sizeof(unsigned) * kProdStage * G));
+        HIPCHK(hipMalloc(&c->p_cnt, sizeof(unsigned) * IPM_NS_MAX * G));
+        HIPCHK(hipMemset(c->c_count, 0, sizeof(unsigned) * G));
+        HIPCHK(hipMemset(c->p_cnt, 0, sizeof(unsigned) * IPM_NS_MAX * G));
+        c->h.c_grid = (int)G;
+    }
     c->h.q_rec = c->qbuf;
+    c->h.c_rec = c->cbuf;
+    c->h.c_cap = c->q_cap;
+    c->h.c_base = c->c_base;
+    c->h.c_count = c->c_count;
+    c->h.p_stage = c->p_stage;
+    c->h.p_meta = c->p_meta;
+    c->h.p_cnt = c->p_cnt;
     c->h.q_tile_count = c->q_tile_count;
     return IPM_OK;
 }
@@ -1123,8 +1364,11 @@ static int launch_particles(ipm_ctx *c, int sel)
 
 static int launch_collide(ipm_ctx *c)
 {
-    hipLaunchKernelGGL(collide_kernel, dim3((unsigned)(c->num_cu * 4)), dim3(kBlock), 0, c->stream, c->d_state,
-                       c->d_ctr);
+    const unsigned G = (unsigned)c->h.c_grid;
+    hipLaunchKernelGGL(collide_kernel, dim3(G), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr);
+    hipLaunchKernelGGL(collide_finish_kernel, dim3(G), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr);
+    hipLaunchKernelGGL(collide_append_kernel, dim3(G), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr);
+    hipLaunchKernelGGL(collide_fold_kernel, dim3(1), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr);
     HIPCHK(hipGetLastError());
     return IPM_OK;
 }
@@ -1220,7 +1464,8 @@ int ipm_destroy(ipm_ctx *c)
     void *ptrs[] = {c->d_state, c->d_ctr, c->raw_all, c->rho_one, c->chi_one, c->rho_tot, c->chi_tot,
                     c->d_chifac, c->inv_vol, c->cell_material, c->hedge, c->vedge, c->mat_type,
                     c->metal_charge, c->sigma, c->Ez, c->Er, c->mcc_counts, c->hist, c->offs,
-                    c->cursor, c->partial, c->partial2, c->d_dst, c->qbuf, c->q_tile_count,
+                    c->cursor, c->partial, c->partial2, c->d_dst, c->qbuf, c->cbuf, c->q_tile_count, c->c_base, c->c_count, c->p_stage,
+                    c->p_meta, c->p_cnt,
                     c->mom, c->phi, c->rezone};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
