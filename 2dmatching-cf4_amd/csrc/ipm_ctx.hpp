@@ -68,6 +68,7 @@ struct ipm_ctx {
     double *p_stage = nullptr;
     unsigned *p_meta = nullptr, *p_cnt = nullptr;
     unsigned *q_tile_count = nullptr;      // q_ntiles
+    int *tanc = nullptr;                   // q_ntiles deposit-window anchors (z, r)
     long long q_cap = 0, q_ntiles = 0;
     double *mom = nullptr;                 // moment sums, 5 * nz * nr (ipm_moments)
     double *phi = nullptr;                 // potential (ipm_set_potential), nz * nr

@@ -65,6 +65,9 @@ struct DevState {
     // 64-byte records (Z R Vz Vr Vt WQ, index in tile, pad), written and read whole.
     double *q_rec;
     unsigned int *q_tile_count;
+    // Per tile (indexed like q_tile_count): the (z, r) corner of its LDS
+    // deposit window, set from the tile's first particle by the previous pass.
+    int *tile_anchor;
     long long q_region[IPM_NS_MAX], q_cbase[IPM_NS_MAX], q_tile_cap[IPM_NS_MAX];
     double q_weight[IPM_NS_MAX];       // expected entries per tile (collision-pass split)
     // Collisions (non-null reactions) found by the collision pass and

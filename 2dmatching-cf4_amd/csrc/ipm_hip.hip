@@ -286,7 +286,6 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
 {
     const DevState &st = *stp;
     __shared__ double win[kWinZ * kWinR];
-    __shared__ int anchor[2];
     __shared__ unsigned long long dead_blk[IPM_NS_MAX];
     // Entries of the tile in the collision queue so far (two counters so
     // the next tile never waits for the reset), and each wave's staging
@@ -339,20 +338,13 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
         const double efactor = st.efactor[s];
         double *__restrict__ raw = st.raw[s];
 
-        if (MODE & M_DEPOSIT) {
-            if (threadIdx.x == 0) {
-                double z0 = F[0][base], r0 = F[1][base];
-                if (is_dead(z0) || !(z0 > 0.0) || !(r0 >= 0.0) || z0 > 1e9 || r0 > 1e9) {
-                    anchor[0] = -1000000; anchor[1] = -1000000;
-                } else {
-                    anchor[0] = (int)ceil(z0) - 4;
-                    anchor[1] = (int)ceil(r0) - 3;
-                }
-            }
-            __syncthreads();
-        }
-        const int zw0 = (MODE & M_DEPOSIT) ? anchor[0] : 0;
-        const int rw0 = (MODE & M_DEPOSIT) ? anchor[1] : 0;
+        // Deposit window of the tile: placed by the tile's first particle as
+        // the previous pass left it (any placement is correct: cells outside
+        // the window go straight to HBM), so no wave waits for a load here.
+        int *const anc = st.tile_anchor + 2 * (st.q_cbase[s] + (t - tbeg));
+        const int zw0 = (MODE & M_DEPOSIT) ? anc[0] : 0;
+        const int rw0 = (MODE & M_DEPOSIT) ? anc[1] : 0;
+        int na0 = zw0, na1 = rw0;   // next pass's window, from slot base after this pass
 
         // Wave w of the block takes kItems consecutive runs of 64 slots, so a
         // wave walks a contiguous, cell-sorted stretch of the tile.
@@ -403,6 +395,11 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
                 if (MODE & M_ABSORB) deleted = absorb_one(st, charge, p);
             }
             const bool alive = live && !deleted;
+            if ((MODE & M_DEPOSIT) && threadIdx.x == 0 && k == 0 && alive && p.Z > 0.0 && p.R >= 0.0 &&
+                p.Z < 1e9 && p.R < 1e9) {
+                na0 = (int)ceil(p.Z) - 4;
+                na1 = (int)ceil(p.R) - 3;
+            }
 
             if (MODE & M_DEPOSIT) {
                 // P2CES, R/src/pic/field/FieldOne.f90:151-166.  Contributions
@@ -514,6 +511,7 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
                 }
             }
             __syncthreads();
+            if (threadIdx.x == 0) { anc[0] = na0; anc[1] = na1; }   // every wave has read anc by now
         }
         if (MODE & M_MCC) {
             // publish the tile's entry count (the deposit flush above already
@@ -1302,10 +1300,14 @@ static int ensure_queue(ipm_ctx *c)
         if (c->cbuf) HIPCHK(hipFree(c->cbuf));
         c->cbuf = nullptr;
         if (c->q_tile_count) HIPCHK(hipFree(c->q_tile_count));
+        if (c->tanc) HIPCHK(hipFree(c->tanc));
+        c->tanc = nullptr;
         c->qbuf = nullptr; c->q_tile_count = nullptr; c->q_cap = 0; c->q_ntiles = 0;
         HIPCHK(hipMalloc(&c->qbuf, sizeof(double) * kRecD * (size_t)need));
         HIPCHK(hipMalloc(&c->cbuf, sizeof(double) * kCollFields * (size_t)need));
         HIPCHK(hipMalloc(&c->q_tile_count, sizeof(unsigned) * (size_t)ntiles));
+        HIPCHK(hipMalloc(&c->tanc, sizeof(int) * 2 * (size_t)ntiles));
+        HIPCHK(hipMemset(c->tanc, 0, sizeof(int) * 2 * (size_t)ntiles));
         HIPCHK(hipMemset(c->q_tile_count, 0, sizeof(unsigned) * (size_t)ntiles));
         c->q_cap = need;
         c->q_ntiles = ntiles;
@@ -1326,6 +1328,7 @@ static int ensure_queue(ipm_ctx *c)
         c->h.c_grid = (int)G;
     }
     c->h.q_rec = c->qbuf;
+    c->h.tile_anchor = c->tanc;
     c->h.c_rec = c->cbuf;
     c->h.c_cap = c->q_cap;
     c->h.c_base = c->c_base;
@@ -1464,7 +1467,7 @@ int ipm_destroy(ipm_ctx *c)
     void *ptrs[] = {c->d_state, c->d_ctr, c->raw_all, c->rho_one, c->chi_one, c->rho_tot, c->chi_tot,
                     c->d_chifac, c->inv_vol, c->cell_material, c->hedge, c->vedge, c->mat_type,
                     c->metal_charge, c->sigma, c->Ez, c->Er, c->mcc_counts, c->hist, c->offs,
-                    c->cursor, c->partial, c->partial2, c->d_dst, c->qbuf, c->cbuf, c->q_tile_count, c->c_base, c->c_count, c->p_stage,
+                    c->cursor, c->partial, c->partial2, c->d_dst, c->qbuf, c->cbuf, c->q_tile_count, c->tanc, c->c_base, c->c_count, c->p_stage,
                     c->p_meta, c->p_cnt,
                     c->mom, c->phi, c->rezone};
     for (void *p : ptrs)
