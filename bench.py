@@ -195,7 +195,7 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int, default=-1, help="-1: autotuned; 0: one block per tile")
     ap.add_argument("--no-autotune", action="store_true", help="keep the config's launch shape and sort interval")
     ap.add_argument("--traffic-probe", action="store_true",
-                    help="for rocprofv3 --pmc runs: a 1 GiB device copy (counter calibration), then 4 steps")
+                    help="for rocprofv3 --pmc runs: a 1 GiB device copy (counter calibration), then 2 warmup and 4 timed steps")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -216,7 +216,7 @@ def main():
         b.copy_(a)                      # reads and writes exactly 2**30 bytes
         torch.cuda.synchronize()
         del a, b
-        args.steps, args.warmup, args.no_cpu_baseline, args.no_extra, args.no_adjoint = 4, 0, True, True, True
+        args.steps, args.warmup, args.no_cpu_baseline, args.no_extra, args.no_adjoint = 4, 2, True, True, True
 
     cfg = W.make_config(args.config)
     sort_interval = cfg.sort_interval if args.sort_interval < 0 else args.sort_interval
