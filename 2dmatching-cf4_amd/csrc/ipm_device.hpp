@@ -56,7 +56,7 @@ struct DevState {
     int count_events;
     int pad;
     // Queue of particles selected for a collision (null-collision pass).
-    // Species s owns q_maxtiles[s] = ceil(cap/1024) tile regions of
+    // Species s owns q_maxtiles[s] = ceil(cap/kTile) tile regions of
     // q_tile_cap[s] entries; entry k of live tile lt of species s sits at
     // q_region[s] + lt*q_tile_cap[s] + k and holds the particle's state and
     // its index inside the tile.  q_tile_count[q_cbase[s] + lt] is that

@@ -23,7 +23,7 @@ using namespace ipm;
 namespace {
 
 constexpr int kBlock = 256;          // threads per block (4 wave64)
-constexpr int kItems = 4;            // particles per thread per tile
+constexpr int kItems = 8;            // particles per thread per tile
 constexpr int kTile = kBlock * kItems;
 constexpr int kWinZ = 64;            // LDS deposit window (nodes in z)
 constexpr int kWinR = 8;             // LDS deposit window (nodes in r)
@@ -347,17 +347,21 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
         int na0 = zw0, na1 = rw0;   // next pass's window, from slot base after this pass
 
         // Wave w of the block takes kItems consecutive runs of 64 slots, so a
-        // wave walks a contiguous, cell-sorted stretch of the tile.
+        // wave walks a contiguous, cell-sorted stretch of the tile.  The
+        // particle streams are read and written once per pass: non-temporal
+        // accesses keep them from evicting the field grids and the deposit
+        // targets from L2.
         const int wv = threadIdx.x >> 6;
         auto load_item = [&](long long i, Part &q) {
-            q.Z = gp(F[0])[i];
+            q.Z = __builtin_nontemporal_load(&gp(F[0])[i]);
             if (MODE & (M_PUSH | M_ABSORB | M_DEPOSIT)) {
-                q.R = gp(F[1])[i];
-                q.WQ = gp(F[7])[i];
+                q.R = __builtin_nontemporal_load(&gp(F[1])[i]);
+                q.WQ = __builtin_nontemporal_load(&gp(F[7])[i]);
             }
             if (MODE & (M_PUSH | M_ABSORB)) {
-                q.Vz = gp(F[2])[i]; q.Vr = gp(F[3])[i]; q.Vt = gp(F[4])[i];
-                q.Az = gp(F[5])[i]; q.Ar = gp(F[6])[i];
+                q.Vz = __builtin_nontemporal_load(&gp(F[2])[i]); q.Vr = __builtin_nontemporal_load(&gp(F[3])[i]);
+                q.Vt = __builtin_nontemporal_load(&gp(F[4])[i]);
+                q.Az = __builtin_nontemporal_load(&gp(F[5])[i]); q.Ar = __builtin_nontemporal_load(&gp(F[6])[i]);
             }
         };
         // Queue records staged by this wave (wave-uniform count).  A flush
@@ -489,8 +493,10 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
                 gp(F[0])[i] = kDead;
             } else if (alive) {
                 if (MODE & M_PUSH) {
-                    gp(F[0])[i] = p.Z; gp(F[1])[i] = p.R; gp(F[2])[i] = p.Vz; gp(F[3])[i] = p.Vr;
-                    gp(F[4])[i] = p.Vt; gp(F[5])[i] = p.Az; gp(F[6])[i] = p.Ar;
+                    __builtin_nontemporal_store(p.Z, &gp(F[0])[i]); __builtin_nontemporal_store(p.R, &gp(F[1])[i]);
+                    __builtin_nontemporal_store(p.Vz, &gp(F[2])[i]); __builtin_nontemporal_store(p.Vr, &gp(F[3])[i]);
+                    __builtin_nontemporal_store(p.Vt, &gp(F[4])[i]); __builtin_nontemporal_store(p.Az, &gp(F[5])[i]);
+                    __builtin_nontemporal_store(p.Ar, &gp(F[6])[i]);
                 } else if (MODE & M_ABSORB) {
                     gp(F[1])[i] = p.R; gp(F[3])[i] = p.Vr; gp(F[6])[i] = p.Ar;
                 }

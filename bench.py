@@ -313,7 +313,7 @@ def main():
             "grid": [cfg.nz, cfg.nr],
             "sort_interval": sort_interval,
             "launch": ("autotuned: " if tuned else "") + (
-                "one block per 1024-slot tile" if (tuned["blocks_per_cu"] if tuned else max(args.blocks_per_cu, 0)) == 0
+                "one block per 2048-slot tile" if (tuned["blocks_per_cu"] if tuned else max(args.blocks_per_cu, 0)) == 0
                 else f"{tuned['blocks_per_cu'] if tuned else args.blocks_per_cu} persistent blocks per CU"),
         },
         "roofline": {

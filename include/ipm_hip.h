@@ -136,7 +136,7 @@ int ipm_take_timing(ipm_ctx *ctx, double *ms, int64_t *launches);
 int ipm_sort(ipm_ctx *ctx, int s);
 
 /* Launch shape of the particle pass: block must be 0 or 256 (threads per
- * block); blocks_per_cu = 0 launches one block per tile of 1024 particles
+ * block); blocks_per_cu = 0 launches one block per tile of 2048 particles
  * (grid sized from the capacities, the default), 1..64 a persistent grid of
  * that many blocks per compute unit looping over the tiles. */
 int ipm_set_launch(ipm_ctx *ctx, int block, int blocks_per_cu);
