@@ -102,6 +102,18 @@ struct DevCounters {
 
 // Global-address-space views of device pointers: loads, stores and atomics
 // through them are global_* instructions instead of flat_*.
+// Element at a 32-bit byte offset from a base pointer: the address is the
+// base (scalar registers) plus one zero-extended vector register, the SADDR
+// form of the global instructions, with no 64-bit address arithmetic per
+// access.  Particle arrays are limited to 2^32 bytes per field for it.
+__device__ __forceinline__ __attribute__((address_space(1))) double *at_off(double *base, unsigned boff)
+{
+    return (__attribute__((address_space(1))) double *)((__attribute__((address_space(1))) char *)base + boff);
+}
+__device__ __forceinline__ const __attribute__((address_space(1))) double *at_off(const double *base, unsigned boff)
+{
+    return (const __attribute__((address_space(1))) double *)((const __attribute__((address_space(1))) char *)base + boff);
+}
 template <class T>
 __device__ __forceinline__ __attribute__((address_space(1))) T *gp(T *p)
 {

@@ -353,15 +353,16 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
         // targets from L2.
         const int wv = threadIdx.x >> 6;
         auto load_item = [&](long long i, Part &q) {
-            q.Z = __builtin_nontemporal_load(&gp(F[0])[i]);
+            const unsigned bo = (unsigned)i << 3;
+            q.Z = __builtin_nontemporal_load(at_off(F[0], bo));
             if (MODE & (M_PUSH | M_ABSORB | M_DEPOSIT)) {
-                q.R = __builtin_nontemporal_load(&gp(F[1])[i]);
-                q.WQ = __builtin_nontemporal_load(&gp(F[7])[i]);
+                q.R = __builtin_nontemporal_load(at_off(F[1], bo));
+                q.WQ = __builtin_nontemporal_load(at_off(F[7], bo));
             }
             if (MODE & (M_PUSH | M_ABSORB)) {
-                q.Vz = __builtin_nontemporal_load(&gp(F[2])[i]); q.Vr = __builtin_nontemporal_load(&gp(F[3])[i]);
-                q.Vt = __builtin_nontemporal_load(&gp(F[4])[i]);
-                q.Az = __builtin_nontemporal_load(&gp(F[5])[i]); q.Ar = __builtin_nontemporal_load(&gp(F[6])[i]);
+                q.Vz = __builtin_nontemporal_load(at_off(F[2], bo)); q.Vr = __builtin_nontemporal_load(at_off(F[3], bo));
+                q.Vt = __builtin_nontemporal_load(at_off(F[4], bo));
+                q.Az = __builtin_nontemporal_load(at_off(F[5], bo)); q.Ar = __builtin_nontemporal_load(at_off(F[6], bo));
             }
         };
         // Queue records staged by this wave (wave-uniform count).  A flush
@@ -493,10 +494,11 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
                 gp(F[0])[i] = kDead;
             } else if (alive) {
                 if (MODE & M_PUSH) {
-                    __builtin_nontemporal_store(p.Z, &gp(F[0])[i]); __builtin_nontemporal_store(p.R, &gp(F[1])[i]);
-                    __builtin_nontemporal_store(p.Vz, &gp(F[2])[i]); __builtin_nontemporal_store(p.Vr, &gp(F[3])[i]);
-                    __builtin_nontemporal_store(p.Vt, &gp(F[4])[i]); __builtin_nontemporal_store(p.Az, &gp(F[5])[i]);
-                    __builtin_nontemporal_store(p.Ar, &gp(F[6])[i]);
+                    const unsigned bo = (unsigned)i << 3;
+                    __builtin_nontemporal_store(p.Z, at_off(F[0], bo)); __builtin_nontemporal_store(p.R, at_off(F[1], bo));
+                    __builtin_nontemporal_store(p.Vz, at_off(F[2], bo)); __builtin_nontemporal_store(p.Vr, at_off(F[3], bo));
+                    __builtin_nontemporal_store(p.Vt, at_off(F[4], bo)); __builtin_nontemporal_store(p.Az, at_off(F[5], bo));
+                    __builtin_nontemporal_store(p.Ar, at_off(F[6], bo));
                 } else if (MODE & M_ABSORB) {
                     gp(F[1])[i] = p.R; gp(F[3])[i] = p.Vr; gp(F[6])[i] = p.Ar;
                 }
@@ -1487,6 +1489,9 @@ int ipm_set_species(ipm_ctx *c, int s, double charge, double mass, int64_t capac
 {
     if (!c || s < 0 || s >= c->ns) return fail(IPM_EINVAL, "invalid species index %d", s);
     if (!(mass > 0) || capacity < 1) return fail(IPM_EINVAL, "invalid mass/capacity for species %d", s);
+    if (capacity > IPM_CAPACITY_MAX)
+        return fail(IPM_EINVAL, "capacity %lld of species %d exceeds IPM_CAPACITY_MAX (%lld): the particle pass "
+                    "addresses each field with 32-bit byte offsets", (long long)capacity, s, (long long)IPM_CAPACITY_MAX);
     HIPCHK(hipStreamSynchronize(c->stream));
     for (int b = 0; b < 2; ++b) {
         if (c->buf[b][s]) HIPCHK(hipFree(c->buf[b][s]));

@@ -37,6 +37,9 @@ extern "C" {
 #define IPM_NS_MAX 8
 #define IPM_NG_MAX 3
 #define IPM_NR_MAX 128     /* reactions per (species, gas) table */
+/* Largest capacity of one species (2^29 - 32 particles, 4 GiB per field):
+ * the particle pass addresses each field with 32-bit byte offsets. */
+#define IPM_CAPACITY_MAX 536870880LL
 
 typedef struct ipm_ctx ipm_ctx;
 
@@ -47,7 +50,8 @@ int ipm_destroy(ipm_ctx *ctx);
 const char *ipm_last_error(void);
 
 /* Species parameters: ParticleBundle%Charge/%Mass (ParticleBundle.f90:350-351)
- * and the device capacity (npar_max, ParticleBundle.f90:354-359). */
+ * and the device capacity (npar_max, ParticleBundle.f90:354-359), at most
+ * IPM_CAPACITY_MAX. */
 int ipm_set_species(ipm_ctx *ctx, int s, double charge, double mass, int64_t capacity);
 
 /* Geometry for absorption and deposit (R/src/base/Geometry.F90:21-45,

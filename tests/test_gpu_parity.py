@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from helpers import (AMU, ME, QE, assert_close, random_fields, random_particles, small_geometry, sort_set)
-from ipm2d import HotPath, CapacityError
+from ipm2d import HotPath, CapacityError, IPMError
 from ipm2d import workload as W
 from ipm2d import xsec
 
@@ -311,6 +311,14 @@ def test_graph_replay_equals_eager(rng):
             assert a.count(s) == b.count(s)
     for s in range(cfg.ns):
         assert_close(sort_set(a.download(s)), sort_set(b.download(s)), rtol=1e-12, atol=1e-300, what=f"sp {s}")
+
+
+def test_capacity_above_the_addressing_limit_rejected():
+    geom = small_geometry()
+    hp = HotPath(geom.nz, geom.nr, geom.dx, DT, 1, 1)
+    with pytest.raises(IPMError):
+        hp.set_species(0, -QE, ME, (1 << 29))
+    hp.close()
 
 
 def test_capacity_overflow_raises(rng):
