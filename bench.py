@@ -186,6 +186,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default="c2")
+    ap.add_argument("--share", type=int, default=1,
+                    help="K > 1: one GPU's share of a K-GPU run (every species' per-cell population / K, full grid)")
     ap.add_argument("--sort-interval", type=int, default=-1)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-workers", type=int, default=0, help="0: min(16, cores)")
@@ -219,6 +221,9 @@ def main():
         args.steps, args.warmup, args.no_cpu_baseline, args.no_extra, args.no_adjoint = 4, 2, True, True, True
 
     cfg = W.make_config(args.config)
+    if args.share > 1:
+        import dataclasses
+        cfg.species = [dataclasses.replace(sp, ppc=sp.ppc / args.share) for sp in cfg.species]
     sort_interval = cfg.sort_interval if args.sort_interval < 0 else args.sort_interval
     t = time.perf_counter()
     geom = W.make_geometry(cfg)
@@ -268,7 +273,8 @@ def main():
     bytes_per_launch = 120.0 * (psteps / args.steps) + grid_bytes
     achieved = bytes_per_launch / avg_s / 1e9
     traffic = None
-    if os.path.exists(PROFILE_TRAFFIC):
+    # the committed PMC figure belongs to the default workload (c2, whole)
+    if os.path.exists(PROFILE_TRAFFIC) and args.config == "c2" and args.share == 1:
         try:
             with open(PROFILE_TRAFFIC) as f:
                 traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -305,7 +311,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"{cfg.name}: {cfg.nz}x{cfg.nr} nodes r-z, {ntot} particles "
+            "workload": f"{cfg.name}{f' (one GPU share, 1/{args.share})' if args.share > 1 else ''}: "
+                        f"{cfg.nz}x{cfg.nr} nodes r-z, {ntot} particles "
                         f"({'/'.join(sp.name for sp in cfg.species)}), CF4 {cfg.pressure_mtorr} mTorr, "
                         f"dt {cfg.dt:g} s, cell sort every {sort_interval} steps",
             "particles": ntot,
