@@ -47,6 +47,7 @@ class HotPath:
         self._h = h
         self.nmetal = 0
         self.capacity = [0] * ns
+        self.species_params = [None] * ns     # (charge, mass) per species
 
     # ------------------------------------------------------------ lifecycle
     def close(self):
@@ -70,6 +71,7 @@ class HotPath:
     def set_species(self, s, charge, mass, capacity):
         N.call("ipm_set_species", self._h, s, float(charge), float(mass), int(capacity))
         self.capacity[s] = int(capacity)
+        self.species_params[s] = (float(charge), float(mass))
 
     def set_geometry(self, geom):
         inv = fortran(geom.inv_volume).astype(np.float64)
@@ -226,6 +228,35 @@ class HotPath:
         N.call("ipm_recombine", self._h, rl.shape[0], _i(flat), _d(rt), float(time_step),
                out.ctypes.data_as(C.POINTER(C.c_int64)))
         return out
+
+    def load_restart(self, s, path):
+        """PB(s)%Load (ParticleBundle.f90:86-113) from a text (.dat) or HDF5
+        (.h5) bundle dump: positions and velocities are brought to grid
+        units if the dump is not scaled, the NPar live particles replace
+        species s.  Returns the dump (header fields and particles)."""
+        from . import restart
+        if str(path).endswith(".h5"):
+            from . import h5lite
+            d = h5lite.read_bundle(path)
+        else:
+            d = restart.read_dat(path)
+        d = d.to_grid_units()
+        self.upload(s, d.particles())
+        return d
+
+    def dump_restart(self, s, path, weight):
+        """PB(s)%Dump (ParticleBundle.f90:117-174) of the live particles of
+        species s, in grid units, with PB%Weight = weight; .h5 writes HDF5,
+        anything else the text form."""
+        from . import restart
+        charge, mass = self.species_params[s]
+        d = restart.dump_from_particles(self.download(s), charge, mass, weight, self.dx, self.dt)
+        if str(path).endswith(".h5"):
+            from . import h5lite
+            h5lite.write_bundle(path, d)
+        else:
+            restart.write_dat(path, d)
+        return d
 
     def mcc(self):
         N.call("ipm_mcc", self._h)
