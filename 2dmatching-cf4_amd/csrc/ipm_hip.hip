@@ -406,6 +406,24 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
                 na1 = (int)ceil(p.R) - 3;
             }
 
+            // Null-collision selection (SelectParticleVW, R/src/mc/MCCPublic.F90:123-155):
+            // draw 0 of the (slot, step, species|gas<<8) stream against
+            // CollisionRatio.  Selected particles go to the collision queue
+            // with their post-push state; collide_kernel finishes them.
+            bool selected = false;
+            if ((MODE & M_MCC) && alive) {
+                if (!(MODE & (M_PUSH | M_ABSORB))) {
+                    if (!(MODE & M_DEPOSIT)) p.R = gp(F[1])[i];
+                    p.Vz = gp(F[2])[i]; p.Vr = gp(F[3])[i]; p.Vt = gp(F[4])[i];
+                }
+                const StateKey sk = state_key(p.Z, p.R, p.Vz, p.Vr, p.Vt);
+                for (int g = 0; g < st.ng; ++g) {
+                    const DevTable &tb = st.tab[s * st.ng + g];
+                    if (tb.model == 0) continue;
+                    Rng rg = make_rng(sk, seed, step, (uint32_t)s | ((uint32_t)g << 8));
+                    if (rg.next() < tb.ratio) selected = true;
+                }
+            }
             if (MODE & M_DEPOSIT) {
                 // P2CES, R/src/pic/field/FieldOne.f90:151-166.  Contributions
                 // of a run of particles in one cell are summed across the wave
@@ -448,24 +466,6 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
                 }
             }
 
-            // Null-collision selection (SelectParticleVW, R/src/mc/MCCPublic.F90:123-155):
-            // draw 0 of the (slot, step, species|gas<<8) stream against
-            // CollisionRatio.  Selected particles go to the collision queue
-            // with their post-push state; collide_kernel finishes them.
-            bool selected = false;
-            if ((MODE & M_MCC) && alive) {
-                if (!(MODE & (M_PUSH | M_ABSORB))) {
-                    if (!(MODE & M_DEPOSIT)) p.R = gp(F[1])[i];
-                    p.Vz = gp(F[2])[i]; p.Vr = gp(F[3])[i]; p.Vt = gp(F[4])[i];
-                }
-                const StateKey sk = state_key(p.Z, p.R, p.Vz, p.Vr, p.Vt);
-                for (int g = 0; g < st.ng; ++g) {
-                    const DevTable &tb = st.tab[s * st.ng + g];
-                    if (tb.model == 0) continue;
-                    Rng rg = make_rng(sk, seed, step, (uint32_t)s | ((uint32_t)g << 8));
-                    if (rg.next() < tb.ratio) selected = true;
-                }
-            }
             if (MODE & M_MCC) {
                 // stage the wave's selections as queue records in LDS
                 const unsigned long long m = __ballot(selected);
@@ -493,14 +493,31 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
             if (deleted) {
                 gp(F[0])[i] = kDead;
             } else if (alive) {
+                // With one gas, a selected particle's VelRes(VFactor) and
+                // VelRes(1/VFactor) (R/src/mc/MCCPublic.F90:62, 91) are all a
+                // null event leaves, so they are applied here, in the
+                // streaming store, and the collision pass writes back only
+                // real reactions (the queue record keeps the unrounded state).
+                const bool rt = (MODE & M_MCC) && selected && st.ng == 1;
+                double vz = p.Vz, vr = p.Vr, vt = p.Vt;
+                if (rt) {
+                    vz = vz * st.vfactor * st.inv_vfactor;
+                    vr = vr * st.vfactor * st.inv_vfactor;
+                    vt = vt * st.vfactor * st.inv_vfactor;
+                }
                 if (MODE & M_PUSH) {
                     const unsigned bo = (unsigned)i << 3;
                     __builtin_nontemporal_store(p.Z, at_off(F[0], bo)); __builtin_nontemporal_store(p.R, at_off(F[1], bo));
-                    __builtin_nontemporal_store(p.Vz, at_off(F[2], bo)); __builtin_nontemporal_store(p.Vr, at_off(F[3], bo));
-                    __builtin_nontemporal_store(p.Vt, at_off(F[4], bo)); __builtin_nontemporal_store(p.Az, at_off(F[5], bo));
+                    __builtin_nontemporal_store(vz, at_off(F[2], bo)); __builtin_nontemporal_store(vr, at_off(F[3], bo));
+                    __builtin_nontemporal_store(vt, at_off(F[4], bo)); __builtin_nontemporal_store(p.Az, at_off(F[5], bo));
                     __builtin_nontemporal_store(p.Ar, at_off(F[6], bo));
-                } else if (MODE & M_ABSORB) {
-                    gp(F[1])[i] = p.R; gp(F[3])[i] = p.Vr; gp(F[6])[i] = p.Ar;
+                } else {
+                    if (MODE & M_ABSORB) {
+                        gp(F[1])[i] = p.R; gp(F[3])[i] = vr; gp(F[6])[i] = p.Ar;
+                    }
+                    if ((MODE & M_MCC) && rt) {
+                        gp(F[2])[i] = vz; gp(F[3])[i] = vr; gp(F[4])[i] = vt;
+                    }
                 }
             }
         }
@@ -719,10 +736,11 @@ collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
                     break;
                 }
                 // write back only what the collisions changed (null events
-                // change nothing but VelRes rounding; Az/Ar never): each
-                // field is a separate cache line of a separate array, stored
-                // non-temporally so the scattered lines do not crowd L2
-                if (!handed && (__double_as_longlong(p.Vz) != __double_as_longlong(p0.Vz) ||
+                // change nothing but VelRes rounding, which the particle
+                // pass already stored when there is one gas; Az/Ar never):
+                // each field is a separate cache line of a separate array,
+                // stored non-temporally so the scattered lines do not crowd L2
+                if (!handed && st.ng != 1 && (__double_as_longlong(p.Vz) != __double_as_longlong(p0.Vz) ||
                                 __double_as_longlong(p.Vr) != __double_as_longlong(p0.Vr) ||
                                 __double_as_longlong(p.Vt) != __double_as_longlong(p0.Vt))) {
                     __builtin_nontemporal_store(p.Vz, &F[2][i]);
