@@ -63,9 +63,7 @@ struct ipm_ctx {
     double **d_dst = nullptr;
     double *qbuf = nullptr;                // q_cap records of 8 doubles
     double *cbuf = nullptr;                // collision records, kCollFields arrays of q_cap doubles
-    long long *c_base = nullptr;           // per collision block: record start, count,
-    unsigned *c_count = nullptr;           //   staged products and their meta / counts
-    double *p_stage = nullptr;
+    double *p_stage = nullptr;             // per collision block: staged products, meta, counts
     unsigned *p_meta = nullptr, *p_cnt = nullptr;
     unsigned *q_tile_count = nullptr;      // q_ntiles
     int *tanc = nullptr;                   // q_ntiles deposit-window anchors (z, r)

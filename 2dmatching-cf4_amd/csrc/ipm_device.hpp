@@ -70,16 +70,15 @@ struct DevState {
     int *tile_anchor;
     long long q_region[IPM_NS_MAX], q_cbase[IPM_NS_MAX], q_tile_cap[IPM_NS_MAX];
     double q_weight[IPM_NS_MAX];       // expected entries per tile (collision-pass split)
-    // Collisions (non-null reactions) found by the collision pass and
-    // finished by the finishing pass: kCollFields arrays of c_cap doubles
-    // (structure of arrays), one record per collision.
+    // Collisions (non-null reactions) a collision block found, kept until
+    // the same block finishes them: kCollFields arrays of c_cap doubles
+    // (structure of arrays), one record per collision, in the region of the
+    // block's queue tiles.
     double *c_rec;
     long long c_cap;
-    // Per collision block (c_grid blocks): its records' start and count,
-    // and the products it staged (kProdStage records of 8 doubles, each
-    // with species | rank << 8, and per-species counts) for the append pass.
-    long long *c_base;
-    unsigned *c_count;
+    // Per collision block (c_grid blocks): the products it staged
+    // (kProdStage records of 8 doubles, each with species | rank << 8, and
+    // per-species counts) for the append pass.
     double *p_stage;
     unsigned *p_meta;
     unsigned *p_cnt;

@@ -563,31 +563,31 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
 }
 
 // ------------------------------------------------------- collision pass
-// Four kernels finish the queued particles, with no atomic that many
+// Three kernels finish the queued particles, with no atomic that many
 // threads contend for (device-wide atomics on one address serialize, and
 // the reference's append of products, MCCAddNewParticle,
 // R/src/mc/MCCPublic.F90:209-226, would otherwise put one on every product).
-//   collide_kernel: for each gas in order, the selection draw is repeated
-//     and, if it hits, VelRes(VFactor), Updater and the reaction choice of
-//     MCCPublic.F90:61-97 follow.  A null event ends there (VelRes back,
-//     results to the particle's slot); a real reaction is written to a
-//     collision record in the block's own region (the region of its queue
-//     tiles) and the rest of the particle's work moves on.
-//   collide_finish_kernel: block b runs the records of collide_kernel's
-//     block b with full wavefronts (the reaction, VelRes back, the gases
-//     after it, the write-back) and stages the products in LDS, then in its
-//     own slice of a global stage.
+//   collide_kernel, first part: for each gas in order, the selection draw is
+//     repeated and, if it hits, VelRes(VFactor), Updater and the reaction
+//     choice of MCCPublic.F90:61-97 follow.  A null event ends there (its
+//     VelRes round trip is already in the particle's slot with one gas); a
+//     real reaction is written to a collision record in the block's own
+//     region (the region of its queue tiles).
+//   collide_kernel, second part: the block runs its records with full
+//     wavefronts (the reaction, VelRes back, the gases after it, the
+//     write-back) and stages the products in LDS, then in its own slice of
+//     a global stage.
 //   collide_append_kernel: block b appends its staged products after the
 //     products of blocks 0..b-1 of the same species.
 //   collide_fold_kernel: folds the step's products into n[] and advances the
 //     random stream's step key.
-// Most selections are null events, so the split also keeps the reaction
-// code (scattering angles, products) off the wavefronts that would mostly
-// idle in it.  collide_kernel splits its grid over the species in
-// proportion to their expected work (every species with live tiles gets a
-// block); a block takes a contiguous range of its species' tiles, scans
-// their entry counts in LDS and processes the entries flat.  All four use
-// fixed grids, so they can be captured.
+// Most selections are null events, so the two parts keep the reaction code
+// (scattering angles, products) off the wavefronts that would mostly idle
+// in it.  collide_kernel splits its grid over the species in proportion to
+// their expected work (every species with live tiles gets a block); a block
+// takes a contiguous range of its species' tiles, scans their entry counts
+// in LDS and processes the entries flat.  All three use fixed grids, so
+// they can be captured.
 constexpr int kChunk = 1024;   // tile counts scanned per LDS pass
 constexpr int kCollFields = 24;
 // Record fields: 0-5 poi, 6-11 pot (Z R Vz Vr Vt WQ), 12 miu, 13 mass_ratio,
@@ -613,16 +613,83 @@ __device__ __forceinline__ void put_coll(const DevState &st, unsigned long long 
     for (int f = 0; f < kCollFields; ++f) gp(c)[(size_t)f * C] = v[f];
 }
 
-__global__ void __launch_bounds__(kBlock, 3)
+// The records [rbase, rbase + nrec) of one collide_kernel block, one per
+// thread: the reaction, VelRes back, the gases after it, the write-back.
+// Products are staged in the block's LDS sink and then written, with their
+// per-species counts, to block b's slice of the product stage.
+__device__ __forceinline__ void finish_records(const DevState &st, DevCounters *ctr, long long rbase, unsigned nrec,
+                                               double *prec, unsigned *pmeta, unsigned *pcount, unsigned *pspc)
+{
+    const ProdSink sink = {prec, pmeta, pcount, pspc};
+    const uint32_t step = ctr->step;
+    const unsigned long long seed = ctr->seed;
+    const long long C = st.c_cap;
+    for (unsigned e = threadIdx.x; e < nrec; e += kBlock) {
+        const double *c = st.c_rec + rbase + e;
+        double v[kCollFields];
+#pragma unroll
+        for (int f = 0; f < kCollFields; ++f) v[f] = gp(c)[(size_t)f * C];
+        const long long i = __double_as_longlong(v[22]);
+        const unsigned tag = (unsigned)__double_as_longlong(v[23]);
+        const int s = (int)(tag & 0xffu), g = (int)((tag >> 8) & 0xffu), idx = (int)(tag >> 16);
+        double *const *F = st.f[s];
+        const double az0 = gp(F[5])[i], ar0 = gp(F[6])[i];
+        const DevTable &t0 = st.tab[s * st.ng + g];
+        Mc m;
+        m.poi.Z = v[0]; m.poi.R = v[1]; m.poi.Vz = v[2]; m.poi.Vr = v[3];
+        m.poi.Vt = v[4]; m.poi.WQ = v[5]; m.poi.Az = az0; m.poi.Ar = ar0;
+        m.pot.Z = v[6]; m.pot.R = v[7]; m.pot.Vz = v[8]; m.pot.Vr = v[9];
+        m.pot.Vt = v[10]; m.pot.WQ = v[11];
+        m.pot.Ar = ar0 * st.mass[s] / t0.gas_mass;   // Updater, R/src/mc/TypeMCC.f90:90-91
+        m.pot.Az = az0 * st.mass[s] / t0.gas_mass;
+        m.miu = v[12]; m.mass_ratio = v[13]; m.energy = v[14];
+        m.gx = v[15]; m.gy = v[16]; m.gz = v[17]; m.gper = v[18]; m.g = v[19];
+        const unsigned long long w0 = (unsigned long long)__double_as_longlong(v[20]);
+        const unsigned long long w1 = (unsigned long long)__double_as_longlong(v[21]);
+        StateKey sk;
+        sk.c0 = (uint32_t)w0; sk.c1 = (uint32_t)(w0 >> 32); sk.c2 = (uint32_t)w1;
+        Rng rg = make_rng(sk, seed, step, (uint32_t)s | ((uint32_t)g << 8));
+        rg.k = (uint32_t)(w1 >> 32);
+        Part p;
+        mcc_finish(st, ctr, s, g, t0, m, rg, idx, p, &sink);
+        for (int g2 = g + 1; g2 < st.ng; ++g2) {
+            const DevTable &tb = st.tab[s * st.ng + g2];
+            if (tb.model == 0) continue;
+            Rng r2 = make_rng(sk, seed, step, (uint32_t)s | ((uint32_t)g2 << 8));
+            if (!(r2.next() < tb.ratio)) continue;
+            mcc_collide(st, ctr, s, g2, tb, p, r2, &sink);
+        }
+        __builtin_nontemporal_store(p.Vz, &F[2][i]);
+        __builtin_nontemporal_store(p.Vr, &F[3][i]);
+        __builtin_nontemporal_store(p.Vt, &F[4][i]);
+        if (__double_as_longlong(p.Az) != __double_as_longlong(az0) ||
+            __double_as_longlong(p.Ar) != __double_as_longlong(ar0)) {
+            __builtin_nontemporal_store(p.Az, &F[5][i]);
+            __builtin_nontemporal_store(p.Ar, &F[6][i]);
+        }
+    }
+    __syncthreads();
+    const unsigned np = min(*pcount, (unsigned)kProdStage);
+    double *gs = st.p_stage + (size_t)blockIdx.x * kProdStage * 8;
+    for (unsigned k = threadIdx.x; k < np * 8u; k += kBlock) gs[k] = prec[k];
+    for (unsigned k = threadIdx.x; k < np; k += kBlock) st.p_meta[(size_t)blockIdx.x * kProdStage + k] = pmeta[k];
+    if (threadIdx.x < IPM_NS_MAX) st.p_cnt[(size_t)blockIdx.x * IPM_NS_MAX + threadIdx.x] = pspc[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(kBlock, 2)
 collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
 {
     const DevState &st = *stp;
     __shared__ unsigned cpre[kChunk + 1];
     __shared__ unsigned wtot[kBlock / 64];
     __shared__ unsigned nrec;
+    __shared__ double prec[kProdStage * 8];
+    __shared__ unsigned pmeta[kProdStage];
+    __shared__ unsigned pcount, pspc[IPM_NS_MAX];
     const int ns = st.ns;
     const unsigned G = gridDim.x;
-    if (threadIdx.x == 0) nrec = 0;
+    if (threadIdx.x == 0) { nrec = 0; pcount = 0; }
+    if (threadIdx.x < IPM_NS_MAX) pspc[threadIdx.x] = 0;
 
     // species of this block and its tile range
     int sb = -1;
@@ -751,81 +818,9 @@ collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
             __syncthreads();
         }
     }
+    // this block's real reactions, with the block's wavefronts full
     __syncthreads();
-    if (threadIdx.x == 0) {
-        st.c_base[blockIdx.x] = rbase;
-        st.c_count[blockIdx.x] = nrec;
-    }
-}
-
-// The records of collide_kernel's block b, one per thread; products are
-// staged in LDS and written to block b's slice of the product stage.
-__global__ void __launch_bounds__(kBlock, 2)
-collide_finish_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
-{
-    const DevState &st = *stp;
-    __shared__ double prec[kProdStage * 8];
-    __shared__ unsigned pmeta[kProdStage];
-    __shared__ unsigned pcount, pspc[IPM_NS_MAX];
-    if (threadIdx.x == 0) pcount = 0;
-    if (threadIdx.x < IPM_NS_MAX) pspc[threadIdx.x] = 0;
-    __syncthreads();
-    const ProdSink sink = {prec, pmeta, &pcount, pspc};
-    const long long rbase = st.c_base[blockIdx.x];
-    const unsigned nrec = st.c_count[blockIdx.x];
-    const uint32_t step = ctr->step;
-    const unsigned long long seed = ctr->seed;
-    const long long C = st.c_cap;
-    for (unsigned e = threadIdx.x; e < nrec; e += kBlock) {
-        const double *c = st.c_rec + rbase + e;
-        double v[kCollFields];
-#pragma unroll
-        for (int f = 0; f < kCollFields; ++f) v[f] = gp(c)[(size_t)f * C];
-        const long long i = __double_as_longlong(v[22]);
-        const unsigned tag = (unsigned)__double_as_longlong(v[23]);
-        const int s = (int)(tag & 0xffu), g = (int)((tag >> 8) & 0xffu), idx = (int)(tag >> 16);
-        double *const *F = st.f[s];
-        const double az0 = gp(F[5])[i], ar0 = gp(F[6])[i];
-        const DevTable &t0 = st.tab[s * st.ng + g];
-        Mc m;
-        m.poi.Z = v[0]; m.poi.R = v[1]; m.poi.Vz = v[2]; m.poi.Vr = v[3];
-        m.poi.Vt = v[4]; m.poi.WQ = v[5]; m.poi.Az = az0; m.poi.Ar = ar0;
-        m.pot.Z = v[6]; m.pot.R = v[7]; m.pot.Vz = v[8]; m.pot.Vr = v[9];
-        m.pot.Vt = v[10]; m.pot.WQ = v[11];
-        m.pot.Ar = ar0 * st.mass[s] / t0.gas_mass;   // Updater, R/src/mc/TypeMCC.f90:90-91
-        m.pot.Az = az0 * st.mass[s] / t0.gas_mass;
-        m.miu = v[12]; m.mass_ratio = v[13]; m.energy = v[14];
-        m.gx = v[15]; m.gy = v[16]; m.gz = v[17]; m.gper = v[18]; m.g = v[19];
-        const unsigned long long w0 = (unsigned long long)__double_as_longlong(v[20]);
-        const unsigned long long w1 = (unsigned long long)__double_as_longlong(v[21]);
-        StateKey sk;
-        sk.c0 = (uint32_t)w0; sk.c1 = (uint32_t)(w0 >> 32); sk.c2 = (uint32_t)w1;
-        Rng rg = make_rng(sk, seed, step, (uint32_t)s | ((uint32_t)g << 8));
-        rg.k = (uint32_t)(w1 >> 32);
-        Part p;
-        mcc_finish(st, ctr, s, g, t0, m, rg, idx, p, &sink);
-        for (int g2 = g + 1; g2 < st.ng; ++g2) {
-            const DevTable &tb = st.tab[s * st.ng + g2];
-            if (tb.model == 0) continue;
-            Rng r2 = make_rng(sk, seed, step, (uint32_t)s | ((uint32_t)g2 << 8));
-            if (!(r2.next() < tb.ratio)) continue;
-            mcc_collide(st, ctr, s, g2, tb, p, r2, &sink);
-        }
-        __builtin_nontemporal_store(p.Vz, &F[2][i]);
-        __builtin_nontemporal_store(p.Vr, &F[3][i]);
-        __builtin_nontemporal_store(p.Vt, &F[4][i]);
-        if (__double_as_longlong(p.Az) != __double_as_longlong(az0) ||
-            __double_as_longlong(p.Ar) != __double_as_longlong(ar0)) {
-            __builtin_nontemporal_store(p.Az, &F[5][i]);
-            __builtin_nontemporal_store(p.Ar, &F[6][i]);
-        }
-    }
-    __syncthreads();
-    const unsigned np = min(pcount, (unsigned)kProdStage);
-    double *gs = st.p_stage + (size_t)blockIdx.x * kProdStage * 8;
-    for (unsigned k = threadIdx.x; k < np * 8u; k += kBlock) gs[k] = prec[k];
-    for (unsigned k = threadIdx.x; k < np; k += kBlock) st.p_meta[(size_t)blockIdx.x * kProdStage + k] = pmeta[k];
-    if (threadIdx.x < IPM_NS_MAX) st.p_cnt[(size_t)blockIdx.x * IPM_NS_MAX + threadIdx.x] = pspc[threadIdx.x];
+    finish_records(st, ctr, rbase, nrec, prec, pmeta, &pcount, pspc);
 }
 
 // Block b appends its staged products of species d after n[d], the
@@ -1342,14 +1337,11 @@ static int ensure_queue(ipm_ctx *c)
             c->graph = nullptr;
         }
     }
-    if (!c->c_base) {
+    if (!c->p_stage) {
         const size_t G = (size_t)c->num_cu * 6;
-        HIPCHK(hipMalloc(&c->c_base, sizeof(long long) * G));
-        HIPCHK(hipMalloc(&c->c_count, sizeof(unsigned) * G));
         HIPCHK(hipMalloc(&c->p_stage, sizeof(double) * 8 * kProdStage * G));
         HIPCHK(hipMalloc(&c->p_meta, sizeof(unsigned) * kProdStage * G));
         HIPCHK(hipMalloc(&c->p_cnt, sizeof(unsigned) * IPM_NS_MAX * G));
-        HIPCHK(hipMemset(c->c_count, 0, sizeof(unsigned) * G));
         HIPCHK(hipMemset(c->p_cnt, 0, sizeof(unsigned) * IPM_NS_MAX * G));
         c->h.c_grid = (int)G;
     }
@@ -1357,8 +1349,6 @@ static int ensure_queue(ipm_ctx *c)
     c->h.tile_anchor = c->tanc;
     c->h.c_rec = c->cbuf;
     c->h.c_cap = c->q_cap;
-    c->h.c_base = c->c_base;
-    c->h.c_count = c->c_count;
     c->h.p_stage = c->p_stage;
     c->h.p_meta = c->p_meta;
     c->h.p_cnt = c->p_cnt;
@@ -1395,7 +1385,6 @@ static int launch_collide(ipm_ctx *c)
 {
     const unsigned G = (unsigned)c->h.c_grid;
     hipLaunchKernelGGL(collide_kernel, dim3(G), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr);
-    hipLaunchKernelGGL(collide_finish_kernel, dim3(G), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr);
     hipLaunchKernelGGL(collide_append_kernel, dim3(G), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr);
     hipLaunchKernelGGL(collide_fold_kernel, dim3(1), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr);
     HIPCHK(hipGetLastError());
@@ -1493,7 +1482,7 @@ int ipm_destroy(ipm_ctx *c)
     void *ptrs[] = {c->d_state, c->d_ctr, c->raw_all, c->rho_one, c->chi_one, c->rho_tot, c->chi_tot,
                     c->d_chifac, c->inv_vol, c->cell_material, c->hedge, c->vedge, c->mat_type,
                     c->metal_charge, c->sigma, c->Ez, c->Er, c->mcc_counts, c->hist, c->offs,
-                    c->cursor, c->partial, c->partial2, c->d_dst, c->qbuf, c->cbuf, c->q_tile_count, c->tanc, c->c_base, c->c_count, c->p_stage,
+                    c->cursor, c->partial, c->partial2, c->d_dst, c->qbuf, c->cbuf, c->q_tile_count, c->tanc, c->p_stage,
                     c->p_meta, c->p_cnt,
                     c->mom, c->phi, c->rezone};
     for (void *p : ptrs)
