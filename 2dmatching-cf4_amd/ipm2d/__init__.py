@@ -5,4 +5,4 @@ particle interface.  The compute runs in the HIP library built from
 from . import _native, geometry, xsec, workload  # noqa: F401
 from .hotpath import HotPath, from_config  # noqa: F401
 from ._native import IPMError, CapacityError  # noqa: F401
-from . import circuit, recombination, rezoning  # noqa: F401
+from . import circuit, recombination, refconfig, rezoning  # noqa: F401

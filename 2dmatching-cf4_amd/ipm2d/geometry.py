@@ -123,14 +123,23 @@ def build_geometry(nz, nr, dx, cell_material, nmetal, ndiel, permittivity=None):
     if permittivity is not None:
         for m, e in permittivity.items():
             eps[cm == m] = e
-    pt = np.zeros((nz, nr), dtype=np.int32)
-    metal = ctype == MAT_METAL
-    for a, b in ((0, 0), (1, 0), (0, 1), (1, 1)):
-        sl = pt[a:a + nz - 1, b:b + nr - 1]
-        sl[metal] = cm[metal]
+    pt = metal_point_types(np.zeros((nz, nr), dtype=np.int32), cm, ctype == MAT_METAL)
     return Geometry(nz=nz, nr=nr, dx=dx, cell_material=cm, material_type=mt, ndiel=ndiel, nmetal=nmetal,
                     hedge_type=he, vedge_type=ve, volume_factor=vf, volume=vol, inv_volume=inv,
                     cell_load=vac.astype(np.int32), cell_epsilon=eps, point_type=pt)
+
+
+def metal_point_types(point_type, cell_material, metal):
+    """Each metal cell marks its four nodes with its material index; cells
+    are visited i-outer, j-inner and a later cell overwrites an earlier one
+    (R/src/base/init.F90:388-398), so a node shared by two metals takes the
+    index of the cell that comes last in that order."""
+    pt = np.array(point_type, dtype=np.int32, copy=True)
+    nzc, nrc = cell_material.shape
+    for a, b in ((1, 1), (1, 0), (0, 1), (0, 0)):    # lowest priority first
+        sl = pt[a:a + nzc, b:b + nrc]
+        sl[metal] = cell_material[metal]
+    return pt
 
 
 def ccp_geometry(nz, nr, dx, electrode_layers=1, dielectric_ring=0):

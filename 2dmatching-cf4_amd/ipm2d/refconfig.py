@@ -1,0 +1,512 @@
+"""The reference's run configuration -- config.json and the geometry and
+materials inputs it names -- read into this package's host-side structures.
+
+The reference loads ./config.json with json-fortran (R/src/onestep.F90:65-70)
+and reads it in the Init*Json routines of R/src/base/init.F90.  This module
+reads the same keys under the same rules:
+
+* every key the reference reads must be there, or the load fails
+  (CHECK_FOUND / FoundError, init.F90:24 and 880-890);
+* a key is read only where the reference reads it: grid.specified_deps_in_*
+  only with deps_type 1 (init.F90:172-193), gas.recb.* only with
+  is_open_recombination (init.F90:832-878), a circuit's parameters only for
+  its type, frequency lists only with n_freq > 0, pulse segments only with
+  seg > 1, and only the first circuits_number entries of circuits_list
+  (init.F90:574-800);
+* paths are json-fortran's: dots between keys, 1-based [i] into arrays,
+  case-sensitive keys, the first of duplicate keys;
+* values convert between JSON types as json-fortran does without strict type
+  checking (R/lib/json/json_value_module.F90: integers 8134-8179, reals
+  8346-8400, logicals 8838-8874; a string becomes an integer through an Iw
+  read, R/lib/json/json_string_utilities.F90:116-142): a real truncates to an
+  integer, a logical is 1 or 0, a number is true when > 0, a string is true
+  only as "true", null is NaN as a real and missing otherwise; a failed
+  conversion counts as a missing key.
+
+Derived quantities follow the reference:
+
+* Nz, Nr are the geometry file's attributes; dz = dr = z_length / (Nz - 1)
+  and r_length = dz (Nr - 1) (init.F90:105-135);
+* with diag_freq > 0 the time step gives way to the diagnostic period:
+  Period = diag_preiod, then dt = 1 / diag_freq / Period when Period > 0,
+  else Period = int(1 / diag_freq / dt); with diag_freq <= 0 a zero Period is
+  an error (init.F90:152-167);
+* a bundle's base weight and array sizes (AllInitializationParticleBundle,
+  R/src/pic/particle/ParticleBundle.f90:343-357);
+* the geometry's material indices: the file holds 0-based indices into the
+  materials list; metals become 1..nmetal and dielectrics -1..-ndiel in list
+  order, vacuum 0, a dielectric's cells take its permittivity, and each
+  metal cell marks its four nodes with its index, later cells in i-then-j
+  order overwriting earlier ones (init.F90:316-398); node volumes follow
+  init.F90:400-424 from the file's volume fractions.
+"""
+import dataclasses
+import json
+import math
+import os
+import re
+from typing import List, Optional, Tuple
+
+import numpy as np
+
+from . import geometry as G
+from .rezoning import RezoningLimits
+
+INT_MAX = 2 ** 31 - 1
+
+CIRCUIT_GROUND, CIRCUIT_VOLTAGE, CIRCUIT_PULSE, CIRCUIT_RLC, CIRCUIT_IMN = 1, 2, 3, 4, 5
+DEPS_UNIFORM, DEPS_SPECIFY = 0, 1
+
+
+class ConfigError(ValueError):
+    """A key the reference reads is missing or does not convert (the
+    reference stops with "The key is not exit in josn file.")."""
+
+
+# ------------------------------------------------------------ json-fortran
+_MISSING = object()
+_PATH_TOKEN = re.compile(r"([^.\[\]]+)|\[(\d+)\]")
+
+
+def _first_key_object(pairs):
+    out = {}
+    for k, v in pairs:
+        if k not in out:
+            out[k] = v
+    return out
+
+
+def _lookup(doc, path):
+    node = doc
+    pos = 0
+    while pos < len(path):
+        if path[pos] == ".":
+            pos += 1
+            continue
+        m = _PATH_TOKEN.match(path, pos)
+        if not m:
+            raise ConfigError(f"bad path {path!r}")
+        pos = m.end()
+        if m.group(1) is not None:
+            if not isinstance(node, dict) or m.group(1) not in node:
+                return _MISSING
+            node = node[m.group(1)]
+        else:
+            i = int(m.group(2))
+            if not isinstance(node, list) or not 1 <= i <= len(node):
+                return _MISSING
+            node = node[i - 1]
+    return node
+
+
+def _to_int(v):
+    if isinstance(v, bool):
+        return 1 if v else 0
+    if isinstance(v, int):
+        return v
+    if isinstance(v, float):
+        if not math.isfinite(v) or abs(v) >= 2.0 ** 31:
+            return _MISSING
+        return int(v)                       # int(dbl_value, IK): toward zero
+    if isinstance(v, str):
+        t = v.replace(" ", "")               # an Iw read skips blanks
+        if not re.fullmatch(r"[+-]?\d+", t):
+            return _MISSING
+        x = int(t)
+        return x if -INT_MAX - 1 <= x <= INT_MAX else _MISSING
+    return _MISSING
+
+
+def _to_real(v):
+    if isinstance(v, bool):
+        return 1.0 if v else 0.0
+    if isinstance(v, (int, float)):
+        return float(v)
+    if v is None:
+        return float("nan")                  # null_to_real_mode 2, quiet NaN
+    if isinstance(v, str):
+        from .restart import _value
+        try:
+            return _value(v)
+        except ValueError:
+            return _MISSING
+    return _MISSING
+
+
+def _to_bool(v):
+    if isinstance(v, bool):
+        return v
+    if isinstance(v, (int, float)):
+        return v > 0
+    if isinstance(v, str):
+        return v == "true"
+    return _MISSING
+
+
+def _to_str(v):
+    return v if isinstance(v, str) else _MISSING
+
+
+class _Doc:
+    def __init__(self, doc, source):
+        self.doc, self.source = doc, source
+
+    def _get(self, path, conv):
+        v = _lookup(self.doc, path)
+        if v is _MISSING:
+            raise ConfigError(f"{self.source}: key {path!r} not found")
+        x = conv(v)
+        if x is _MISSING:
+            raise ConfigError(f"{self.source}: key {path!r} does not convert ({v!r})")
+        if conv is _to_int and not -INT_MAX - 1 <= x <= INT_MAX:
+            raise ConfigError(f"{self.source}: key {path!r} exceeds a 4-byte integer")
+        return x
+
+    def int(self, path):
+        return self._get(path, _to_int)
+
+    def real(self, path):
+        return self._get(path, _to_real)
+
+    def bool(self, path):
+        return self._get(path, _to_bool)
+
+    def str(self, path):
+        return self._get(path, _to_str)
+
+
+def _load_json(path):
+    with open(path) as f:
+        return _Doc(json.load(f, object_pairs_hook=_first_key_object), str(path))
+
+
+# ------------------------------------------------------------ structures
+@dataclasses.dataclass
+class Diag:
+    diag_type: int
+    cycles: int
+    period: int              # diag_preiod as read
+    freq: float
+    nt: int
+
+
+@dataclasses.dataclass
+class Grid:
+    z_length: float
+    geometry: str            # path as resolved against the configuration's folder
+    materials: str
+    deps_z: int
+    deps_r: int
+    deps_type: int
+    specified_z: Optional[List[float]] = None
+    specified_r: Optional[List[float]] = None
+
+
+@dataclasses.dataclass
+class Plasma:
+    init_density: float
+    init_electron_temperature: float
+    ppg: int
+    alpha: float
+    load_type: int
+    particle_number_max_ratio: float
+    particle_limit_max_velocity: float
+    rezoning: RezoningLimits
+
+
+@dataclasses.dataclass
+class Gas:
+    ng: int
+    pressure: float
+    temperature: float
+    names: List[str]
+    ratios: List[float]
+    collision_section_type: int
+
+
+@dataclasses.dataclass
+class RecombinationSpec:
+    enabled: bool
+    time_step_number: int = 0
+    reactions: List[Tuple[int, int, int]] = dataclasses.field(default_factory=list)
+    rates: List[float] = dataclasses.field(default_factory=list)
+
+
+@dataclasses.dataclass
+class Circuit:
+    """One entry of circuit.circuits_list.  ``index`` is its 1-based rank
+    among the circuits of its type (EC%circuits_index, init.F90:619-622)."""
+    type: int
+    metal_index: int
+    name: str
+    index: int
+    vdc: float = 0.0
+    frequency: List[float] = dataclasses.field(default_factory=list)
+    amplitude: List[float] = dataclasses.field(default_factory=list)
+    phase: List[float] = dataclasses.field(default_factory=list)
+    time: List[float] = dataclasses.field(default_factory=list)
+    params: dict = dataclasses.field(default_factory=dict)
+
+
+@dataclasses.dataclass
+class Material:
+    type: int
+    name: str
+    permittivity: Optional[float] = None
+    permeability: Optional[float] = None
+    conductivity: Optional[float] = None
+
+
+@dataclasses.dataclass
+class RunConfig:
+    source: str
+    nrun: int
+    dt: float                # after the diagnostic-period rule
+    dt_read: float           # control.dt as written
+    period: int              # CF%Period
+    is_restart: bool
+    pic_type: int
+    diag: Diag
+    grid: Grid
+    plasma: Plasma
+    gas: Gas
+    recombination: RecombinationSpec
+    circuits: List[Circuit]
+    nz: int
+    nr: int
+
+    @property
+    def dz(self):
+        return self.grid.z_length / float(self.nz - 1)
+
+    @property
+    def r_length(self):
+        return self.grid.z_length / float(self.nz - 1) * float(self.nr - 1)
+
+    def base_weight(self, ppg=None):
+        """PB%Weight at load, ParticleBundle.f90:352."""
+        ppg = self.plasma.ppg if ppg is None else int(ppg)
+        nz, nr, d = float(self.nz - 1), float(self.nr - 1), self.dz
+        return (self.plasma.init_density * G.PI * (nr * d) * (nr * d) * (nz * d)) / (nr * nz * float(ppg))
+
+    def bundle_sizes(self, ppg=None):
+        """(initial size, npar_max) of a bundle, ParticleBundle.f90:354-357."""
+        ppg = self.plasma.ppg if ppg is None else int(ppg)
+        cells = (self.nz - 1) * (self.nr - 1)
+        return ppg * cells, int(math.ceil(self.plasma.particle_number_max_ratio * ppg * (self.nz - 1) * (self.nr - 1)))
+
+    def cell_load_weight(self, ir, ppg=None):
+        """Wq of a particle loaded in radial cell ir (1-based j), ParticleBundle.f90:368."""
+        ppg = self.plasma.ppg if ppg is None else int(ppg)
+        d = self.dz
+        return self.plasma.init_density / float(ppg) * 2.0 * G.PI * d * d * d * float(ir)
+
+    def rezoning(self, hp, ppg=None):
+        """A Rezoning control for the species of ``hp`` with the base weight
+        every bundle starts from (ppg_dif(k) = ppg for every species,
+        init.F90:552-556)."""
+        from .rezoning import Rezoning
+        ppg = self.plasma.ppg if ppg is None else int(ppg)
+        w = self.base_weight(ppg)
+        return Rezoning(hp, self.nz, self.nr, [w] * hp.ns, [ppg] * hp.ns, alpha=self.plasma.alpha,
+                        period=self.period, limits=dataclasses.replace(self.plasma.rezoning))
+
+    def recombination_control(self, hp):
+        from .recombination import Recombination
+        r = self.recombination
+        if not r.enabled:
+            return Recombination(hp, np.zeros((0, 3), dtype=np.int64), [], 1, self.dt, enabled=False)
+        return Recombination(hp, r.reactions, r.rates, r.time_step_number, self.dt, enabled=True)
+
+    def imn(self, **kw):
+        """circuit.IMN from the first IMN entry (its single frequency; the
+        electrode capacitance and induced charge come from the field solve,
+        keyword arguments)."""
+        from .circuit import IMN
+        for c in self.circuits:
+            if c.type != CIRCUIT_IMN:
+                continue
+            if len(c.frequency) != 1:
+                raise ValueError("the integrator drives one frequency; this IMN entry has "
+                                 f"{len(c.frequency)}")
+            p = c.params
+            return IMN(Rs=p["Rs"], Cm1=p["Cm1"], Cm2=p["Cm2"], Lm=p["Lm"], Rm=p["Rm"],
+                       stray=p["is_open_capacitance"], C_stray=p["C_stray"], L_stray=p["L_stray"],
+                       R_stray=p["R_stray"], frequency=c.frequency[0], amplitude=c.amplitude[0],
+                       phase_deg=c.phase[0], **kw)
+        raise ValueError("no IMN circuit among the first circuits_number entries")
+
+
+# ------------------------------------------------------------ reading
+def _circuits(J):
+    n = J.int("circuit.circuits_number")
+    out = []
+    counts = {}
+    for i in range(1, n + 1):
+        J.int(f"circuit.circuits_list[{i}].type")   # every type is read first, init.F90:590-598
+    for i in range(1, n + 1):
+        p = f"circuit.circuits_list[{i}]"
+        t = J.int(p + ".type")
+        metal = J.int(p + ".metal_index")
+        name = J.str(p + ".name")
+        if t not in (CIRCUIT_GROUND, CIRCUIT_VOLTAGE, CIRCUIT_PULSE, CIRCUIT_RLC, CIRCUIT_IMN):
+            continue                                 # no case of the select matches
+        counts[t] = counts.get(t, 0) + 1
+        c = Circuit(t, metal, name, counts[t])
+        if t == CIRCUIT_VOLTAGE:
+            nf = J.int(p + ".n_freq")
+            c.vdc = J.real(p + ".Vdc")
+            for j in range(1, nf + 1):
+                c.frequency.append(J.real(f"{p}.frequency[{j}]"))
+                c.amplitude.append(J.real(f"{p}.amplitude[{j}]"))
+                c.phase.append(J.real(f"{p}.phase[{j}]"))
+        elif t == CIRCUIT_PULSE:
+            seg = J.int(p + ".seg")
+            if seg > 1:
+                for j in range(1, seg + 1):
+                    c.time.append(J.real(f"{p}.time[{j}]"))
+                    c.amplitude.append(J.real(f"{p}.amplitude[{j}]"))
+        elif t in (CIRCUIT_RLC, CIRCUIT_IMN):
+            nf = J.int(p + ".n_freq")
+            if nf > 0:
+                for j in range(1, nf + 1):
+                    c.frequency.append(J.real(f"{p}.frequency[{j}]"))
+                    c.amplitude.append(J.real(f"{p}.amplitude[{j}]"))
+                    c.phase.append(J.real(f"{p}.phase[{j}]"))
+                if t == CIRCUIT_RLC:
+                    keys = (("Rs", J.real), ("Cs", J.real), ("Ls", J.real))
+                else:
+                    keys = (("Rs", J.real), ("Cm1", J.real), ("Cm2", J.real), ("Lm", J.real), ("Rm", J.real),
+                            ("is_open_capacitance", J.bool), ("C_stray", J.real), ("L_stray", J.real),
+                            ("R_stray", J.real))
+                for k, get in keys:
+                    c.params[k] = get(f"{p}.{k}")
+        out.append(c)
+    return out
+
+
+def read_materials(path):
+    """The materials list of grid.materials (init.F90:316-385)."""
+    J = _load_json(path)
+    n = J.int("material_amount")
+    if n <= 0:
+        raise ConfigError(f"{path}: number of materials less than 0")
+    out = []
+    for i in range(1, n + 1):
+        p = f"materials[{i}]"
+        t = J.int(p + ".type")
+        m = Material(t, J.str(p + ".name"))
+        if t == G.MAT_DIEL:
+            m.permittivity = J.real(p + ".permittivity")
+            m.permeability = J.real(p + ".permeability")
+            m.conductivity = J.real(p + ".conductivity")
+        out.append(m)
+    return out
+
+
+def _resolve(base, p):
+    return p if os.path.isabs(p) else os.path.normpath(os.path.join(base, p))
+
+
+def load(path, nz=None, nr=None):
+    """Read a config.json.  Relative paths in it are taken from the file's
+    folder (the reference runs with that folder as its working directory).
+    Nz, Nr come from the geometry file unless given."""
+    J = _load_json(path)
+    base = os.path.dirname(os.path.abspath(path))
+    diag = Diag(J.int("diag.diag_type"), J.int("diag.diag_cycles"), J.int("diag.diag_preiod"),
+                J.real("diag.diag_freq"), J.int("diag.diag_Nt"))
+    nrun = J.int("control.nrun")
+    dt = J.real("control.dt")
+    z_length = J.real("grid.z_length")
+    is_restart = J.bool("control.is_restart")
+    load_type = J.int("plasma.load_type")
+    pic_type = J.int("control.pic_type")
+    geom_path = _resolve(base, J.str("grid.geometry"))
+    if nz is None or nr is None:
+        from . import h5lite
+        a = h5lite.File(geom_path).attrs
+        nz, nr = int(a["Nz"]), int(a["Nr"])
+    deps = (J.int("grid.deps_in_z"), J.int("grid.deps_in_r"), J.int("grid.deps_type"))
+    dt_read = dt
+    period = diag.period
+    if diag.freq > 0.0:
+        if diag.period == 0:
+            period = int(1.0 / diag.freq / dt)
+        else:
+            dt = 1.0 / diag.freq / float(period)
+    if diag.freq <= 0.0 and period == 0:
+        raise ConfigError(f"{path}: invalid diag period")
+    grid = Grid(z_length, geom_path, "", *deps)
+    if deps[2] == DEPS_SPECIFY:
+        grid.specified_z = [J.real(f"grid.specified_deps_in_z[{i}]") for i in range(1, deps[0] + 1)]
+        grid.specified_r = [J.real(f"grid.specified_deps_in_r[{i}]") for i in range(1, deps[1] + 1)]
+    limits = RezoningLimits(J.bool("plasma.is_open_split_merge"), J.real("plasma.paritlc_upper_limit"),
+                            J.real("plasma.paritlc_lower_limit"), J.real("plasma.min_mp_to_split_k"),
+                            J.real("plasma.merge_limit"))
+    grid.materials = _resolve(base, J.str("grid.materials"))
+    ng = J.int("gas.ng")
+    pressure, temperature = J.real("gas.pressure"), J.real("gas.temperature")
+    te = J.real("plasma.init_electron_temperature")
+    cst = J.int("gas.collision_section_type")
+    if ng <= 0:
+        raise ConfigError(f"{path}: the amount of gas must be greater than 0")
+    names = [J.str(f"gas.name[{i}]") for i in range(1, ng + 1)]
+    ratios = []
+    for i in range(1, ng + 1):
+        ratios.append(J.real(f"gas.ratio[{i}]"))
+    gas = Gas(ng, pressure, temperature, names, ratios, cst)
+    density, ppg = J.real("plasma.init_density"), J.int("plasma.ppg")
+    max_ratio, max_v = J.real("plasma.particle_number_max_ratio"), J.real("plasma.particle_limit_max_velocity")
+    plasma = Plasma(density, te, ppg, J.real("plasma.alpha"), load_type, max_ratio, max_v, limits)
+    circuits = _circuits(J)
+    recb = RecombinationSpec(J.bool("gas.is_open_recombination"))
+    if recb.enabled:
+        recb.time_step_number = J.int("gas.recb.recb_time_step_num")
+        n = J.int("gas.recb.recb_reaction_num")
+        for i in range(1, n + 1):
+            r = tuple(J.int(f"gas.recb.recb_list[{i}][{k}]") for k in (1, 2, 3))
+            recb.reactions.append(r)
+            recb.rates.append(J.real(f"gas.recb.recb_rate[{i}]"))
+    return RunConfig(str(path), nrun, dt, dt_read, period, is_restart, pic_type, diag, grid, plasma, gas,
+                     recb, circuits, int(nz), int(nr))
+
+
+def geometry_from_files(run, materials=None):
+    """The reference's Geometry for ``run``: its geometry file and materials
+    list, indices remapped and volumes formed as initGeometryAndMaterialsJson
+    does (init.F90:216-430)."""
+    from . import h5lite
+    mats = read_materials(run.grid.materials) if materials is None else materials
+    g = h5lite.read_geometry(run.grid.geometry)
+    nz, nr = g["Nz"], g["Nr"]
+    if (nz, nr) != (run.nz, run.nr):
+        raise ValueError(f"geometry is {nz} x {nr}, configuration says {run.nz} x {run.nr}")
+    raw = np.asarray(g["cell_material"], dtype=np.int64)
+    remap = np.zeros(len(mats), dtype=np.int32)
+    eps = np.array(g["epsilon_r"], dtype=np.float64, copy=True)
+    ndiel = nmetal = 0
+    for i, m in enumerate(mats):
+        if m.type == G.MAT_DIEL:
+            ndiel += 1
+            remap[i] = -ndiel
+            eps[raw == i] = m.permittivity
+        elif m.type == G.MAT_METAL:
+            nmetal += 1
+            remap[i] = nmetal
+    if raw.min() < 0 or raw.max() >= len(mats):
+        raise ValueError("the geometry names a material the materials list does not hold")
+    cm = remap[raw]
+    mt = np.zeros(ndiel + nmetal + 1, dtype=np.int32)
+    mt[:ndiel] = G.MAT_DIEL
+    mt[ndiel] = G.MAT_VACUUM
+    mt[ndiel + 1:] = G.MAT_METAL
+    factor = np.asarray(g["volume"], dtype=np.float64)
+    vol, inv = G.node_volumes(nz, nr, run.dz, factor)
+    metal = mt[cm + ndiel] == G.MAT_METAL
+    pt = G.metal_point_types(g["point_type"], cm, metal)
+    return G.Geometry(nz=nz, nr=nr, dx=run.dz, cell_material=cm, material_type=mt, ndiel=ndiel, nmetal=nmetal,
+                      hedge_type=np.asarray(g["hedge_type"], dtype=np.int32),
+                      vedge_type=np.asarray(g["vedge_type"], dtype=np.int32), volume_factor=factor,
+                      volume=vol, inv_volume=inv, cell_load=np.asarray(g["load"], dtype=np.int32),
+                      cell_epsilon=eps, point_type=pt)

@@ -1,0 +1,251 @@
+"""The reference's run configuration (ipm2d.refconfig), CPU checks:
+
+* the reference's own config.json and the inputs it names (skipped where the
+  reference tree is absent; nothing of it is copied here):
+  - Nz, Nr come from the geometry file and dt from the diagnostic period
+    (1 / 13.56 MHz / 150, not the 5e-10 written under control.dt), as
+    InitControlFlowJson does (R/src/base/init.F90:105-167);
+  - only the first circuits_number = 3 entries of the seven-entry
+    circuits_list are taken, each with its type's keys;
+  - the string "1" under plasma.load_type reads as the integer 1;
+  - the geometry rebuilt from geom.h5 and geom.json agrees with what the
+    file already holds: edge types follow the materials' types, the
+    dielectrics' permittivities equal the file's epsilon_r, metal nodes are
+    exactly the nodes the file marks as metal, and the node volumes are
+    positive exactly where the file's volume fractions exceed 1e-6;
+  - the base weight times the particles at load equals the initial density
+    times the cylinder's volume;
+* json-fortran's reading rules on synthetic configurations: type
+  conversions, 1-based paths, first of duplicate keys, keys read only where
+  the reference reads them, missing keys, the diagnostic-period branches;
+* the metal node marking against the reference's loop (i outer, j inner,
+  later cells overwrite) on small random maps where metals touch.
+"""
+import copy
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from ipm2d import geometry as G
+from ipm2d import h5lite, refconfig as RC
+
+REF = "/root/reference/iPM2D-multi-EI/config.json"
+needs_ref = pytest.mark.skipif(not os.path.exists(REF), reason="reference tree absent")
+
+BASE = {
+    "control": {"nrun": 10, "dt": 1e-10, "is_restart": False, "pic_type": 1},
+    "diag": {"diag_type": 1, "diag_cycles": 1, "diag_preiod": 20, "diag_freq": 0.0, "diag_Nt": 5},
+    "grid": {"z_length": 0.02, "deps_in_z": 1, "deps_in_r": 1, "deps_type": 0,
+             "materials": "m.json", "geometry": "g.h5"},
+    "plasma": {"init_density": 1e15, "init_electron_temperature": 3.0, "ppg": 8, "alpha": 0.05,
+               "is_open_split_merge": True, "paritlc_upper_limit": 1.02, "paritlc_lower_limit": 0.98,
+               "min_mp_to_split_k": 0.2, "merge_limit": 0.3, "load_type": 1,
+               "particle_number_max_ratio": 4.0, "particle_limit_max_velocity": 10.0},
+    "gas": {"ng": 1, "pressure": 50, "temperature": 300, "name": ["CF4"], "ratio": [1.0],
+            "collision_section_type": 1, "is_open_recombination": False},
+    "circuit": {"circuits_number": 1, "circuits_list": [{"type": 1, "metal_index": 2, "name": "ground"}]},
+}
+
+
+def _write(tmp_path, doc, text=None):
+    p = tmp_path / "config.json"
+    p.write_text(text if text is not None else json.dumps(doc))
+    return p
+
+
+def _load(tmp_path, doc=None, text=None):
+    return RC.load(_write(tmp_path, doc, text), nz=9, nr=5)
+
+
+# ------------------------------------------------------------ the reference's input
+@needs_ref
+def test_reference_config_derived_quantities():
+    run = RC.load(REF)
+    assert (run.nz, run.nr) == (201, 201)
+    assert run.dt_read == 5e-10
+    assert run.period == 150 and run.dt == 1.0 / 13.56e6 / 150.0
+    assert run.dz == 0.1054 / 200.0 and run.r_length == run.dz * 200.0
+    assert run.plasma.load_type == 1 and run.plasma.ppg == 50
+    assert [(c.type, c.metal_index, c.index) for c in run.circuits] == [(5, 1, 1), (4, 2, 1), (1, 3, 1)]
+    rlc = run.circuits[1]
+    assert rlc.params == {"Rs": 0.5, "Cs": 5e-09, "Ls": 1e-08} and rlc.amplitude == [50.0]
+    imn = run.imn()
+    assert (imn.Rs, imn.Cm1, imn.Cm2, imn.Lm, imn.Rm) == (50.0, 1.5e-10, 2e-10, 4.3e-6, 0.5)
+    assert imn.stray and (imn.C_stray, imn.L_stray, imn.R_stray) == (2.4e-11, 1e-8, 0.5)
+    assert (imn.frequency, imn.amplitude, imn.phase_deg) == (13.56e6, 100.0, 0.0)
+    r = run.recombination
+    assert r.enabled and r.time_step_number == 50
+    assert r.reactions == [(1, 2, 0), (2, 3, 0), (0, 2, 1)] and r.rates == [1e-13, 1e-13, 3.95e-15]
+    lim = run.plasma.rezoning
+    assert (lim.is_open_split_merge, lim.paritlc_upper_limit, lim.paritlc_lower_limit,
+            lim.min_mp_to_split_k, lim.merge_limit) == (True, 1.01, 0.99, 0.1, 0.34)
+    assert run.gas.names == ["CF4"] and run.gas.pressure == 100.0 and run.gas.temperature == 300.0
+    # total weight at load = density x cylinder volume
+    n0, _ = run.bundle_sizes()
+    vol = math.pi * run.r_length ** 2 * run.grid.z_length
+    assert run.base_weight() * n0 == pytest.approx(run.plasma.init_density * vol, rel=1e-14)
+    assert run.bundle_sizes()[1] == 100 * n0
+
+
+@needs_ref
+def test_reference_geometry_rebuilt_from_its_inputs():
+    run = RC.load(REF)
+    mats = RC.read_materials(run.grid.materials)
+    assert [m.type for m in mats] == [0, 1, 1, 1, 2, 2]
+    g = RC.geometry_from_files(run, mats)
+    f = h5lite.read_geometry(run.grid.geometry)
+    assert (g.nmetal, g.ndiel) == (3, 2)
+    assert set(np.unique(g.cell_material)) == {-2, -1, 0, 1, 2, 3}
+    # metals 1..3 and dielectrics -1..-2 in list order
+    for i, want in enumerate([0, 1, 2, 3, -1, -2]):
+        assert np.all(g.cell_material[f["cell_material"] == i] == want)
+    ctype = np.array([m.type for m in mats])[f["cell_material"]]
+    he, ve = G._edge_types(ctype)
+    np.testing.assert_array_equal(g.hedge_type, he)
+    np.testing.assert_array_equal(g.vedge_type, ve)
+    np.testing.assert_array_equal(g.cell_epsilon, f["epsilon_r"])
+    np.testing.assert_array_equal(g.point_type > 0, f["point_type"] == 1)
+    np.testing.assert_array_equal(g.point_type[g.point_type <= 0], f["point_type"][g.point_type <= 0])
+    assert np.array_equal(g.inv_volume > 0, f["volume"] > 1e-6)
+    assert np.all(g.mtype(g.cell_material)[f["load"] == 1] == G.MAT_VACUUM)
+
+
+# ------------------------------------------------------------ json-fortran rules
+def test_type_conversions(tmp_path):
+    d = copy.deepcopy(BASE)
+    d["plasma"]["load_type"] = " 1 2 "           # Iw read: blanks skipped
+    d["plasma"]["ppg"] = 8.9                     # real -> integer toward zero
+    d["control"]["nrun"] = True                  # logical -> 1
+    d["control"]["is_restart"] = 2               # number > 0 -> true
+    d["plasma"]["is_open_split_merge"] = "True"  # only "true" is true
+    d["gas"]["pressure"] = "1.5D+01"             # string -> real
+    d["plasma"]["alpha"] = None                  # null -> NaN
+    run = _load(tmp_path, d)
+    assert run.plasma.load_type == 12 and run.plasma.ppg == 8 and run.nrun == 1
+    assert run.is_restart is True and run.plasma.rezoning.is_open_split_merge is False
+    assert run.gas.pressure == 15.0 and math.isnan(run.plasma.alpha)
+    d["plasma"]["ppg"] = -8.9
+    assert _load(tmp_path, d).plasma.ppg == -8
+    for bad in ("1.5", "x", ""):
+        d["plasma"]["load_type"] = bad
+        with pytest.raises(RC.ConfigError):
+            _load(tmp_path, d)
+    d["plasma"]["load_type"] = 2 ** 31
+    with pytest.raises(RC.ConfigError):
+        _load(tmp_path, d)
+
+
+def test_first_duplicate_key_wins_and_keys_are_case_sensitive(tmp_path):
+    text = json.dumps(BASE).replace('"ppg": 8', '"ppg": 8, "ppg": 99, "PPG": 7')
+    assert _load(tmp_path, text=text).plasma.ppg == 8
+    d = copy.deepcopy(BASE)
+    d["plasma"]["PPG"] = d["plasma"].pop("ppg")
+    with pytest.raises(RC.ConfigError, match="plasma.ppg"):
+        _load(tmp_path, d)
+
+
+def test_missing_keys_fail_and_conditional_keys_are_optional(tmp_path):
+    d = copy.deepcopy(BASE)
+    run = _load(tmp_path, d)                         # no recb, no specified deps: fine
+    assert not run.recombination.enabled and run.grid.specified_z is None
+    d["grid"]["deps_type"] = 1
+    d["grid"]["deps_in_z"] = 2
+    with pytest.raises(RC.ConfigError, match="specified_deps_in_z"):
+        _load(tmp_path, d)
+    d["grid"]["specified_deps_in_z"] = [1, 3]
+    d["grid"]["specified_deps_in_r"] = [2]
+    run = _load(tmp_path, d)
+    assert run.grid.specified_z == [1.0, 3.0] and run.grid.specified_r == [2.0]
+    d["gas"]["is_open_recombination"] = True
+    with pytest.raises(RC.ConfigError, match="recb_time_step_num"):
+        _load(tmp_path, d)
+    d["gas"]["recb"] = {"recb_time_step_num": 5, "recb_reaction_num": 2,
+                        "recb_list": [[1, 2, 0], [0, 1, 1]], "recb_rate": [1e-13]}
+    with pytest.raises(RC.ConfigError, match=r"recb_rate\[2\]"):
+        _load(tmp_path, d)
+    d["gas"]["recb"]["recb_rate"].append(2e-14)
+    r = _load(tmp_path, d).recombination
+    assert r.reactions == [(1, 2, 0), (0, 1, 1)] and r.rates == [1e-13, 2e-14]
+    for sec, key in (("gas", "temperature"), ("control", "dt"), ("diag", "diag_Nt")):
+        e = copy.deepcopy(BASE)
+        del e[sec][key]
+        with pytest.raises(RC.ConfigError, match=f"{sec}.{key}"):
+            _load(tmp_path, e)
+    e = copy.deepcopy(BASE)
+    e["gas"]["ng"] = 0
+    with pytest.raises(RC.ConfigError):
+        _load(tmp_path, e)
+
+
+def test_circuits_read_as_the_reference_reads_them(tmp_path):
+    d = copy.deepcopy(BASE)
+    d["circuit"] = {"circuits_number": 5, "circuits_list": [
+        {"type": 2, "metal_index": 1, "name": "v", "Vdc": 3, "n_freq": 2, "frequency": [1e6, 2e6],
+         "amplitude": [10, 20], "phase": [0, 90]},
+        {"type": 7, "metal_index": 4, "name": "unknown type"},
+        {"type": 3, "metal_index": 2, "name": "pulse one segment", "seg": 1},
+        {"type": 3, "metal_index": 2, "name": "pulse", "seg": 2, "time": [0, 1e-8], "amplitude": [0, 5]},
+        {"type": 4, "metal_index": 3, "name": "rlc without frequencies", "n_freq": 0},
+        {"type": 5, "metal_index": 1, "name": "beyond circuits_number"},
+    ]}
+    c = _load(tmp_path, d).circuits
+    assert [(x.type, x.index, x.name) for x in c] == [
+        (2, 1, "v"), (3, 1, "pulse one segment"), (3, 2, "pulse"), (4, 1, "rlc without frequencies")]
+    assert c[0].vdc == 3.0 and c[0].frequency == [1e6, 2e6] and c[0].phase == [0.0, 90.0]
+    assert c[1].time == [] and c[2].time == [0.0, 1e-8] and c[2].amplitude == [0.0, 5.0]
+    assert c[3].params == {} and c[3].frequency == []
+    with pytest.raises(ValueError, match="no IMN"):
+        _load(tmp_path, d).imn()
+    d["circuit"]["circuits_number"] = 6
+    with pytest.raises(RC.ConfigError, match=r"circuits_list\[6\].n_freq"):
+        _load(tmp_path, d)
+    d["circuit"]["circuits_number"] = 7                    # every type is read before any entry
+    with pytest.raises(RC.ConfigError, match=r"circuits_list\[7\].type"):
+        _load(tmp_path, d)
+
+
+def test_diagnostic_period_branches(tmp_path):
+    d = copy.deepcopy(BASE)
+    run = _load(tmp_path, d)                                # diag_freq 0: dt kept
+    assert run.dt == 1e-10 and run.period == 20
+    d["diag"]["diag_freq"] = 13.56e6                        # period given: dt replaced
+    run = _load(tmp_path, d)
+    assert run.period == 20 and run.dt == 1.0 / 13.56e6 / 20.0
+    d["diag"]["diag_preiod"] = 0                            # period from dt
+    run = _load(tmp_path, d)
+    assert run.dt == 1e-10 and run.period == int(1.0 / 13.56e6 / 1e-10) == 737
+    d["diag"]["diag_freq"] = 0.0
+    with pytest.raises(RC.ConfigError, match="diag period"):
+        _load(tmp_path, d)
+
+
+def test_relative_paths_follow_the_config_folder(tmp_path):
+    run = _load(tmp_path, BASE)
+    assert run.grid.geometry == str(tmp_path / "g.h5") and run.grid.materials == str(tmp_path / "m.json")
+
+
+# ------------------------------------------------------------ metal node marking
+def _reference_marking(pt, cm, metal):
+    out = np.array(pt, copy=True)
+    for i in range(cm.shape[0]):
+        for j in range(cm.shape[1]):
+            if metal[i, j]:
+                for a in (0, 1):
+                    for b in (0, 1):
+                        out[i + a, j + b] = cm[i, j]
+    return out
+
+
+def test_metal_marking_matches_the_loop_order(rng):
+    for _ in range(50):
+        cm = rng.integers(-1, 4, size=(7, 6)).astype(np.int32)
+        metal = cm > 0
+        pt = rng.integers(-2, 2, size=(8, 7)).astype(np.int32)
+        np.testing.assert_array_equal(G.metal_point_types(pt, cm, metal), _reference_marking(pt, cm, metal))
+    # the order matters where two metals share a node
+    cm = np.array([[1, 2], [3, 1]], dtype=np.int32)
+    got = G.metal_point_types(np.zeros((3, 3), np.int32), cm, cm > 0)
+    assert got[1, 1] == 1 and got[0, 1] == 2 and got[1, 0] == 3
