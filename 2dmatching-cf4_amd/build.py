@@ -4,6 +4,7 @@
 
 Outputs:
   ipm2d/_lib/libipm_hip.so      HIP kernels + C ABI (include/ipm_hip.h)
+  ipm2d/_lib/libipm_probe.so    bandwidth calibration kernels the benchmark runs
   fortran/build/                iso_c_binding module and the OneStep driver
 """
 import os
@@ -16,6 +17,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIBDIR = os.path.join(PKG, "ipm2d", "_lib")
 LIB = os.path.join(LIBDIR, "libipm_hip.so")
+PROBE_LIB = os.path.join(LIBDIR, "libipm_probe.so")
 FDIR = os.path.join(PKG, "fortran")
 FBUILD = os.path.join(FDIR, "build")
 
@@ -44,6 +46,9 @@ def build_hip():
     tmp = LIB + ".tmp"
     run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC, *srcs, "-o", tmp])
     os.replace(tmp, LIB)
+    tmp = PROBE_LIB + ".tmp"
+    run([HIPCC, *HIP_FLAGS, os.path.join(CSRC, "ipm_probe.hip"), "-o", tmp])
+    os.replace(tmp, PROBE_LIB)
     return LIB
 
 

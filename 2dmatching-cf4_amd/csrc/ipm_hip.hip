@@ -16,6 +16,8 @@
 #include <vector>
 #include <algorithm>
 #include <cmath>
+#include <hipcub/block/block_radix_sort.hpp>
+#include <hipcub/block/block_scan.hpp>
 #include "ipm_device.hpp"
 
 using namespace ipm;
@@ -1164,71 +1166,97 @@ __global__ void rezone_fold_kernel(DevCounters *ctr, int s, long long cap)
 
 // ------------------------------------------------------------ cell sort
 
-// Wave-aggregated increment: lanes with equal keys share one atomic.
-// Returns each lane's position among the increments made to its key.
-__device__ __forceinline__ unsigned long long wave_bump(unsigned long long *ctr, unsigned long long key,
-                                                        bool valid)
-{
-    const int lane = threadIdx.x & 63;
-    unsigned long long active = __ballot(valid);
-    unsigned long long mine = 0;
-    while (active) {
-        const int leader = __ffsll((long long)active) - 1;
-        const unsigned long long lkey = __shfl(key, leader);
-        const unsigned long long same = __ballot(valid && key == lkey) & active;
-        unsigned long long base = 0;
-        if (lane == leader) base = atomicAdd(&ctr[lkey], (unsigned long long)__popcll(same));
-        base = __shfl(base, leader);
-        if ((same >> lane) & 1ull) mine = base + __popcll(same & ((1ull << lane) - 1ull));
-        active &= ~same;
-    }
-    return mine;
-}
+// One block per tile of kSortTile slots.  The tile's cell keys are sorted in
+// LDS first, so each run of equal cells in the tile costs one atomic on that
+// cell's counter, and the run's particles go to consecutive slots of the
+// cell's range.  Dead slots and slots past n take the sentinel key ncell and
+// are dropped.  The order of particles within a cell depends on the order in
+// which tiles reach the counters; nothing downstream depends on it (the
+// collision draws are keyed by particle state, not by slot).
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kBlock * kSortItems;
+using TileSort = hipcub::BlockRadixSort<unsigned, kBlock, kSortItems, unsigned short>;
+using TileScan = hipcub::BlockScan<int, kBlock>;
 
+struct SortTileLds {
+    union {
+        typename TileSort::TempStorage sort;
+        typename TileScan::TempStorage scan;
+        unsigned short src[kSortTile];     // tile slot of each sorted position
+    } tmp;
+    unsigned key[kSortTile];               // sorted keys, then destination slots
+    unsigned base[kSortTile];              // at a run's first position: its first rank in the cell
+};
+
+// SCATTER false: counts[cell] += particles of the cell (the histogram).
+// SCATTER true: counts[cell] hands out ranks within the cell (zeroed before),
+// and each particle's eight fields go to dst at offs[cell] + rank.
+template <bool SCATTER>
 __global__ void __launch_bounds__(kBlock)
-sort_hist_kernel(const DevState *__restrict__ stp, const DevCounters *__restrict__ ctr, int s,
-                 unsigned long long *__restrict__ hist)
+sort_tile_kernel(const DevState *__restrict__ stp, const DevCounters *__restrict__ ctr, int s, int end_bit,
+                 unsigned long long *__restrict__ counts, const unsigned long long *__restrict__ offs,
+                 double *const *__restrict__ dst)
 {
+    __shared__ SortTileLds L;
     const DevState &st = *stp;
     const long long n = (long long)ctr->n[s];
-    const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
-    bool valid = false;
-    unsigned long long key = 0;
-    if (i < n) {
-        double z = st.f[s][0][i];
-        if (!is_dead(z)) {
-            valid = true;
-            key = cell_of(st, z, st.f[s][1][i]);
+    const long long tbase = (long long)blockIdx.x * kSortTile;
+    const unsigned S = (unsigned)((st.nz - 1) * (st.nr - 1));
+    const int t = threadIdx.x;
+    unsigned key[kSortItems];
+    unsigned short loc[kSortItems];
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+        const int l = t + j * kBlock;
+        const long long i = tbase + l;
+        unsigned k = S;
+        if (i < n) {
+            const double z = st.f[s][0][i];
+            if (!is_dead(z)) k = (unsigned)cell_of(st, z, st.f[s][1][i]);
+        }
+        key[j] = k;
+        loc[j] = (unsigned short)l;
+    }
+    TileSort(L.tmp.sort).Sort(key, loc, 0, end_bit);      // blocked: thread t holds 8t .. 8t+7
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) L.key[t * kSortItems + j] = key[j];
+    __syncthreads();
+    int head[kSortItems], start[kSortItems];
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+        const int p = t * kSortItems + j;
+        head[j] = (p == 0 || L.key[p - 1] != key[j]) ? p : 0;
+    }
+    TileScan(L.tmp.scan).InclusiveScan(head, start, hipcub::Max());   // first position of each run
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+        const int p = t * kSortItems + j;
+        const bool run_end = p == kSortTile - 1 || L.key[p + 1] != key[j];
+        if (run_end && key[j] < S) {
+            const unsigned long long len = (unsigned long long)(p - start[j] + 1);
+            const unsigned long long b = atomicAdd(&counts[key[j]], len);
+            if (SCATTER) L.base[start[j]] = (unsigned)b;
         }
     }
-    (void)wave_bump(hist, key, valid);
-}
-
-__global__ void __launch_bounds__(kBlock)
-sort_scatter_kernel(const DevState *__restrict__ stp, const DevCounters *__restrict__ ctr, int s,
-                    const unsigned long long *__restrict__ offs, unsigned long long *__restrict__ cursor,
-                    double *const *__restrict__ dst)
-{
-    const DevState &st = *stp;
-    const long long n = (long long)ctr->n[s];
-    const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
-    bool valid = false;
-    unsigned long long key = 0;
-    double z = 0.0, r = 0.0;
-    if (i < n) {
-        z = st.f[s][0][i];
-        if (!is_dead(z)) {
-            valid = true;
-            r = st.f[s][1][i];
-            key = cell_of(st, z, r);
-        }
+    if (!SCATTER) return;
+    __syncthreads();                        // base[] written; tmp.scan and key[] reads done
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+        const int p = t * kSortItems + j;
+        L.key[p] = key[j] < S ? (unsigned)(offs[key[j]] + L.base[start[j]] + (unsigned)(p - start[j])) : 0xffffffffu;
+        L.tmp.src[p] = loc[j];
     }
-    unsigned long long rank = wave_bump(cursor, key, valid);
-    if (valid) {
-        const unsigned long long pos = offs[key] + rank;
-        dst[0][pos] = z;
-        dst[1][pos] = r;
-        for (int f = 2; f < 8; ++f) dst[f][pos] = st.f[s][f][i];
+    __syncthreads();
+    // striped copy: neighbouring lanes write neighbouring slots of a run
+#pragma unroll 2
+    for (int j = 0; j < kSortItems; ++j) {
+        const int p = t + j * kBlock;
+        const unsigned d = L.key[p];
+        if (d != 0xffffffffu) {
+            const long long i = tbase + L.tmp.src[p];
+#pragma unroll
+            for (int f = 0; f < 8; ++f) dst[f][d] = st.f[s][f][i];
+        }
     }
 }
 
@@ -1986,15 +2014,18 @@ int ipm_sort(ipm_ctx *c, int s)
     const size_t ncell = (size_t)(c->nz - 1) * (c->nr - 1);
     const unsigned nblk = (unsigned)((ncell + kScanBlock - 1) / kScanBlock);
     if (nblk > kScanBlock) return fail(IPM_EINVAL, "grid too large for the two-level scan");
+    int end_bit = 0;                        // bits of the sentinel key ncell
+    while (end_bit < 32 && (ncell >> end_bit) != 0) ++end_bit;
     DevCounters ctr;
     if ((rc = read_counters(c, &ctr))) return rc;
     for (int sp = s0; sp < s1; ++sp) {
         const size_t n = (size_t)ctr.n[sp];
         HIPCHK(hipMemsetAsync(c->hist, 0, sizeof(unsigned long long) * ncell, c->stream));
         HIPCHK(hipMemsetAsync(c->cursor, 0, sizeof(unsigned long long) * ncell, c->stream));
-        const unsigned pblk = (unsigned)((n + kBlock - 1) / kBlock);
+        const unsigned pblk = (unsigned)((n + kSortTile - 1) / kSortTile);
         if (pblk > 0)
-            hipLaunchKernelGGL(sort_hist_kernel, dim3(pblk), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr, sp, c->hist);
+            hipLaunchKernelGGL(sort_tile_kernel<false>, dim3(pblk), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr, sp,
+                               end_bit, c->hist, (const unsigned long long *)nullptr, (double *const *)nullptr);
         hipLaunchKernelGGL(scan_block_kernel, dim3(nblk), dim3(kScanBlock), 0, c->stream, c->hist, c->offs, c->partial, ncell);
         hipLaunchKernelGGL(scan_block_kernel, dim3(1), dim3(kScanBlock), 0, c->stream, c->partial, c->partial2,
                            c->partial + nblk, (size_t)nblk);
@@ -2006,8 +2037,8 @@ int ipm_sort(ipm_ctx *c, int s)
         for (int f = 0; f < 8; ++f) dst[f] = c->buf[other][sp] + (size_t)f * cap;
         HIPCHK(hipMemcpyAsync(c->d_dst, dst, sizeof dst, hipMemcpyHostToDevice, c->stream));
         if (pblk > 0)
-            hipLaunchKernelGGL(sort_scatter_kernel, dim3(pblk), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr, sp,
-                               c->offs, c->cursor, c->d_dst);
+            hipLaunchKernelGGL(sort_tile_kernel<true>, dim3(pblk), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr, sp,
+                               end_bit, c->cursor, c->offs, c->d_dst);
         hipLaunchKernelGGL(set_count_kernel, dim3(1), dim3(64), 0, c->stream, c->d_ctr, sp, c->offs, c->hist, ncell);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(c->stream));

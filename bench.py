@@ -31,6 +31,32 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def stream_ceiling(n, reps=5):
+    """What a pure streaming pass with the particle pass's access pattern
+    (8 arrays of n doubles read, 7 written: 120 B per element) and a plain
+    one-array copy reach on this device (csrc/ipm_probe.hip)."""
+    import ctypes as C
+    import torch
+    lib = C.CDLL(os.path.join(ROOT, "2dmatching-cf4_amd", "ipm2d", "_lib", "libipm_probe.so"))
+    lib.ipm_probe_stream.argtypes = [C.POINTER(C.c_void_p), C.c_longlong, C.c_int, C.c_void_p,
+                                     C.POINTER(C.c_double)]
+    lib.ipm_probe_stream.restype = C.c_int
+    arrs = [torch.zeros(n, dtype=torch.float64, device="cuda") for _ in range(8)]
+    ptrs = (C.c_void_p * 8)(*[a.data_ptr() for a in arrs])
+    ms = (C.c_double * 2)()
+    torch.cuda.synchronize()
+    rc = lib.ipm_probe_stream(ptrs, n, reps, C.c_void_p(torch.cuda.current_stream().cuda_stream), ms)
+    torch.cuda.synchronize()
+    del arrs
+    if rc != 0:
+        raise RuntimeError(f"ipm_probe_stream failed with HIP error {rc}")
+    return {"value": 120.0 * n / (ms[0] * 1e-3) / 1e9, "unit": "GB/s",
+            "copy_GBps": 16.0 * n / (ms[1] * 1e-3) / 1e9,
+            "what": f"median of {reps} passes of a kernel that reads 8 and writes 7 arrays of {n} doubles "
+                    f"(the particle pass's streams, no compute), one element per thread; copy_GBps: one-array "
+                    f"device copy (read + write)"}
+
+
 def live_total(hp, ns):
     return sum(hp.count(s) for s in range(ns))
 
@@ -293,6 +319,13 @@ def main():
         log(f"[bench] graph {extra['graph_value']:.4g}, plain {extra['plain_per_species_value']:.4g}")
     hp.close()
 
+    ceiling = None
+    if not args.traffic_probe:
+        ceiling = stream_ceiling(ntot)
+        ceiling["frac_of_ceiling"] = achieved / ceiling["value"]
+        log(f"[bench] streaming ceiling {ceiling['value']:.0f} GB/s (copy {ceiling['copy_GBps']:.0f}); "
+            f"particle kernel at {ceiling['frac_of_ceiling']:.3f} of it")
+
     adj = None
     if not args.no_adjoint:
         adj = adjoint_leg()
@@ -333,6 +366,7 @@ def main():
             "kernel": "particle_kernel<push|absorb|deposit|mcc-select>",
             "kernel_avg_ms": avg_s * 1e3,
             "algorithmic_bytes_per_launch": bytes_per_launch,
+            "stream_ceiling": ceiling,
         },
         "kernel_ms_per_step": {k: (v[0] / v[1] if v[1] else None) for k, v in tm.items()},
         "cpu_baseline": cpu,
