@@ -28,7 +28,10 @@ constexpr int kBlock = 256;          // threads per block (4 wave64)
 constexpr int kItems = 8;            // particles per thread per tile
 constexpr int kTile = kBlock * kItems;
 constexpr int kWinZ = 64;            // LDS deposit window (nodes in z)
-constexpr int kWinR = 8;             // LDS deposit window (nodes in r)
+constexpr int kWinR = 24;            // LDS deposit window (nodes in r)
+// A tile's window starts this many nodes before its first particle's cell.
+constexpr int kAnchorZ = 4;
+constexpr int kAnchorR = kWinR / 2 - 1;
 
 enum : int { M_PUSH = 1, M_ABSORB = 2, M_DEPOSIT = 4, M_MCC = 8 };
 
@@ -404,8 +407,8 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
             const bool alive = live && !deleted;
             if ((MODE & M_DEPOSIT) && threadIdx.x == 0 && k == 0 && alive && p.Z > 0.0 && p.R >= 0.0 &&
                 p.Z < 1e9 && p.R < 1e9) {
-                na0 = (int)ceil(p.Z) - 4;
-                na1 = (int)ceil(p.R) - 3;
+                na0 = (int)ceil(p.Z) - kAnchorZ;
+                na1 = (int)ceil(p.R) - kAnchorR;
             }
 
             // Null-collision selection (SelectParticleVW, R/src/mc/MCCPublic.F90:123-155):
@@ -1299,6 +1302,22 @@ __global__ void scan_add_kernel(unsigned long long *__restrict__ out, const unsi
     if (i < n) out[i] += add[blockIdx.x];
 }
 
+// Deposit-window anchors of a species' tiles in the order a sort left: each
+// tile's window is placed by its first particle, as the particle pass does
+// for the pass after it.  anc points at the species' first tile.
+__global__ void anchor_kernel(const DevCounters *__restrict__ ctr, int s, const double *__restrict__ Z,
+                              const double *__restrict__ R, int *__restrict__ anc, long long max_tiles)
+{
+    const long long n = (long long)ctr->n[s];
+    const long long nt = min((n + kTile - 1) / kTile, max_tiles);
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const double z = Z[t * kTile], r = R[t * kTile];
+    if (is_dead(z) || !(z > 0.0) || !(r >= 0.0) || z >= 1e9 || r >= 1e9) return;
+    anc[2 * t] = (int)ceil(z) - kAnchorZ;
+    anc[2 * t + 1] = (int)ceil(r) - kAnchorR;
+}
+
 __global__ void set_count_kernel(DevCounters *ctr, int s, const unsigned long long *offs,
                                  const unsigned long long *hist, size_t ncell)
 {
@@ -2040,6 +2059,14 @@ int ipm_sort(ipm_ctx *c, int s)
             hipLaunchKernelGGL(sort_tile_kernel<true>, dim3(pblk), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr, sp,
                                end_bit, c->cursor, c->offs, c->d_dst);
         hipLaunchKernelGGL(set_count_kernel, dim3(1), dim3(64), 0, c->stream, c->d_ctr, sp, c->offs, c->hist, ncell);
+        if (c->tanc && c->species_set[sp]) {
+            const long long max_tiles = ((long long)c->h.cap[sp] + kTile - 1) / kTile;
+            if (c->h.q_cbase[sp] + max_tiles <= c->q_ntiles) {
+                const unsigned ablk = (unsigned)((max_tiles + kBlock - 1) / kBlock);
+                hipLaunchKernelGGL(anchor_kernel, dim3(ablk), dim3(kBlock), 0, c->stream, c->d_ctr, sp, dst[0], dst[1],
+                                   c->tanc + 2 * c->h.q_cbase[sp], max_tiles);
+            }
+        }
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(c->stream));
         c->cur[sp] = other;
