@@ -1,0 +1,69 @@
+"""Cell sort of a drifted population (ipm_sort; the reference has no
+counterpart: its bundles keep insertion order, and neither the deposit nor
+the collisions depend on slot order).
+
+c1 (65x129 nodes, 2.46 M particles, e / CF3+ / F-) after 30 fused steps, so
+that tiles hold many runs of different cells and neighbouring tiles share
+cells:
+
+* each species keeps exactly its live particles (multiset of 64-bit hashes
+  of the 8 fields) and its live count;
+* slots are in nondecreasing cell order;
+* the deposit after the sort equals the deposit before it to 1e-12 of its
+  magnitude (atomic summation order);
+* the next 5 fused steps from the sorted state give the same particle
+  multisets as the same steps from an unsorted twin (collision draws follow
+  the particle's state, not its slot).
+"""
+import numpy as np
+import pytest
+
+import ipm2d
+from ipm2d import workload as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _hashes(P):
+    b = np.ascontiguousarray(P).view(np.uint64)
+    h = np.full(b.shape[1], 0x9E3779B97F4A7C15, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        for f in range(8):
+            h ^= b[f] * np.uint64(0xBF58476D1CE4E5B9 + 2 * f)
+            h = (h << np.uint64(27)) | (h >> np.uint64(37))
+            h *= np.uint64(0x94D049BB133111EB)
+    return np.sort(h)
+
+
+def test_sort_after_drift():
+    cfg = W.make_config("c1")
+    geom = W.make_geometry(cfg)
+    tables = W.build_tables(cfg)
+    rng = np.random.default_rng(cfg.seed)
+    parts = [W.load_species(cfg, geom, s, rng) for s in range(cfg.ns)]
+    a, _, _, _ = ipm2d.from_config(cfg, geom=geom, tables=tables, particles=parts, capacity_factor=1.5)
+    b, _, _, _ = ipm2d.from_config(cfg, geom=geom, tables=tables, particles=parts, capacity_factor=1.5)
+    for _ in range(30):
+        a.one_step()
+        b.one_step()
+    before = [a.download(s) for s in range(cfg.ns)]
+    for s in range(cfg.ns):
+        np.testing.assert_array_equal(_hashes(before[s]), _hashes(b.download(s)), err_msg=f"twin {s}")
+    r0, _ = a.p2ces_all()
+    a.sort(-1)
+    for s in range(cfg.ns):
+        after = a.download(s)
+        assert after.shape[1] == before[s].shape[1] == a.count(s)
+        np.testing.assert_array_equal(_hashes(after), _hashes(before[s]), err_msg=f"species {s}")
+        cell = (np.ceil(after[0]) - 1) + (np.maximum(np.ceil(after[1]), 1) - 1) * (cfg.nz - 1)
+        assert np.all(np.diff(cell) >= 0), f"species {s} not in cell order"
+    r1, _ = a.p2ces_all()
+    np.testing.assert_allclose(r1, r0, rtol=1e-12, atol=1e-12 * np.abs(r0).max())
+    for _ in range(5):
+        a.one_step()
+        b.one_step()
+    for s in range(cfg.ns):
+        np.testing.assert_array_equal(_hashes(a.download(s)), _hashes(b.download(s)),
+                                      err_msg=f"species {s} after the sort")
+    a.close()
+    b.close()
