@@ -228,7 +228,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU path is a design "
-              f"described in DESIGN.md and is not built in this repository.", file=sys.stderr)
+              f"(particles sharded per species, one all-reduce of Rho and Chi per step) and is not built here.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
