@@ -38,7 +38,14 @@ Derived quantities follow the reference:
   order, vacuum 0, a dielectric's cells take its permittivity, and each
   metal cell marks its four nodes with its index, later cells in i-then-j
   order overwriting earlier ones (init.F90:316-398); node volumes follow
-  init.F90:400-424 from the file's volume fractions.
+  init.F90:400-424 from the file's volume fractions;
+* the species and gases of an lxcat gas set (GasInitPegasus,
+  R/src/mc/ReactionOnePegasus.f90:521-600): the electron first, then each
+  gas's species in its file's order, masses in amu times AtomicMass, charge
+  numbers times ElectronCharge, gas density (p mTorrtoPa) / (kB T) times the
+  gas's ratio; and each (species, gas) collision table from
+  input/gas/lxcat/<gas>/<species><gas>.txt (MCCBundleInit,
+  R/src/mc/MCCInit.F90:15-120).
 """
 import dataclasses
 import json
@@ -510,3 +517,125 @@ def geometry_from_files(run, materials=None):
                       vedge_type=np.asarray(g["vedge_type"], dtype=np.int32), volume_factor=factor,
                       volume=vol, inv_volume=inv, cell_load=np.asarray(g["load"], dtype=np.int32),
                       cell_epsilon=eps, point_type=pt)
+
+
+# ------------------------------------------------------------ gases and species
+@dataclasses.dataclass
+class SpeciesDef:
+    """A species as SpecyOnePhysics2SpecyOne fills it (R/src/base/SpecyOne.F90:167-178)."""
+    name: str
+    natom: int
+    charge: float            # C
+    mass: float              # kg
+    radius: float
+    gas: int                 # 0 for the electron, else the 1-based gas
+
+
+@dataclasses.dataclass
+class GasDef:
+    """A gas as GasPhysics2Gas and GasInitPegasus fill it (SpecyOne.F90:181-196;
+    ReactionOnePegasus.f90:565-585)."""
+    name: str
+    mc_model: int
+    natom: int
+    ns: int
+    mass: float              # kg
+    radius: float
+    beta_max: float
+    temperature: float
+    density: float           # m^-3
+    index_start: int         # species of this gas are index_start + 1 .. index_start + ns
+
+
+_NML_ITEM = re.compile(r"GP%(\w+)(?:\((\d+)\))?\s*=", re.IGNORECASE)
+
+
+def read_gas_physics(path):
+    """The GasPhysicsfile namelist of input/gas/lxcat/<gas>/<gas>.txt, as
+    LoadGasPhysicalLxCat reads it (ReactionOnePegasus.f90:497-517): a dict of
+    GP%Name, MCModel, NAtom, Ns, Mass [amu], Radius, BetaMax, and 'SP', the
+    species lines GP%SP(j) = name, natom, charge number, mass [amu], radius."""
+    from .restart import _value
+    with open(path, encoding="latin-1") as f:
+        lines = [ln.split("!", 1)[0] for ln in f.read().splitlines()]
+    text = " ".join(lines)
+    a = text.lower().find("&gasphysicsfile")
+    if a < 0:
+        raise ConfigError(f"{path}: no &GasPhysicsfile group")
+    text = text[a + len("&gasphysicsfile"):]
+    end = text.find("/")                     # the group ends at the first '/'
+    body = text[:end] if end >= 0 else text
+    items = list(_NML_ITEM.finditer(body))
+    out = {"SP": {}}
+    for k, m in enumerate(items):
+        stop = items[k + 1].start() if k + 1 < len(items) else len(body)
+        toks = [t.strip() for t in body[m.end():stop].split(",")]
+        toks = [t for t in toks if t]
+        key = m.group(1).lower()
+        if m.group(2) is not None:
+            if key != "sp" or len(toks) < 5:
+                raise ConfigError(f"{path}: bad GP%{m.group(1)}({m.group(2)}) entry")
+            out["SP"][int(m.group(2))] = (toks[0], int(toks[1]), int(toks[2]), _value(toks[3]), _value(toks[4]))
+        elif key == "name":
+            out["Name"] = toks[0]
+        elif key in ("mcmodel", "natom", "ns"):
+            out[{"mcmodel": "MCModel", "natom": "NAtom", "ns": "Ns"}[key]] = int(toks[0])
+        elif key in ("mass", "radius", "betamax"):
+            out[{"mass": "Mass", "radius": "Radius", "betamax": "BetaMax"}[key]] = _value(toks[0])
+    for k in ("Name", "NAtom", "Ns", "Mass"):
+        if k not in out:
+            raise ConfigError(f"{path}: GP%{k} missing")
+    if sorted(out["SP"]) != list(range(1, out["Ns"] + 1)):
+        raise ConfigError(f"{path}: GP%SP(1..{out['Ns']}) expected, found {sorted(out['SP'])}")
+    return out
+
+
+def _input_root(run):
+    return os.path.dirname(os.path.abspath(run.source))
+
+
+def gases_and_species(run, root=None):
+    """(species, gases) of the run's gas list, GasInitPegasus order."""
+    from . import xsec
+    if run.gas.collision_section_type != 1:
+        raise NotImplementedError("only lxcat gas sets (collision_section_type 1) are read")
+    root = _input_root(run) if root is None else root
+    species = [SpeciesDef("Electron", 1, float(-1) * xsec.ELECTRON_CHARGE,
+                          (xsec.ELECTRON_MASS / xsec.ATOMIC_MASS) * xsec.ATOMIC_MASS, 0.0, 0)]
+    gases = []
+    index = 0
+    for g, (name, ratio) in enumerate(zip(run.gas.names, run.gas.ratios), start=1):
+        gp = read_gas_physics(os.path.join(root, "input", "gas", "lxcat", name, f"{name}.txt"))
+        gases.append(GasDef(gp["Name"], gp.get("MCModel", 0), gp["NAtom"], gp["Ns"], gp["Mass"] * xsec.ATOMIC_MASS,
+                            gp.get("Radius", 0.0), gp.get("BetaMax", 0.0), run.gas.temperature,
+                            xsec.gas_density(run.gas.pressure, run.gas.temperature, ratio), index))
+        for j in range(1, gp["Ns"] + 1):
+            nm, natom, z, m_amu, radius = gp["SP"][j]
+            species.append(SpeciesDef(nm, natom, float(z) * xsec.ELECTRON_CHARGE, m_amu * xsec.ATOMIC_MASS,
+                                      radius, g))
+            index += 1
+    return species, gases
+
+
+def mcc_tables(run, species, gases, root=None):
+    """{(s, g): MCCBundle} for every species s and 0-based gas g, from
+    input/gas/lxcat/<gas>/<species><gas>.txt.  The reference falls back to a
+    hard-sphere table for a species of another gas or without a file
+    (MCCInit.F90:107-116); those cases are refused here."""
+    from . import xsec
+    root = _input_root(run) if root is None else root
+    out = {}
+    for s, sp in enumerate(species):
+        for g, gas in enumerate(gases):
+            if s > 0 and sp.gas != g + 1:
+                raise NotImplementedError(f"species {sp.name} of another gas: the hard-sphere table is not built")
+            path = os.path.join(root, "input", "gas", "lxcat", gas.name, f"{sp.name}{gas.name}.txt")
+            if not os.path.exists(path):
+                raise NotImplementedError(f"{path} missing: the hard-sphere table is not built")
+            with open(path, encoding="latin-1") as f:
+                sf = xsec.parse_sigma_file(f.read())
+            out[(s, g)] = xsec.build_bundle(sf, sp.mass, sp.natom,
+                                            {"mass": gas.mass, "temperature": gas.temperature,
+                                             "density": gas.density, "natom": gas.natom, "beta_max": gas.beta_max},
+                                            run.dt, index_start=gas.index_start)
+    return out

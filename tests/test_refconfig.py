@@ -15,11 +15,20 @@
     positive exactly where the file's volume fractions exceed 1e-6;
   - the base weight times the particles at load equals the initial density
     times the cylinder's volume;
+  - the CF4 gas set gives the electron, F-, CF3+, CF3- in that order (the
+    configuration's recombination list, written against the same indices,
+    pairs opposite charges and gives its temperature-dependent reaction to
+    the electron and a positive ion); the four collision tables build, the
+    electron's normalised cumulative table peaks at exactly 1, its
+    selection ratio is 1 - exp(-nu_max dt), and the reactive ratio, which
+    depends on the reduced mass only, is equal for CF3+ and CF3-;
 * json-fortran's reading rules on synthetic configurations: type
   conversions, 1-based paths, first of duplicate keys, keys read only where
   the reference reads them, missing keys, the diagnostic-period branches;
 * the metal node marking against the reference's loop (i outer, j inner,
-  later cells overwrite) on small random maps where metals touch.
+  later cells overwrite) on small random maps where metals touch;
+* the GasPhysicsfile namelist: comments, d/D exponents, a missing species
+  line.
 """
 import copy
 import json
@@ -249,3 +258,58 @@ def test_metal_marking_matches_the_loop_order(rng):
     cm = np.array([[1, 2], [3, 1]], dtype=np.int32)
     got = G.metal_point_types(np.zeros((3, 3), np.int32), cm, cm > 0)
     assert got[1, 1] == 1 and got[0, 1] == 2 and got[1, 0] == 3
+
+
+# ------------------------------------------------------------ gases, species, tables
+@needs_ref
+def test_reference_gas_set_species_and_tables():
+    from ipm2d import xsec
+    run = RC.load(REF)
+    species, gases = RC.gases_and_species(run)
+    assert [sp.name for sp in species] == ["Electron", "F-", "CF3+", "CF3-"]
+    assert [round(sp.charge / xsec.ELECTRON_CHARGE) for sp in species] == [-1, -1, 1, -1]
+    # the configuration's recombination list pairs opposite charges, and its
+    # temperature-dependent reaction (type 1) is the electron with a positive ion
+    for a, b, kind in run.recombination.reactions:
+        assert species[a].charge * species[b].charge < 0
+        assert kind == 0 or (a == 0 and species[b].charge > 0)
+    g = gases[0]
+    assert (g.name, g.natom, g.ns, g.index_start) == ("CF4", 5, 3, 0)
+    assert g.density == (100.0 * 0.13332) / (1.3807e-23 * 300.0)
+    assert g.mass == 88.0043128 * 1.660539040e-27
+    assert species[0].mass == (9.1095e-31 / 1.660539040e-27) * 1.660539040e-27
+    tables = RC.mcc_tables(run, species, gases)
+    assert sorted(tables) == [(0, 0), (1, 0), (2, 0), (3, 0)]
+    e = tables[(0, 0)]
+    assert (e.model, e.nreaction, e.nsigma) == (1, 8, 10001)
+    assert e.prob.max() == 1.0 and np.all(np.diff(e.prob, axis=1) >= 0)
+    assert e.collision_ratio == 1.0 - math.exp(-e.sigma_max * run.dt)
+    for s in (1, 2, 3):
+        b = tables[(s, 0)]
+        assert b.model == 3 and b.nsigma == 1001
+        assert np.all((b.resultant >= 0) & (b.resultant < len(species)))
+    # the reactive selection ratio depends on the reduced mass only: equal for CF3+ and CF3-
+    assert tables[(2, 0)].collision_ratio == tables[(3, 0)].collision_ratio != tables[(1, 0)].collision_ratio
+
+
+def test_gas_physics_namelist(tmp_path):
+    p = tmp_path / "X.txt"
+    p.write_text(""" &GasPhysicsfile
+ GP%Name      =           Xe,   ! a comment
+ GP%MCModel   =           2,
+ GP%NAtom     =           1,
+ GP%Ns        =           2,
+ GP%Mass      =           131.293,
+ GP%Radius    =           2.16d-10,
+ GP%BetaMax   =           3.d0
+ GP%SP(1)     =           Xe+,1,1,131.293,2.d-10,
+ GP%SP(2)     =           Xe2+,1,2,131.293,1.9D-10,
+ /
+""")
+    gp = RC.read_gas_physics(p)
+    assert (gp["Name"], gp["MCModel"], gp["NAtom"], gp["Ns"]) == ("Xe", 2, 1, 2)
+    assert gp["Mass"] == 131.293 and gp["Radius"] == 2.16e-10 and gp["BetaMax"] == 3.0
+    assert gp["SP"] == {1: ("Xe+", 1, 1, 131.293, 2e-10), 2: ("Xe2+", 1, 2, 131.293, 1.9e-10)}
+    p.write_text(p.read_text().replace(" GP%SP(2)     =           Xe2+,1,2,131.293,1.9D-10,\n", ""))
+    with pytest.raises(RC.ConfigError, match="SP"):
+        RC.read_gas_physics(p)
