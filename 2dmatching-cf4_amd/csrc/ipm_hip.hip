@@ -233,11 +233,15 @@ __device__ __forceinline__ int dpp_shr_i(int x)
 {
     return __builtin_amdgcn_update_dpp(0, x, 0x110 + D, 0xf, 0xf, false);
 }
-__device__ __forceinline__ double readlane_d(double x, int l)
+// DPP row broadcasts: CTRL 0x142 (row_bcast:15) gives rows 1 and 3 the last
+// lane of the row before them, CTRL 0x143 (row_bcast:31) gives rows 2 and 3
+// lane 31; ROWS masks the receiving rows, every other lane receives 0.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_bcast(double x)
 {
     const long long b = __double_as_longlong(x);
-    const int lo = __builtin_amdgcn_readlane((int)b, l);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xf, false);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
@@ -246,6 +250,18 @@ __device__ __forceinline__ void seg_step(double (&v)[4], int &f)
 {
     const double a0 = dpp_shr<D>(v[0]), a1 = dpp_shr<D>(v[1]), a2 = dpp_shr<D>(v[2]), a3 = dpp_shr<D>(v[3]);
     const int fu = dpp_shr_i<D>(f);
+    if (!f) {
+        v[0] = v[0] + a0; v[1] = v[1] + a1; v[2] = v[2] + a2; v[3] = v[3] + a3;
+    }
+    f |= fu;
+}
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void seg_bcast(double (&v)[4], int &f)
+{
+    const double a0 = dpp_bcast<CTRL, ROWS>(v[0]), a1 = dpp_bcast<CTRL, ROWS>(v[1]);
+    const double a2 = dpp_bcast<CTRL, ROWS>(v[2]), a3 = dpp_bcast<CTRL, ROWS>(v[3]);
+    const int fu = __builtin_amdgcn_update_dpp(0, f, CTRL, ROWS, 0xf, false);
     if (!f) {
         v[0] = v[0] + a0; v[1] = v[1] + a1; v[2] = v[2] + a2; v[3] = v[3] + a3;
     }
@@ -266,17 +282,8 @@ __device__ __forceinline__ bool wave_segmented_sum(int key, double (&v)[4])
     seg_step<2>(v, f);
     seg_step<4>(v, f);
     seg_step<8>(v, f);
-#pragma unroll
-    for (int row = 1; row < 4; ++row) {
-        const int src = row * 16 - 1;
-        const double c0 = readlane_d(v[0], src), c1 = readlane_d(v[1], src);
-        const double c2 = readlane_d(v[2], src), c3 = readlane_d(v[3], src);
-        const int cf = __builtin_amdgcn_readlane(f, src);
-        if ((lane >> 4) == row && !f) {
-            v[0] = v[0] + c0; v[1] = v[1] + c1; v[2] = v[2] + c2; v[3] = v[3] + c3;
-        }
-        if ((lane >> 4) == row) f |= cf;
-    }
+    seg_bcast<0x142, 0xa>(v, f);    // rows 1, 3 take the ends of rows 0, 2
+    seg_bcast<0x143, 0xc>(v, f);    // rows 2, 3 take lane 31 (rows 0-1 combined)
     return lane == 63 || knext != key;
 }
 
