@@ -639,3 +639,94 @@ def mcc_tables(run, species, gases, root=None):
                                              "density": gas.density, "natom": gas.natom, "beta_max": gas.beta_max},
                                             run.dt, index_start=gas.index_start)
     return out
+
+
+# ------------------------------------------------------------ particles at load
+C_EXP = 2.71828          # the constant `exp` of R/src/base/Constants.F90
+
+
+def initial_particles(run, species, s, geom, rng, ppg=None):
+    """Species s at load, (8, n) in grid units: the steps of
+    AllInitializationParticleBundle (R/src/pic/particle/ParticleBundle.f90:
+    362-385) with numpy's generator in place of RANDOM_NUMBER.
+
+    Every loadable cell (i, j), 1-based, gets ppg candidates with R uniform in
+    [j-1, j) and Z in [i-1, i) (PositionRandomInitializationParticleOne,
+    ParticleOne.f90:35-48); a candidate is kept when a uniform is below R / j
+    and then gets WQ = n0 / ppg * 2 pi dx^3 j, a speed from
+    VelocityMaxwellianInitializationParticleOne (ParticleOne.f90:75-94: draw
+    R1, R2 until R1^2 <= -e R2 ln R2, v = sqrt(-3 ln R2 kT / m)), an isotropic
+    direction (ParticleOne.f90:107-121; the conversion to cylindrical
+    components is the identity there), velocities times dt/dx and
+    Az = Ar = 0.  The electron takes plasma.init_electron_temperature, every
+    other species gas.temperature (GasInitPegasus)."""
+    from . import xsec
+    ppg = run.plasma.ppg if ppg is None else int(ppg)
+    sp = species[s]
+    temperature = run.plasma.init_electron_temperature if s == 0 else run.gas.temperature
+    dx = run.dz
+    load = np.asarray(geom.cell_load).astype(bool)
+    ci, cj = np.nonzero(load)                      # 0-based cell (i-1, j-1)
+    ci = np.repeat(ci, ppg).astype(np.float64)
+    cj = np.repeat(cj, ppg).astype(np.float64)
+    m = ci.size
+    r = cj + rng.random(m)
+    z = ci + rng.random(m)
+    keep = rng.random(m) < r / (cj + 1.0)
+    z, r, j = z[keep], r[keep], cj[keep] + 1.0
+    n = z.size
+    # rejection sampling of the speed, all candidates at once
+    r2 = np.empty(n)
+    todo = np.arange(n)
+    while todo.size:
+        a = rng.random(todo.size)
+        b = rng.random(todo.size)
+        ok = a * a <= -C_EXP * b * np.log(b)
+        r2[todo[ok]] = b[ok]
+        todo = todo[~ok]
+    beta = 1.0 / (xsec.KB * temperature)
+    v = np.sqrt(-3.0 * np.log(r2) / beta / sp.mass)
+    cos_t = 1.0 - 2.0 * rng.random(n)
+    sin_t = np.sqrt(1.0 - cos_t * cos_t)
+    phi = 2.0 * G.PI * rng.random(n)
+    vres = 1.0 / (dx / run.dt)
+    out = np.empty((8, n))
+    out[0] = z
+    out[1] = r
+    out[2] = v * cos_t * vres
+    out[3] = v * sin_t * np.cos(phi) * vres
+    out[4] = v * sin_t * np.sin(phi) * vres
+    out[5] = 0.0
+    out[6] = 0.0
+    out[7] = run.plasma.init_density / float(ppg) * 2.0 * G.PI * dx * dx * dx * j
+    return out
+
+
+def build_hotpath(run, root=None, seed=0, capacity=None, hotpath_cls=None):
+    """A HotPath set up from the reference's inputs: the species of the gas
+    set (charge, mass, array size npar_max unless ``capacity`` is given), the
+    geometry rebuilt from its files, the collision tables, the particles at
+    load (``initial_particles``, numpy's generator seeded with ``seed``) and
+    the per-particle random streams.  The fields come from the caller's field
+    solve (``set_fields`` / ``set_potential``).  Returns (hp, species, gases,
+    geometry)."""
+    from .hotpath import HotPath
+    cls = HotPath if hotpath_cls is None else hotpath_cls
+    species, gases = gases_and_species(run, root)
+    tables = mcc_tables(run, species, gases, root)
+    geom = geometry_from_files(run)
+    cap = run.bundle_sizes()[1] if capacity is None else int(capacity)
+    rng = np.random.default_rng(seed)
+    parts = [initial_particles(run, species, s, geom, rng) for s in range(len(species))]
+    hp = cls(run.nz, run.nr, run.dz, run.dt, len(species), len(gases))
+    for s, sp in enumerate(species):
+        if parts[s].shape[1] > cap:
+            raise ValueError(f"species {sp.name}: {parts[s].shape[1]} particles at load exceed capacity {cap}")
+        hp.set_species(s, sp.charge, sp.mass, cap)
+    hp.set_geometry(geom)
+    for (s, g), b in sorted(tables.items()):
+        hp.set_mcc(s, g, b)
+    for s in range(len(species)):
+        hp.upload(s, parts[s])
+    hp.set_rng(seed, 0)
+    return hp, species, gases, geom

@@ -28,7 +28,13 @@
 * the metal node marking against the reference's loop (i outer, j inner,
   later cells overwrite) on small random maps where metals touch;
 * the GasPhysicsfile namelist: comments, d/D exponents, a missing species
-  line.
+  line;
+* particles at load (AllInitializationParticleBundle's steps): only loadable
+  cells, the R / j acceptance (count per row), a uniform density (total
+  weight = n0 x loadable volume), and the speed sampler's closed form: its
+  accepted R2 make the energy Maxwell-Boltzmann, mean 3/2 kT, isotropic;
+* build_hotpath's calls on a recording stand-in, from the reference's own
+  inputs.
 """
 import copy
 import json
@@ -313,3 +319,83 @@ def test_gas_physics_namelist(tmp_path):
     p.write_text(p.read_text().replace(" GP%SP(2)     =           Xe2+,1,2,131.293,1.9D-10,\n", ""))
     with pytest.raises(RC.ConfigError, match="SP"):
         RC.read_gas_physics(p)
+
+
+# ------------------------------------------------------------ particles at load
+def test_initial_particles_follow_the_load_rules(tmp_path):
+    from ipm2d import xsec
+    d = copy.deepcopy(BASE)
+    d["plasma"]["ppg"] = 40
+    d["plasma"]["init_electron_temperature"] = 30000.0
+    run = RC.load(_write(tmp_path, d), nz=33, nr=17)
+    cm = np.zeros((32, 16), dtype=np.int32)
+    cm[0, :] = 1
+    cm[-1, :] = 2
+    geom = G.build_geometry(33, 17, run.dz, cm, nmetal=2, ndiel=0)
+    species = [RC.SpeciesDef("Electron", 1, -xsec.ELECTRON_CHARGE, xsec.ELECTRON_MASS, 0.0, 0),
+               RC.SpeciesDef("CF3+", 4, xsec.ELECTRON_CHARGE, 69.0059096 * xsec.ATOMIC_MASS, 2e-10, 1)]
+    gen = np.random.default_rng(5)
+    for s, temp in ((0, 30000.0), (1, 300.0)):
+        P = RC.initial_particles(run, species, s, geom, gen)
+        n = P.shape[1]
+        iz, ir = np.floor(P[0]).astype(int), np.floor(P[1]).astype(int)
+        assert np.all(geom.cell_load[iz, ir] == 1)
+        # kept with probability R / j: the expected count of row j is ppg (j - 1/2) / j per cell
+        cells_per_row = geom.cell_load.sum(axis=0)
+        j = np.arange(1, 17)
+        p = (j - 0.5) / j
+        mean = (cells_per_row * 40 * p).sum()
+        sd = math.sqrt((cells_per_row * 40 * p * (1 - p)).sum())
+        assert abs(n - mean) < 5 * sd
+        # the weights make the density uniform: total weight = n0 x loadable volume
+        vol = (cells_per_row * math.pi * run.dz ** 3 * (2 * j - 1)).sum()
+        assert P[7].sum() == pytest.approx(run.plasma.init_density * vol, rel=5 * sd / mean)
+        # the speed sampler gives a Maxwell-Boltzmann energy distribution: mean 3/2 kT,
+        # isotropic components, kT/m each
+        vf = run.dz / run.dt
+        v2 = (P[2:5] * vf) ** 2
+        kt_m = xsec.KB * temp / species[s].mass
+        e_mean = 0.5 * species[s].mass * v2.sum(axis=0).mean() / (xsec.KB * temp)
+        assert e_mean == pytest.approx(1.5, rel=5 * math.sqrt(2.0 / 3.0) / math.sqrt(n))
+        for c in range(3):
+            assert v2[c].mean() == pytest.approx(kt_m, rel=0.03)
+            assert abs((P[2 + c] * vf).mean()) < 5 * math.sqrt(kt_m / n)
+        assert np.all(P[5:7] == 0.0)
+
+
+class _Recorder:
+    """Stands in for HotPath and records the set-up calls."""
+
+    def __init__(self, nz, nr, dx, dt, ns, ng):
+        self.args = (nz, nr, dx, dt, ns, ng)
+        self.species, self.mcc, self.uploads = {}, {}, {}
+        self.geom = self.rng = None
+
+    def set_species(self, s, q, m, cap):
+        self.species[s] = (q, m, cap)
+
+    def set_geometry(self, g):
+        self.geom = g
+
+    def set_mcc(self, s, g, b):
+        self.mcc[(s, g)] = b
+
+    def upload(self, s, P):
+        self.uploads[s] = P.shape
+
+    def set_rng(self, seed, step):
+        self.rng = (seed, step)
+
+
+@needs_ref
+def test_build_hotpath_from_the_reference_inputs():
+    run = RC.load(REF)
+    hp, species, gases, geom = RC.build_hotpath(run, seed=9, capacity=4_000_000, hotpath_cls=_Recorder)
+    assert hp.args == (201, 201, run.dz, run.dt, 4, 1)
+    assert sorted(hp.species) == [0, 1, 2, 3] and all(v[2] == 4_000_000 for v in hp.species.values())
+    assert [hp.species[s][:2] for s in range(4)] == [(sp.charge, sp.mass) for sp in species]
+    assert sorted(hp.mcc) == [(0, 0), (1, 0), (2, 0), (3, 0)]
+    assert hp.geom is geom and hp.rng == (9, 0)
+    loadable = int(geom.cell_load.sum())
+    for s in range(4):
+        assert hp.uploads[s][0] == 8 and 0.4 * 50 * loadable < hp.uploads[s][1] < 50 * loadable
