@@ -484,19 +484,16 @@ def _ohdr(msgs):
     return struct.pack("<BBHII", 1, 0, len(msgs), 1, len(body)) + b"\0" * 4 + body
 
 
-def write_bundle(path, dump):
-    """DumpParticleBundleHDF5, ParticleBundle.f90:117-148: root attributes
-    NPar, size (32-bit integers), Charge, Mass, Weight, XFactor, VFactor
-    (doubles), LXScaled, LVScaled (32-bit integers, 1 = true) and datasets
-    Z R Vz Vr Vt Az Ar WQ of `size` doubles, in a superblock-0 file with one
-    symbol-table root group."""
-    po = np.ascontiguousarray(dump.po, dtype="<f8")
-    if po.shape != (8, dump.size):
-        raise ValueError("po must be (8, size)")
+def _write_flat(path, datasets, attrs):
+    """A superblock-0 file whose root group (one symbol table node, at most
+    eight links) holds contiguous datasets (name -> C-order array) and
+    scalar attributes (name, int | bool | float)."""
+    if len(datasets) > 8:
+        raise ValueError("at most eight datasets in the root group")
     w = _Writer()
     sb = w.alloc(96)                                   # superblock with the root symbol table entry
     # local heap of the root group: names of the datasets
-    names = list(restart.FIELDS)
+    names = list(datasets)
     heap_data = bytearray(b"\0" * 8)                   # offset 0 holds the empty name
     name_off = {}
     for n in names:
@@ -510,13 +507,15 @@ def write_bundle(path, dump):
     w.put(heap_seg, bytes(heap_data))
     # datasets: object header + contiguous data
     ds_addr = {}
-    for k, n in enumerate(names):
-        data = po[k].tobytes()
+    for n in names:
+        arr = np.ascontiguousarray(datasets[n])
+        arr = arr.astype(arr.dtype.newbyteorder("<"), copy=False)
+        data = arr.tobytes()
         data_at = w.alloc(len(data))
         w.put(data_at, data)
         fill = _msg(0x05, bytes([2, 2, 2, 0]))        # fill value: late allocation, written if set, undefined
         layout = _msg(0x08, bytes([3, 1]) + struct.pack("<QQ", data_at, len(data)))
-        oh = _ohdr([_msg(0x01, _ds_msg((dump.size,))), _msg(0x03, _dt_msg("<f8"), flags=1), fill, layout])
+        oh = _ohdr([_msg(0x01, _ds_msg(arr.shape)), _msg(0x03, _dt_msg(arr.dtype), flags=1), fill, layout])
         at = w.alloc(len(oh))
         w.put(at, oh)
         ds_addr[n] = at
@@ -532,12 +531,7 @@ def write_bundle(path, dump):
     w.put(tree, b"TREE" + bytes([0, 0]) + struct.pack("<HQQ", 1, UNDEF, UNDEF) +
           struct.pack("<QQQ", 0, snod, name_off[order[-1]]))
     # root group object header: symbol table message and the attributes
-    a = dump
-    attrs = [_attr_msg("NPar", int(a.npar)), _attr_msg("size", int(a.size)), _attr_msg("Charge", float(a.charge)),
-             _attr_msg("Mass", float(a.mass)), _attr_msg("Weight", float(a.weight)),
-             _attr_msg("XFactor", float(a.xfactor)), _attr_msg("VFactor", float(a.vfactor)),
-             _attr_msg("LXScaled", bool(a.lxscaled)), _attr_msg("LVScaled", bool(a.lvscaled))]
-    root = _ohdr([_msg(0x11, struct.pack("<QQ", tree, heap))] + attrs)
+    root = _ohdr([_msg(0x11, struct.pack("<QQ", tree, heap))] + [_attr_msg(k, v) for k, v in attrs])
     root_at = w.alloc(len(root))
     w.put(root_at, root)
     eof = len(w.buf)
@@ -547,3 +541,40 @@ def write_bundle(path, dump):
     w.put(sb, sbb)
     with open(path, "wb") as f:
         f.write(bytes(w.buf))
+
+
+def write_bundle(path, dump):
+    """DumpParticleBundleHDF5, ParticleBundle.f90:117-148: root attributes
+    NPar, size (32-bit integers), Charge, Mass, Weight, XFactor, VFactor
+    (doubles), LXScaled, LVScaled (32-bit integers, 1 = true) and datasets
+    Z R Vz Vr Vt Az Ar WQ of `size` doubles, in a superblock-0 file with one
+    symbol-table root group."""
+    po = np.ascontiguousarray(dump.po, dtype="<f8")
+    if po.shape != (8, dump.size):
+        raise ValueError("po must be (8, size)")
+    a = dump
+    attrs = [("NPar", int(a.npar)), ("size", int(a.size)), ("Charge", float(a.charge)), ("Mass", float(a.mass)),
+             ("Weight", float(a.weight)), ("XFactor", float(a.xfactor)), ("VFactor", float(a.vfactor)),
+             ("LXScaled", bool(a.lxscaled)), ("LVScaled", bool(a.lvscaled))]
+    _write_flat(path, {n: po[k] for k, n in enumerate(restart.FIELDS)}, attrs)
+
+
+_GEOM_FIELDS = (("cell_material", "<i4", -1, -1), ("epsilon_r", "<f8", -1, -1), ("load", "<i4", -1, -1),
+                ("point_type", "<i4", 0, 0), ("volume", "<f8", 0, 0), ("hedge_type", "<i4", 0, -1),
+                ("vedge_type", "<i4", -1, 0))
+
+
+def write_geometry(path, geom):
+    """A geometry input as the reference reads it (R/src/base/init.F90:105-310,
+    written there by R/script/geom/geometry.py:721-738): root attributes Nz,
+    Nr and the datasets of ``read_geometry``, given as Fortran-indexed arrays
+    (cell_material holds 0-based indices into the materials list) and stored
+    with their dimensions reversed, as a Fortran reader expects."""
+    nz, nr = int(geom["Nz"]), int(geom["Nr"])
+    ds = {}
+    for name, dt, dz, dr in _GEOM_FIELDS:
+        a = np.asarray(geom[name])
+        if a.shape != (nz + dz, nr + dr):
+            raise ValueError(f"{name} is {a.shape}, expected {(nz + dz, nr + dr)}")
+        ds[name] = np.ascontiguousarray(a.T).astype(dt)
+    _write_flat(path, ds, [("Nz", nz), ("Nr", nr)])

@@ -1,0 +1,126 @@
+"""From a reference-format input folder to steps on the GPU.
+
+The folder is written here in the reference's formats: config.json (the keys
+R/src/base/init.F90 reads), input/geom.h5 (the geometry datasets and Nz, Nr
+attributes, written by ipm2d.h5lite), input/geom.json (the materials list)
+and the lxcat gas set input/gas/lxcat/CF4/ (GasPhysicsfile namelist and one
+cross-section namelist per species).  The geometry is a 65 x 33 node
+capacitive cell: top and bottom electrodes, a metal outer wall and a
+dielectric ring.
+
+``refconfig.build_hotpath`` sets up two HotPaths from it with the same seed.
+Five steps launched kernel by kernel on one and replayed as a captured graph
+on the other leave the same particles (multisets of 64-bit hashes, bit for
+bit) and deposits equal to 1e-12 of their magnitude, and the metals collect
+charge.
+"""
+import json
+
+import numpy as np
+import pytest
+
+from ipm2d import geometry as G
+from ipm2d import h5lite, refconfig as RC, xsec
+
+pytestmark = pytest.mark.gpu
+
+NZ, NR = 65, 33
+
+
+def _hashes(P):
+    b = np.ascontiguousarray(P).view(np.uint64)
+    h = np.full(b.shape[1], 0x9E3779B97F4A7C15, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        for f in range(8):
+            h ^= b[f] * np.uint64(0xBF58476D1CE4E5B9 + 2 * f)
+            h = (h << np.uint64(27)) | (h >> np.uint64(37))
+            h *= np.uint64(0x94D049BB133111EB)
+    return np.sort(h)
+
+
+def _write_inputs(root):
+    (root / "input" / "gas" / "lxcat" / "CF4").mkdir(parents=True)
+    cm = np.zeros((NZ - 1, NR - 1), dtype=np.int32)      # indices into the materials list
+    cm[:2, :] = 1
+    cm[-2:, :] = 2
+    cm[:, -1] = 3
+    cm[24:40, -3:-1] = 4
+    types = np.array([0, 1, 1, 1, 2])
+    ctype = types[cm]
+    he, ve = G._edge_types(ctype)
+    metal = ctype == G.MAT_METAL
+    geom = {"Nz": NZ, "Nr": NR, "cell_material": cm, "epsilon_r": np.where(cm == 4, 2.1, 1.0),
+            "load": (ctype == 0).astype(np.int32),
+            "point_type": G.metal_point_types(np.zeros((NZ, NR), np.int32), metal.astype(np.int32), metal),
+            "volume": G._volume_factor(NZ, NR, ctype == 0), "hedge_type": he, "vedge_type": ve}
+    h5lite.write_geometry(root / "input" / "geom.h5", geom)
+    mats = [{"type": 0, "name": "vacuum"}, {"type": 1, "name": "top electrode"},
+            {"type": 1, "name": "bottom electrode"}, {"type": 1, "name": "wall"},
+            {"type": 2, "name": "ring", "permittivity": 2.1, "permeability": 1.0, "conductivity": 1.0}]
+    (root / "input" / "geom.json").write_text(json.dumps({"material_amount": len(mats), "materials": mats}))
+    gas = root / "input" / "gas" / "lxcat" / "CF4"
+    (gas / "CF4.txt").write_text(""" &GasPhysicsfile
+ GP%Name      =           CF4,
+ GP%MCModel   =           2,
+ GP%NAtom     =           5,
+ GP%Ns        =           3,
+ GP%Mass      =           88.0043128,
+ GP%Radius    =           1.548d-10,
+ GP%BetaMax   =           3.d0
+ GP%SP(1)     =           F-,1,-1,18.9984030,1.35d-10,
+ GP%SP(2)     =           CF3+,4,1,69.0059096,2.d-10,
+ GP%SP(3)     =           CF3-,4,-1,69.0059096,2.d-10,
+ /
+""")
+    R = xsec.Reaction
+    (gas / "ElectronCF4.txt").write_text(
+        xsec.write_sigma_file(1, xsec.synthetic_electron_reactions(ion_product=2, att_product=1)))
+    (gas / "F-CF4.txt").write_text(xsec.write_sigma_file(3, [R(0, 101, 1, 0.0), R(0, 113, 0, 3.521),
+                                                             R(0, 111, 1, 5.621)]))
+    (gas / "CF3+CF4.txt").write_text(xsec.write_sigma_file(3, [R(0, 101, 2, 0.0), R(0, 111, 2, 5.621),
+                                                               R(0, 100, 0, 5.843)]))
+    (gas / "CF3-CF4.txt").write_text(xsec.write_sigma_file(3, [R(0, 101, 3, 0.0), R(0, 113, 0, 1.871),
+                                                               R(0, 112, 1, 1.927)]))
+    cfg = {
+        "control": {"nrun": 10, "dt": 5e-11, "is_restart": False, "pic_type": 1},
+        "diag": {"diag_type": 1, "diag_cycles": 1, "diag_preiod": 20, "diag_freq": 0.0, "diag_Nt": 5},
+        "grid": {"z_length": 0.032, "deps_in_z": 1, "deps_in_r": 1, "deps_type": 0,
+                 "materials": "./input/geom.json", "geometry": "./input/geom.h5"},
+        "plasma": {"init_density": 1e16, "init_electron_temperature": 60000.0, "ppg": 12, "alpha": 0.01,
+                   "is_open_split_merge": False, "paritlc_upper_limit": 1.01, "paritlc_lower_limit": 0.99,
+                   "min_mp_to_split_k": 0.1, "merge_limit": 0.34, "load_type": "1",
+                   "particle_number_max_ratio": 10.0, "particle_limit_max_velocity": 10.0},
+        "gas": {"ng": 1, "pressure": 300, "temperature": 300, "name": ["CF4"], "ratio": [1.0],
+                "collision_section_type": 1, "is_open_recombination": False},
+        "circuit": {"circuits_number": 1, "circuits_list": [{"type": 1, "metal_index": 2, "name": "ground"}]},
+    }
+    (root / "config.json").write_text(json.dumps(cfg))
+    return root / "config.json"
+
+
+def test_reference_inputs_to_gpu_steps(tmp_path):
+    run = RC.load(_write_inputs(tmp_path))
+    assert (run.nz, run.nr, run.dt) == (NZ, NR, 5e-11)
+    a, species, gases, geom = RC.build_hotpath(run, seed=3, capacity=60000)
+    b, _, _, _ = RC.build_hotpath(run, seed=3, capacity=60000)
+    assert [sp.name for sp in species] == ["Electron", "F-", "CF3+", "CF3-"]
+    assert (geom.nmetal, geom.ndiel) == (3, 1)
+    n0 = [a.count(s) for s in range(4)]
+    assert all(n > 10000 for n in n0)
+    # 150 V on the top electrode falling linearly to the grounded bottom one
+    z = np.arange(NZ)[:, None] * np.ones((1, NR))
+    phi = 150.0 * (1.0 - z / (NZ - 1))
+    for hp in (a, b):
+        hp.set_potential(phi)
+    for _ in range(5):
+        a.one_step()
+        b.one_step_graph()
+    rho_a, _ = a.field_source_sum()
+    rho_b, _ = b.field_source_sum()
+    np.testing.assert_allclose(rho_a, rho_b, rtol=1e-12, atol=1e-12 * np.abs(rho_b).max())
+    for s in range(4):
+        np.testing.assert_array_equal(_hashes(a.download(s)), _hashes(b.download(s)), err_msg=species[s].name)
+    metal, _ = a.take_tallies()
+    assert np.any(metal != 0.0), "nothing reached the metals"
+    a.close()
+    b.close()

@@ -11,8 +11,12 @@
   absent; the file is not copied into this repository.)
 * A bundle dump written by write_bundle reads back bit for bit, attributes
   included, with NPar below size.
-* Compatibility of write_bundle's output with the HDF5 library itself is
-  parity unpinned here: no HDF5 library is installed to read it.
+* A geometry written by write_geometry reads back bit for bit, dtypes and
+  the reversed (Fortran-reader) dimension order included; the reference's
+  own geometry input survives a write and a read unchanged.
+* Compatibility of write_bundle's and write_geometry's output with the HDF5
+  library itself is parity unpinned here: no HDF5 library is installed to
+  read it.
 """
 import itertools
 import os
@@ -106,3 +110,27 @@ def test_not_hdf5_rejected(tmp_path):
     p.write_bytes(b"not an hdf5 file at all" * 100)
     with pytest.raises(ValueError):
         h5lite.File(p)
+
+
+def test_geometry_round_trip(tmp_path, rng):
+    nz, nr = 9, 6
+    g = {"Nz": nz, "Nr": nr, "cell_material": rng.integers(0, 5, (nz - 1, nr - 1)).astype(np.int32),
+         "epsilon_r": rng.random((nz - 1, nr - 1)), "load": rng.integers(0, 2, (nz - 1, nr - 1)).astype(np.int32),
+         "point_type": rng.integers(-2, 2, (nz, nr)).astype(np.int32), "volume": rng.random((nz, nr)),
+         "hedge_type": rng.integers(0, 5, (nz, nr - 1)).astype(np.int32),
+         "vedge_type": rng.integers(0, 5, (nz - 1, nr)).astype(np.int32)}
+    h5lite.write_geometry(tmp_path / "g.h5", g)
+    e = h5lite.read_geometry(tmp_path / "g.h5")
+    assert (e["Nz"], e["Nr"]) == (nz, nr)
+    for k in g:
+        if k not in ("Nz", "Nr"):
+            np.testing.assert_array_equal(e[k], g[k])
+            assert e[k].dtype == g[k].dtype
+    # stored with the dimensions reversed, as a Fortran reader expects
+    assert h5lite.File(tmp_path / "g.h5").read("hedge_type").shape == (nr - 1, nz)
+    if os.path.exists(GEOM):
+        r = h5lite.read_geometry(GEOM)
+        h5lite.write_geometry(tmp_path / "ref.h5", r)
+        back = h5lite.read_geometry(tmp_path / "ref.h5")
+        for k in r:
+            np.testing.assert_array_equal(np.asarray(back[k]), np.asarray(r[k]))
