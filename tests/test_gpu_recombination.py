@@ -10,7 +10,9 @@ surviving particles are equal as sets, bit for bit:
   equal, then a further step runs on the regrouped bundles;
 * the full-size c2 bundles (257x513 nodes, 50M particles): deletion counts
   and the multiset of per-particle 64-bit hashes equal the oracle's;
-* invalid reactions are rejected.
+* invalid reactions are rejected;
+* the scenario of the reference's own recombination test
+  (R/test/test_recb.F90): survivors and deletion counts equal the oracle's.
 """
 import numpy as np
 import pytest
@@ -103,4 +105,45 @@ def test_invalid_reactions_rejected():
     with pytest.raises(IPMError):
         hp.recombine([(0, 1, 2)], [1e-13], 1e-9)
     assert tuple(hp.recombine([], [], 1e-9)) == (0, 0, 0)
+    hp.close()
+
+
+def test_reference_recb_scenario(oracle):
+    """The scenario of the reference's own recombination test
+    (R/test/test_recb.F90): 4 bundles (e, F-, CF3+, CF3-) of 500 000
+    particles uniform over 10 x 20 cells of 0.1 m (Z in [0, 10), R in
+    [0, 20), WQ = R, at rest), reactions e + CF3+ and F- + CF3+ at constant
+    rates 10000 and 5000, one call over 1e-10 s.  The reference writes the
+    density maps and does not check them; here the GPU's survivors and
+    deletion counts equal the oracle's, bit for bit."""
+    from ipm2d import HotPath, geometry
+    from ipm2d import xsec
+    gen = np.random.default_rng(21)
+    nz, nr, dx, dt = 11, 21, 0.1, 1e-10
+    q = xsec.ELECTRON_CHARGE
+    amu = xsec.ATOMIC_MASS
+    species = [(-q, xsec.ELECTRON_MASS), (-q, 18.9984030 * amu), (q, 69.0059096 * amu), (-q, 69.0059096 * amu)]
+    hp = HotPath(nz, nr, dx, dt, 4, 1)
+    parts = []
+    for s, (qs, ms) in enumerate(species):
+        hp.set_species(s, qs, ms, 600000)
+        P = np.zeros((8, 500000))
+        P[0] = 10.0 * gen.random(500000)
+        P[1] = 20.0 * gen.random(500000)
+        P[7] = P[1]
+        parts.append(P)
+    hp.set_geometry(geometry.build_geometry(nz, nr, dx, np.zeros((nz - 1, nr - 1), np.int32), nmetal=0, ndiel=0))
+    for s in range(4):
+        hp.upload(s, parts[s])
+    reactions, rates = [(0, 2, 0), (1, 2, 0)], [10000.0, 5000.0]
+    masses = [m for _, m in species]
+    want, dl_o = oracle.recombine(nz, nr, dx, dt, parts, masses, reactions, rates, 1e-10, 5, 0, fast=True)
+    hp.set_rng(5, 0)
+    dl_g = hp.recombine(reactions, rates, 1e-10)
+    assert tuple(dl_g) == tuple(dl_o)
+    assert dl_g[0] > 0 and dl_g[1] > 0 and dl_g[2] > 0 and dl_g[3] == 0
+    for s in range(4):
+        got = hp.download(s)
+        assert got.shape == want[s].shape
+        np.testing.assert_array_equal(_hashes(got), _hashes(want[s]))
     hp.close()
