@@ -881,9 +881,11 @@ collide_append_kernel(const DevState *__restrict__ stp, DevCounters *ctr)
 
 // Folds the step's products into n[] (capped at the capacity; the append
 // pass flags what did not fit) and advances the random stream's step key.
-__global__ void collide_fold_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
+// n[d] += the products of the collision pass (staged per block, plus the
+// directly reserved ones), capped at the capacity; step += 1.  One block of
+// kBlock threads.
+__device__ __forceinline__ void fold_counts(const DevState &st, DevCounters *__restrict__ ctr)
 {
-    const DevState &st = *stp;
     __shared__ unsigned long long tot[kBlock / 64][IPM_NS_MAX];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int d = 0; d < st.ns; ++d) {
@@ -905,16 +907,25 @@ __global__ void collide_fold_kernel(const DevState *__restrict__ stp, DevCounter
     }
 }
 
+__global__ void __launch_bounds__(kBlock) collide_fold_kernel(const DevState *__restrict__ stp,
+                                                              DevCounters *__restrict__ ctr)
+{
+    fold_counts(*stp, ctr);
+}
+
 // ------------------------------------------------------- grid epilogue
 // P2CES epilogue (R/src/pic/field/FieldOne.f90:168-175) for species in
 // [s0, s1) and, if sum != 0, FieldSource%Sum (R/src/pic/field/FieldSource.f90:77-87).
 // Raw deposits are cleared here so that no memset enters a captured graph.
-__global__ void grid_kernel(const DevState *__restrict__ stp, const double *__restrict__ inv_vol,
-                            const double *__restrict__ chifac, double *__restrict__ rho_one,
-                            double *__restrict__ chi_one, double *__restrict__ rho_tot,
-                            double *__restrict__ chi_tot, int s0, int s1, int sum)
+// With fold != 0 (the fused step, after the collision pass) block 0 also
+// folds the collision products into the counts (fold_counts).
+__global__ void __launch_bounds__(kBlock)
+grid_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr, const double *__restrict__ inv_vol,
+            const double *__restrict__ chifac, double *__restrict__ rho_one, double *__restrict__ chi_one,
+            double *__restrict__ rho_tot, double *__restrict__ chi_tot, int s0, int s1, int sum, int fold)
 {
     const DevState &st = *stp;
+    if (fold && blockIdx.x == 0) fold_counts(st, ctr);
     const int nz = st.nz, nr = st.nr;
     const size_t nn = (size_t)nz * nr;
     const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1435,23 +1446,27 @@ static int launch_particles(ipm_ctx *c, int sel)
     return IPM_OK;
 }
 
-static int launch_collide(ipm_ctx *c)
+// fold: also fold the products into the counts (else the caller's
+// grid_kernel launch does it)
+static int launch_collide(ipm_ctx *c, bool fold)
 {
     const unsigned G = (unsigned)c->h.c_grid;
     hipLaunchKernelGGL(collide_kernel, dim3(G), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr);
     hipLaunchKernelGGL(collide_append_kernel, dim3(G), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr);
-    hipLaunchKernelGGL(collide_fold_kernel, dim3(1), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr);
+    if (fold)
+        hipLaunchKernelGGL(collide_fold_kernel, dim3(1), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr);
     HIPCHK(hipGetLastError());
     return IPM_OK;
 }
 
-static int launch_grid(ipm_ctx *c, int s0, int s1, int sum)
+static int launch_grid(ipm_ctx *c, int s0, int s1, int sum, int fold = 0)
 {
     const size_t nn = (size_t)c->nz * c->nr;
     const size_t ngh = (size_t)(c->nz + 2) * (c->nr + 2);
     const size_t nt = std::max(nn, ngh);
-    hipLaunchKernelGGL(grid_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, c->d_state, c->inv_vol,
-                       c->d_chifac, c->rho_one, c->chi_one, c->rho_tot, c->chi_tot, s0, s1, sum);
+    hipLaunchKernelGGL(grid_kernel, dim3((unsigned)((nt + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream,
+                       c->d_state, c->d_ctr, c->inv_vol, c->d_chifac, c->rho_one, c->chi_one, c->rho_tot, c->chi_tot,
+                       s0, s1, sum, fold);
     HIPCHK(hipGetLastError());
     return IPM_OK;
 }
@@ -1909,7 +1924,7 @@ int ipm_mcc(ipm_ctx *c)
     int rc = check_ready(c, false, false);
     if (rc) return rc;
     if ((rc = launch_particles<M_MCC>(c, -1))) return rc;
-    if ((rc = launch_collide(c))) return rc;
+    if ((rc = launch_collide(c, true))) return rc;
     DevCounters ctr;
     return read_counters(c, &ctr);
 }
@@ -1956,13 +1971,13 @@ static int step_enqueue(ipm_ctx *c, bool capturing)
         c->ev_pairs[0].push_back({a, b});
         a = b;
     }
-    if ((rc = launch_collide(c))) return rc;
+    if ((rc = launch_collide(c, false))) return rc;
     if (t) {
         if ((rc = timed_event(c, &b))) return rc;
         c->ev_pairs[1].push_back({a, b});
         a = b;
     }
-    if ((rc = launch_grid(c, 0, c->ns, 1))) return rc;
+    if ((rc = launch_grid(c, 0, c->ns, 1, 1))) return rc;
     if (t) {
         if ((rc = timed_event(c, &b))) return rc;
         c->ev_pairs[2].push_back({a, b});
