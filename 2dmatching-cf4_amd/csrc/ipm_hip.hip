@@ -57,6 +57,7 @@ __device__ __forceinline__ bool is_dead(double z) { return z < -1.0e299; }
 // Products of one collision block, staged in LDS and appended by
 // collide_append_kernel without any contended atomic.
 constexpr int kProdStage = 256;
+constexpr int kCollideBlocksPerCuMax = 6;   // the product stages are sized for this many
 struct ProdSink {
     double *rec;        // kProdStage x 8 doubles
     unsigned *meta;     // species | rank << 8
@@ -1403,13 +1404,13 @@ static int ensure_queue(ipm_ctx *c)
         }
     }
     if (!c->p_stage) {
-        const size_t G = (size_t)c->num_cu * 6;
+        const size_t G = (size_t)c->num_cu * kCollideBlocksPerCuMax;
         HIPCHK(hipMalloc(&c->p_stage, sizeof(double) * 8 * kProdStage * G));
         HIPCHK(hipMalloc(&c->p_meta, sizeof(unsigned) * kProdStage * G));
         HIPCHK(hipMalloc(&c->p_cnt, sizeof(unsigned) * IPM_NS_MAX * G));
         HIPCHK(hipMemset(c->p_cnt, 0, sizeof(unsigned) * IPM_NS_MAX * G));
-        c->h.c_grid = (int)G;
     }
+    c->h.c_grid = c->num_cu * c->collide_bpc;
     c->h.q_rec = c->qbuf;
     c->h.tile_anchor = c->tanc;
     c->h.c_rec = c->cbuf;
@@ -2109,6 +2110,21 @@ int ipm_set_launch(ipm_ctx *c, int block, int blocks_per_cu)
         c->graph = nullptr;
     }
     return IPM_OK;
+}
+
+int ipm_set_collide_grid(ipm_ctx *c, int blocks_per_cu)
+{
+    if (!c) return fail(IPM_EINVAL, "null context");
+    if (blocks_per_cu < 1 || blocks_per_cu > kCollideBlocksPerCuMax)
+        return fail(IPM_EINVAL, "collision blocks per CU must be 1..%d, got %d", kCollideBlocksPerCuMax, blocks_per_cu);
+    c->collide_bpc = blocks_per_cu;
+    if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    if (!c->p_stage) return IPM_OK;       // applied when the collision buffers are made
+    c->h.c_grid = c->num_cu * c->collide_bpc;
+    return sync_state(c);
 }
 
 int ipm_species_device_ptrs(ipm_ctx *c, int s, double **fields8)

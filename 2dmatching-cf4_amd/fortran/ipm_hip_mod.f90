@@ -164,6 +164,11 @@ module ipm_hip
             type(c_ptr), value :: ctx
             integer(c_int), value :: block, blocks_per_cu
         end function
+        integer(c_int) function ipm_set_collide_grid(ctx, blocks_per_cu) bind(C, name="ipm_set_collide_grid")
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: blocks_per_cu
+        end function
         integer(c_int) function ipm_synchronize(ctx) bind(C, name="ipm_synchronize")
             import :: c_ptr, c_int
             type(c_ptr), value :: ctx

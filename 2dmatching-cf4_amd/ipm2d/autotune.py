@@ -7,6 +7,9 @@ live simulation: every candidate advances the real state by a few steps.
 
 * ``blocks_per_cu``: 0 for one block per tile, else persistent blocks per
   compute unit of the fused particle kernel; chosen by timing a few steps each.
+* ``collide_blocks_per_cu``: grid of the collision pass (blocks per compute
+  unit); chosen by timing a few whole steps each, since it also changes
+  where the pass leaves the caches for the next particle pass.
 * ``sort_interval``: steps between cell sorts.  A sort costs about one
   particle pass but keeps the deposit's LDS window and wave reduction
   effective; the interval minimising the mean step time over a full
@@ -45,6 +48,20 @@ def tune_blocks_per_cu(hp, candidates=(0, 2, 4, 6, 8), steps=6, timer=None):
     return best, res
 
 
+def tune_collide_grid(hp, candidates=(2, 4, 6), steps=6, timer=None):
+    """Return (best, {candidate: seconds per step})."""
+    timer = timer or (lambda: _timed(hp, steps))
+    res = {}
+    hp.sort(-1)
+    for b in candidates:
+        hp.set_collide_grid(b)
+        hp.one_step()
+        res[b] = timer()
+    best = min(res, key=res.get)
+    hp.set_collide_grid(best)
+    return best, res
+
+
 def tune_sort_interval(hp, candidates=(5, 10, 20, 40), timer=None):
     """Mean step time over one full interval (its sort included) per candidate;
     returns (best, {interval: seconds per step})."""
@@ -64,7 +81,7 @@ def cache_key(hp, nparticles):
 
 
 def autotune(hp, nparticles, cache_path=None, bpc_candidates=(0, 2, 4, 6, 8),
-             sort_candidates=(5, 10, 20, 40), steps=6):
+             sort_candidates=(5, 10, 20, 40), steps=6, collide_candidates=(2, 4, 6)):
     """Tune (or load from cache) and apply; returns the chosen settings."""
     key = cache_key(hp, nparticles)
     if cache_path and os.path.exists(cache_path):
@@ -72,11 +89,14 @@ def autotune(hp, nparticles, cache_path=None, bpc_candidates=(0, 2, 4, 6, 8),
             cache = json.load(f)
         if key in cache:
             hp.set_launch(cache[key]["blocks_per_cu"])
+            hp.set_collide_grid(cache[key].get("collide_blocks_per_cu", 6))
             return cache[key]
     bpc, tb = tune_blocks_per_cu(hp, bpc_candidates, steps)
+    cbpc, tc = tune_collide_grid(hp, collide_candidates, steps) if collide_candidates else (6, {})
     si, ts = tune_sort_interval(hp, sort_candidates)
-    out = {"blocks_per_cu": bpc, "sort_interval": si,
+    out = {"blocks_per_cu": bpc, "collide_blocks_per_cu": cbpc, "sort_interval": si,
            "step_seconds_by_blocks_per_cu": {str(k): v for k, v in tb.items()},
+           "step_seconds_by_collide_blocks_per_cu": {str(k): v for k, v in tc.items()},
            "step_seconds_by_sort_interval": {str(k): v for k, v in ts.items()}}
     if cache_path:
         cache = {}

@@ -108,6 +108,9 @@ class HotPath:
     def set_launch(self, blocks_per_cu=0):
         N.call("ipm_set_launch", self._h, 0, int(blocks_per_cu))
 
+    def set_collide_grid(self, blocks_per_cu=6):
+        N.call("ipm_set_collide_grid", self._h, int(blocks_per_cu))
+
     # ------------------------------------------------------------ particles
     def upload(self, s, soa):
         soa = np.ascontiguousarray(soa, dtype=np.float64)

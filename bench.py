@@ -277,7 +277,8 @@ def main():
         from ipm2d import autotune
         tuned = autotune.autotune(hp, ntot)
         sort_interval = tuned["sort_interval"]
-        log(f"[bench] autotuned: blocks_per_cu {tuned['blocks_per_cu']}, sort every {sort_interval} steps")
+        log(f"[bench] autotuned: blocks_per_cu {tuned['blocks_per_cu']}, collision blocks per CU "
+            f"{tuned['collide_blocks_per_cu']}, sort every {sort_interval} steps")
     elif args.blocks_per_cu > 0:
         hp.set_launch(args.blocks_per_cu)
     log(f"[bench] uploaded; warmup {args.warmup} steps")
@@ -369,7 +370,8 @@ def main():
             "sort_interval": sort_interval,
             "launch": ("autotuned: " if tuned else "") + (
                 "one block per 2048-slot tile" if (tuned["blocks_per_cu"] if tuned else max(args.blocks_per_cu, 0)) == 0
-                else f"{tuned['blocks_per_cu'] if tuned else args.blocks_per_cu} persistent blocks per CU"),
+                else f"{tuned['blocks_per_cu'] if tuned else args.blocks_per_cu} persistent blocks per CU") + (
+                f", collision pass {tuned['collide_blocks_per_cu']} blocks per CU" if tuned else ""),
         },
         "roofline": {
             "bound": "hbm",

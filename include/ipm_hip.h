@@ -144,6 +144,11 @@ int ipm_sort(ipm_ctx *ctx, int s);
  * (grid sized from the capacities, the default), 1..64 a persistent grid of
  * that many blocks per compute unit looping over the tiles. */
 int ipm_set_launch(ipm_ctx *ctx, int block, int blocks_per_cu);
+
+/* Grid of the collision pass: blocks_per_cu (1..6, default 6) blocks per
+ * compute unit.  Like ipm_set_launch, a scheduling choice with no
+ * counterpart in the reference: results do not depend on it. */
+int ipm_set_collide_grid(ipm_ctx *ctx, int blocks_per_cu);
 int ipm_synchronize(ipm_ctx *ctx);
 
 /* Device addresses of species s's 8 SoA arrays (for zero-copy interop) and
