@@ -18,7 +18,6 @@ around each (``eager_value``, and the kernel times the roofline uses).
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time
