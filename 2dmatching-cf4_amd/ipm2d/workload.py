@@ -9,7 +9,7 @@ Velocities are Maxwellian at the species temperature; Az = Ar = 0.
 """
 import math
 from dataclasses import dataclass, field
-from typing import Dict, List, Tuple
+from typing import List, Tuple
 
 import numpy as np
 

@@ -12,7 +12,6 @@ import pytest
 from helpers import (AMU, ME, QE, assert_close, random_fields, random_particles, small_geometry, sort_set)
 from ipm2d import HotPath, CapacityError, IPMError
 from ipm2d import workload as W
-from ipm2d import xsec
 
 pytestmark = pytest.mark.gpu
 
