@@ -399,3 +399,35 @@ def test_build_hotpath_from_the_reference_inputs():
     loadable = int(geom.cell_load.sum())
     for s in range(4):
         assert hp.uploads[s][0] == 8 and 0.4 * 50 * loadable < hp.uploads[s][1] < 50 * loadable
+
+
+def test_refused_inputs_say_what_is_missing(tmp_path):
+    d = copy.deepcopy(BASE)
+    d["gas"]["collision_section_type"] = 2
+    run = _load(tmp_path, d)
+    with pytest.raises(NotImplementedError, match="lxcat"):
+        RC.gases_and_species(run, root=str(tmp_path))
+    d["gas"]["collision_section_type"] = 1
+    run = _load(tmp_path, d)
+    gas = tmp_path / "input" / "gas" / "lxcat" / "CF4"
+    gas.mkdir(parents=True)
+    (gas / "CF4.txt").write_text(""" &GasPhysicsfile
+ GP%Name = CF4, GP%MCModel = 2, GP%NAtom = 5, GP%Ns = 1, GP%Mass = 88.0043128,
+ GP%SP(1) = CF3+,4,1,69.0059096,2.d-10,
+ /
+""")
+    species, gases = RC.gases_and_species(run, root=str(tmp_path))
+    assert [sp.name for sp in species] == ["Electron", "CF3+"] and gases[0].ns == 1
+    # no cross-section file for the electron: the reference's hard-sphere fallback is not built
+    with pytest.raises(NotImplementedError, match="ElectronCF4.txt"):
+        RC.mcc_tables(run, species, gases, root=str(tmp_path))
+    # a species from another gas set would need the same fallback
+    species[1].gas = 2
+    (gas / "ElectronCF4.txt").write_text(_xsec_text())
+    with pytest.raises(NotImplementedError, match="another gas"):
+        RC.mcc_tables(run, species, gases, root=str(tmp_path))
+
+
+def _xsec_text():
+    from ipm2d import xsec
+    return xsec.write_sigma_file(1, xsec.synthetic_electron_reactions(ion_product=1, att_product=1))
