@@ -97,3 +97,7 @@ def test_tuning_does_not_change_results(rng):
         np.testing.assert_allclose(ga, gb, rtol=1e-12, atol=1e-300)
     assert out["blocks_per_cu"] in (2, 6) and out["sort_interval"] in (3, 5)
     assert out["collide_blocks_per_cu"] in (1, 6)
+    from ipm2d import IPMError
+    for bad in (0, 7):
+        with pytest.raises(IPMError):
+            a.set_collide_grid(bad)
