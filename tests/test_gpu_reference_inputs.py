@@ -38,7 +38,7 @@ def _hashes(P):
     return np.sort(h)
 
 
-def _write_inputs(root):
+def _write_inputs(root, controls=False):
     (root / "input" / "gas" / "lxcat" / "CF4").mkdir(parents=True)
     cm = np.zeros((NZ - 1, NR - 1), dtype=np.int32)      # indices into the materials list
     cm[:2, :] = 1
@@ -94,6 +94,13 @@ def _write_inputs(root):
                 "collision_section_type": 1, "is_open_recombination": False},
         "circuit": {"circuits_number": 1, "circuits_list": [{"type": 1, "metal_index": 2, "name": "ground"}]},
     }
+    if controls:                            # the shipped configuration's controls
+        cfg["plasma"]["is_open_split_merge"] = True
+        cfg["diag"]["diag_preiod"] = 4
+        cfg["gas"]["is_open_recombination"] = True
+        cfg["gas"]["recb"] = {"recb_time_step_num": 5, "recb_reaction_num": 3,
+                              "recb_list": [[1, 2, 0], [2, 3, 0], [0, 2, 1]],
+                              "recb_rate": [1e-6, 1e-6, 3.95e-15]}
     (root / "config.json").write_text(json.dumps(cfg))
     return root / "config.json"
 
@@ -124,3 +131,35 @@ def test_reference_inputs_to_gpu_steps(tmp_path):
     assert np.any(metal != 0.0), "nothing reached the metals"
     a.close()
     b.close()
+
+
+def test_control_loop_from_reference_inputs(tmp_path):
+    """The loop of the integration notes, with the shipped configuration's
+    controls switched on: split/merge of the electrons every step and of
+    every bundle every Period (4) steps, recombination every 5 steps with
+    the shipped reaction list (the two constant rates raised to 1e-6 m^3/s
+    so that twelve small steps see deletions).  Every control acts, and
+    every particle stays inside the domain with a positive weight."""
+    run = RC.load(_write_inputs(tmp_path, controls=True))
+    hp, species, _, _ = RC.build_hotpath(run, seed=4, capacity=80000)
+    rezone = run.rezoning(hp)
+    recomb = run.recombination_control(hp)
+    assert rezone.period == 4 and recomb.time_step_number == 5
+    z = np.arange(NZ)[:, None] * np.ones((1, NR))
+    hp.set_potential(150.0 * (1.0 - z / (NZ - 1)))
+    touched, deleted = set(), np.zeros(4, dtype=np.int64)
+    for timer in range(1, 13):
+        hp.one_step_graph()
+        hp.field_source_sum()
+        touched |= set(rezone.control(timer))
+        d = recomb.control(timer)
+        if d is not None:
+            deleted += np.asarray(d)
+    assert touched == {0, 1, 2, 3}
+    assert deleted[1] > 0 and deleted[2] > 0 and deleted[3] > 0
+    for s in range(4):
+        P = hp.download(s)
+        assert P.shape[1] > 0, species[s].name
+        assert np.all((P[0] > 0) & (P[0] < NZ - 1) & (P[1] >= 0) & (P[1] < NR - 1)), species[s].name
+        assert np.all(P[7] > 0), species[s].name
+    hp.close()
