@@ -25,8 +25,19 @@
  *   ora_mcc               selection ratio 1-exp(-nu dt), channel fractions
  *                         n sigma v, Vahedi/isotropic cos-chi CDFs,
  *                         Maxwellian mean energy, energy bookkeeping of
- *                         excitation/ionization, momentum of elastic events
- *   ora_sigma_* / ora_probability_*  closed forms on synthetic tables
+ *                         excitation/ionization, momentum of elastic events;
+ *                         through ora_collide_given: hand-worked 111/112/113
+ *                         velocities (with the Vy <- Vz term), charge
+ *                         exchange = target velocity, CM speed conserved by
+ *                         101/11 with isotropic/Vahedi mean deflection,
+ *                         the ionization secondary's direction and inverted
+ *                         CreateEnergy draw, dissociation product species;
+ *                         Langevin fraction 1/beta_max^2 and the
+ *                         sub-Threshold(2) shortcut; gases in loop order
+ *   ora_sigma_* / ora_probability_*  closed forms on synthetic tables,
+ *                         including sigma linear in the log-energy index
+ *                         (the dble(i) weight) and the Emin coordinate of
+ *                         raw energies below the grid
  *   ora_recombine         hand-computed single-cell budgets (constant and
  *                         temperature forms), the visiting order's two
  *                         outcomes at their closed-form frequencies, cells
@@ -675,13 +686,20 @@ uint32_t ora_vel_maxwell(double mass, double temperature, uint64_t seed, uint32_
     return rg.k;
 }
 
-/* Updater, TypeMCC.f90:60-92 */
+/* Updater, TypeMCC.f90:60-92: the target sample, then the relative terms */
+static void mc_relative(mcpo_t *m, double sm, double gm);
 static void mc_update(mcpo_t *m, double sm, double gm, double gT, rng_t *rg)
 {
     m->pot = m->poi;
     m->miu = sm * gm / (sm + gm);
     m->mass_ratio = gm / (sm + gm);
     vel_max_init(&m->pot, gm, gT, rg);
+    mc_relative(m, sm, gm);
+}
+
+/* TypeMCC.f90:78-91: relative velocity, CM energy, target Az/Ar */
+static void mc_relative(mcpo_t *m, double sm, double gm)
+{
     m->gx = m->poi.Vr - m->pot.Vr;
     m->gy = m->poi.Vt - m->pot.Vt;
     m->gz = m->poi.Vz - m->pot.Vz;
@@ -864,6 +882,44 @@ static void collide_ion(mcpo_t *m, int rtype, int resultant, double th, rng_t *r
         break;
     default: break;
     }
+}
+
+/* Test hook: reaction `rtype` of one particle against a given target.
+ * poi: Z R Vz Vr Vt Az Ar WQ with velocities in m/s (after VelRes(VFactor));
+ * pot_v: the target's Vz Vr Vt in m/s, in place of the Maxwellian sample
+ * Updater draws (TypeMCC.f90:60-77); the relative terms follow
+ * (TypeMCC.f90:78-91), then SelectCollisionElectron (model 1) or
+ * SelectCollisionIon (models 2, 3).  Draws the collision needs come from
+ * the stream (seed, step 0, tag 0, c0, 0, 0).  out: the particle afterwards;
+ * energy_out: MCPO%Energy afterwards; prod/prod_species: the products.
+ * Returns the number of products. */
+int ora_collide_given(int model, int species, int rtype, int resultant, double threshold, double species_mass,
+                      double gas_mass, const double poi[8], const double pot_v[3], uint64_t seed, uint32_t c0,
+                      double out[8], double *energy_out, double prod[16], int prod_species[2])
+{
+    mcpo_t m;
+    memset(&m, 0, sizeof m);
+    m.poi.Z = poi[0]; m.poi.R = poi[1]; m.poi.Vz = poi[2]; m.poi.Vr = poi[3];
+    m.poi.Vt = poi[4]; m.poi.Az = poi[5]; m.poi.Ar = poi[6]; m.poi.WQ = poi[7];
+    m.pot = m.poi;
+    m.miu = species_mass * gas_mass / (species_mass + gas_mass);
+    m.mass_ratio = gas_mass / (species_mass + gas_mass);
+    m.pot.Vz = pot_v[0]; m.pot.Vr = pot_v[1]; m.pot.Vt = pot_v[2];
+    mc_relative(&m, species_mass, gas_mass);
+    rng_t rg = {seed, 0, 0, c0, 0, 0, 0};
+    if (model == 1) collide_electron(&m, species, rtype, resultant, threshold, &rg);
+    else collide_ion(&m, rtype, resultant, threshold, &rg);
+    const po_t *q = &m.poi;
+    out[0] = q->Z; out[1] = q->R; out[2] = q->Vz; out[3] = q->Vr;
+    out[4] = q->Vt; out[5] = q->Az; out[6] = q->Ar; out[7] = q->WQ;
+    *energy_out = m.energy;
+    for (int k = 0; k < m.nnew && k < 2; ++k) {
+        const po_t *p = &m.pon[k];
+        double *o = prod + 8 * k;
+        o[0] = p->Z; o[1] = p->R; o[2] = p->Vz; o[3] = p->Vr; o[4] = p->Vt; o[5] = p->Az; o[6] = p->Ar; o[7] = p->WQ;
+        prod_species[k] = m.pon_idx[k];
+    }
+    return m.nnew;
 }
 
 int ora_mcc(const ora_mcc_setup *st, uint64_t seed, uint32_t step,

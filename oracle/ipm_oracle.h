@@ -122,6 +122,14 @@ typedef struct {
 int ora_mcc(const ora_mcc_setup *st, uint64_t seed, uint32_t step,
             int64_t *n, const int64_t *cap, double *(*P)[8], int64_t *counts);
 
+/* Test hook: reaction rtype (MCCElectron.F90:13-147 for model 1, MCCIon.F90:6-74
+ * otherwise) of one particle poi (Z R Vz Vr Vt Az Ar WQ, velocities in m/s)
+ * against the given target velocity pot_v (Vz Vr Vt, m/s) in place of the
+ * Maxwellian sample.  Returns the number of products (prod: 8 per product). */
+int ora_collide_given(int model, int species, int rtype, int resultant, double threshold, double species_mass,
+                      double gas_mass, const double poi[8], const double pot_v[3], uint64_t seed, uint32_t c0,
+                      double out[8], double *energy_out, double prod[16], int prod_species[2]);
+
 /* Maxwellian speed + isotropic direction, ParticleOne.f90:76-123, with draws
  * from a per-particle stream; returns the next draw index. */
 uint32_t ora_vel_maxwell(double mass, double temperature, uint64_t seed, uint32_t slot,

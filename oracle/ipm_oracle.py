@@ -71,6 +71,9 @@ def lib(fast=False):
                               C.c_int, C.c_int64, C.POINTER(_dp)] + [_dp] * 6
     L.ora_mcc.restype = C.c_int
     L.ora_mcc.argtypes = [C.POINTER(OraSetup), C.c_uint64, C.c_uint32, _i64p, _i64p, C.POINTER(_dp * 8), _i64p]
+    L.ora_collide_given.restype = C.c_int
+    L.ora_collide_given.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _dp,
+                                    _dp, C.c_uint64, C.c_uint32, _dp, _dp, _dp, _ip]
     L.ora_vel_maxwell.restype = C.c_uint32
     L.ora_vel_maxwell.argtypes = [C.c_double, C.c_double, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32,
                                   C.c_uint32, _dp, _dp, _dp]
@@ -368,6 +371,21 @@ def mcc(setup, seed, step, species, extra_capacity=None, counts=False, fast=Fals
         raise RuntimeError("oracle MCC capacity overflow")
     out = [bufs[s][:, :nvec[s]].copy() for s in range(ns)]
     return out, (cnt.reshape(ns, setup.ng, 128) if counts else None)
+
+
+def collide_given(model, species, rtype, resultant, threshold, species_mass, gas_mass, poi, pot_v, seed=1, c0=0):
+    """One reaction against a given target velocity (ora_collide_given).
+    poi: 8 values Z R Vz Vr Vt Az Ar WQ (velocities in m/s); pot_v: Vz Vr Vt.
+    Returns (particle afterwards, energy afterwards, [(species, product), ...])."""
+    a = np.ascontiguousarray(poi, dtype=np.float64)
+    t = np.ascontiguousarray(pot_v, dtype=np.float64)
+    out = np.zeros(8)
+    en = C.c_double()
+    prod = np.zeros(16)
+    ps = np.zeros(2, dtype=np.int32)
+    k = lib().ora_collide_given(model, species, rtype, resultant, threshold, species_mass, gas_mass, _d(a), _d(t),
+                                seed, c0, _d(out), C.byref(en), _d(prod), _i(ps))
+    return out, en.value, [(int(ps[j]), prod[8 * j:8 * j + 8].copy()) for j in range(k)]
 
 
 def vel_maxwell(mass, temperature, seed, slot, step, draw, tag):
