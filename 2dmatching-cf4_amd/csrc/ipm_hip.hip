@@ -65,8 +65,13 @@ struct ProdSink {
     unsigned *spc;      // staged per species (IPM_NS_MAX)
 };
 
+// twin != 0: the product would be a bit-identical copy of another particle
+// of its species made by the same collision (dissociation into two equal
+// fragments, 113 giving back the parent's own species); the low bit of its
+// Vt pattern is flipped so that the two draw from different state-keyed
+// streams afterwards, as split copies do.
 __device__ __forceinline__ void emit_product(const DevState &st, DevCounters *ctr, Part q, int d,
-                                             const ProdSink *ps = nullptr)
+                                             const ProdSink *ps = nullptr, bool twin = false)
 {
     if (d < 0 || d >= st.ns) {
         atomicOr(&ctr->overflow, 2);
@@ -75,6 +80,7 @@ __device__ __forceinline__ void emit_product(const DevState &st, DevCounters *ct
     q.Vz = q.Vz * st.inv_vfactor;
     q.Vr = q.Vr * st.inv_vfactor;
     q.Vt = q.Vt * st.inv_vfactor;
+    if (twin) q.Vt = __longlong_as_double(__double_as_longlong(q.Vt) ^ 1ll);
     if (ps) {
         const unsigned k = atomicAdd(ps->count, 1u);
         if (k < (unsigned)kProdStage) {
@@ -174,7 +180,7 @@ __device__ __forceinline__ void mcc_finish(const DevState &st, DevCounters *ctr,
                 mc_vel_update(m, VF);
                 post_collision(m, true, rg);
                 emit_product(st, ctr, m.pot, res % 10, ps);
-                emit_product(st, ctr, m.pot, res / 10, ps);
+                emit_product(st, ctr, m.pot, res / 10, ps, res % 10 == res / 10);
             } break;
             default:
                 break;
@@ -191,7 +197,7 @@ __device__ __forceinline__ void mcc_finish(const DevState &st, DevCounters *ctr,
                 break;
             case 113:
                 ion_reactive(m, th);
-                emit_product(st, ctr, m.poi, res, ps);
+                emit_product(st, ctr, m.poi, res, ps, res == s);
                 break;
             default: break;   // 100 and -1 change nothing
             }
@@ -714,6 +720,9 @@ collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
             const long long nt = ((long long)ctr->n[s] + kTile - 1) / kTile;
             if (nt > 0 && st.q_weight[s] > 0.0) { W += (double)nt * st.q_weight[s]; ++A; }
         }
+        // every species with work needs a block of its own: a grid smaller
+        // than that would leave queued selections uncollided
+        if (A > (int)G && blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&ctr->overflow, 4);
         if (A > 0 && G >= (unsigned)A) {
             const double Grem = (double)(G - A);
             double Wlt = 0.0;
@@ -1411,6 +1420,8 @@ static int ensure_queue(ipm_ctx *c)
         HIPCHK(hipMemset(c->p_cnt, 0, sizeof(unsigned) * IPM_NS_MAX * G));
     }
     c->h.c_grid = c->num_cu * c->collide_bpc;
+    if (c->h.c_grid < c->ns)
+        return fail(IPM_EINVAL, "collision grid of %d blocks is smaller than the %d species", c->h.c_grid, c->ns);
     c->h.q_rec = c->qbuf;
     c->h.tile_anchor = c->tanc;
     c->h.c_rec = c->cbuf;
@@ -1658,6 +1669,7 @@ int ipm_upload_species(ipm_ctx *c, int s, int64_t n, const double *po, int layou
     DevCounters ctr;
     HIPCHK(hipMemcpy(&ctr, c->d_ctr, sizeof ctr, hipMemcpyDeviceToHost));
     ctr.n[s] = (unsigned long long)n; ctr.n_new[s] = 0; ctr.n_dead[s] = 0;
+    ctr.overflow = 0;     // a fresh population: earlier overflow reports are void
     HIPCHK(hipMemcpy(c->d_ctr, &ctr, sizeof ctr, hipMemcpyHostToDevice));
     return IPM_OK;
 }
@@ -1916,6 +1928,7 @@ int ipm_set_rng(ipm_ctx *c, uint64_t seed, uint32_t step)
     HIPCHK(hipMemcpy(&ctr, c->d_ctr, sizeof ctr, hipMemcpyDeviceToHost));
     ctr.seed = seed;
     ctr.step = step;
+    ctr.overflow = 0;     // a restart of the stream clears earlier overflow reports
     HIPCHK(hipMemcpy(c->d_ctr, &ctr, sizeof ctr, hipMemcpyHostToDevice));
     return IPM_OK;
 }
@@ -2122,6 +2135,9 @@ int ipm_set_collide_grid(ipm_ctx *c, int blocks_per_cu)
         (void)hipGraphExecDestroy(c->graph);
         c->graph = nullptr;
     }
+    if (c->num_cu * blocks_per_cu < c->ns)
+        return fail(IPM_EINVAL, "collision grid of %d blocks is smaller than the %d species", c->num_cu * blocks_per_cu,
+                    c->ns);
     if (!c->p_stage) return IPM_OK;       // applied when the collision buffers are made
     c->h.c_grid = c->num_cu * c->collide_bpc;
     return sync_state(c);

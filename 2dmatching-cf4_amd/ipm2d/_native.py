@@ -17,7 +17,7 @@ IPM_ECAPACITY = -3
 IPM_ENODEV = -4
 LAYOUT_AOS = 0
 LAYOUT_SOA = 1
-NS_MAX = 8
+NS_MAX = 9
 NG_MAX = 3
 NR_MAX = 128
 

@@ -34,7 +34,7 @@ extern "C" {
 #define IPM_LAYOUT_AOS 0   /* n records of 8 doubles (the reference's PO(:) array) */
 #define IPM_LAYOUT_SOA 1   /* 8 consecutive arrays of n doubles */
 
-#define IPM_NS_MAX 8
+#define IPM_NS_MAX 9       /* the reference's ns_max (R/config.json:83) */
 #define IPM_NG_MAX 3
 #define IPM_NR_MAX 128     /* reactions per (species, gas) table */
 /* Largest capacity of one species (2^29 - 32 particles, 4 GiB per field):
@@ -64,7 +64,9 @@ int ipm_set_geometry(ipm_ctx *ctx, const double *inv_volume, const int *cell_mat
                      int ndiel, int nmetal, const int *material_type);
 
 /* Particle transfer (restart Dump/Load, ParticleBundle.f90:117-150, and the
- * explicit host sync the reference's other per-particle readers need). */
+ * explicit host sync the reference's other per-particle readers need).
+ * An upload clears the device's pending error reports (capacity, queue),
+ * so a context recovers from IPM_ECAPACITY by re-uploading its species. */
 int ipm_upload_species(ipm_ctx *ctx, int s, int64_t n, const double *po, int layout);
 /* Downloads the live particles (dead slots dropped) into po (capacity cap). */
 int ipm_download_species(ipm_ctx *ctx, int s, int64_t *n, double *po, int64_t cap, int layout);
@@ -106,7 +108,8 @@ int ipm_set_mcc(ipm_ctx *ctx, int s, int g, int model, int nreaction, int nsigma
                 const double *prob, double gas_mass, double gas_temperature, double beta_max);
 /* Counter-based random stream (Philox4x32-10) replacing the reference's
  * unseeded random_number: draws are keyed by (seed, step, species|gas<<8,
- * slot, draw).  step advances by one after every MCC pass. */
+ * slot, draw).  step advances by one after every MCC pass.  Also clears the
+ * device's pending error reports, like ipm_upload_species. */
 int ipm_set_rng(ipm_ctx *ctx, uint64_t seed, uint32_t step);
 /* MCC(Ns, Ng, PB, SO, GO, MCCB) (R/src/mc/MCCPublic.F90:38-121) including
  * MCCAddNewParticle (209-226): products are appended to their species. */

@@ -118,6 +118,14 @@ void ora_state_key(double Z, double R, double Vz, double Vr, double Vt, uint32_t
     out[2] = (uint32_t)vz ^ rotl32((uint32_t)vr, 11) ^ rotl32((uint32_t)(z >> 32), 3) ^ rotl32((uint32_t)(r >> 32), 19);
 }
 
+static double flip_low_bit(double d)
+{
+    uint64_t u = dbits(d) ^ 1u;
+    double r;
+    memcpy(&r, &u, sizeof r);
+    return r;
+}
+
 typedef struct { uint64_t seed; uint32_t step, tag, c0, c1, c2, k; } rng_t;
 static double rnext(rng_t *r) { return ora_uniform(r->seed, r->step, r->tag, r->c0, r->c1, r->c2, r->k++); }
 
@@ -644,6 +652,7 @@ typedef struct {
     int nnew;
     po_t pon[2];
     int pon_idx[2];
+    int pon_twin[2];   /* product k duplicates another particle of its species */
 } mcpo_t;
 
 /* VecRanXYZInit, ParticleOne.f90:108-123 */
@@ -845,6 +854,7 @@ static void collide_electron(mcpo_t *m, int species, int rtype, int resultant, d
         m->pon[1] = m->pot;
         m->pon_idx[0] = resultant % 10;
         m->pon_idx[1] = resultant / 10;
+        m->pon_twin[1] = (resultant % 10 == resultant / 10);
     } break;
     default:
         break;
@@ -864,7 +874,7 @@ static void ion_reactive(mcpo_t *m, double th)
 }
 
 /* SelectCollisionIon, MCCIon.F90:6-51 */
-static void collide_ion(mcpo_t *m, int rtype, int resultant, double th, rng_t *rg)
+static void collide_ion(mcpo_t *m, int species, int rtype, int resultant, double th, rng_t *rg)
 {
     switch (rtype) {
     case 101: post_collision(m, 0, rg); break;
@@ -879,6 +889,7 @@ static void collide_ion(mcpo_t *m, int rtype, int resultant, double th, rng_t *r
     case 113:
         ion_reactive(m, th);
         m->nnew = 1; m->pon[0] = m->poi; m->pon_idx[0] = resultant;
+        m->pon_twin[0] = (resultant == species);
         break;
     default: break;
     }
@@ -908,7 +919,7 @@ int ora_collide_given(int model, int species, int rtype, int resultant, double t
     mc_relative(&m, species_mass, gas_mass);
     rng_t rg = {seed, 0, 0, c0, 0, 0, 0};
     if (model == 1) collide_electron(&m, species, rtype, resultant, threshold, &rg);
-    else collide_ion(&m, rtype, resultant, threshold, &rg);
+    else collide_ion(&m, species, rtype, resultant, threshold, &rg);
     const po_t *q = &m.poi;
     out[0] = q->Z; out[1] = q->R; out[2] = q->Vz; out[3] = q->Vr;
     out[4] = q->Vt; out[5] = q->Az; out[6] = q->Ar; out[7] = q->WQ;
@@ -930,6 +941,7 @@ int ora_mcc(const ora_mcc_setup *st, uint64_t seed, uint32_t step,
     int64_t nst = 0, nstcap = 1024;
     po_t *stage = (po_t *)malloc(sizeof(po_t) * nstcap);
     int *stage_idx = (int *)malloc(sizeof(int) * nstcap);
+    int *stage_twin = (int *)malloc(sizeof(int) * nstcap);
     int rc = 0;
     /* every particle's stream is keyed by its state at the start of MCC */
     uint32_t *keys[64] = {0};
@@ -963,7 +975,7 @@ int ora_mcc(const ora_mcc_setup *st, uint64_t seed, uint32_t step,
                     if (t->model == 1)
                         collide_electron(&m, s, ty, t->resultant[idx - 1], t->threshold[idx - 1], &rg);
                     else
-                        collide_ion(&m, ty, t->resultant[idx - 1], t->threshold[idx - 1], &rg);
+                        collide_ion(&m, s, ty, t->resultant[idx - 1], t->threshold[idx - 1], &rg);
                 }
                 if (counts) counts[((int64_t)s * st->ng + g) * 128 + (idx & 127)] += 1;
                 m.poi.Vz = m.poi.Vz * invVF; m.poi.Vr = m.poi.Vr * invVF; m.poi.Vt = m.poi.Vt * invVF;
@@ -974,9 +986,11 @@ int ora_mcc(const ora_mcc_setup *st, uint64_t seed, uint32_t step,
                         nstcap *= 2;
                         stage = (po_t *)realloc(stage, sizeof(po_t) * nstcap);
                         stage_idx = (int *)realloc(stage_idx, sizeof(int) * nstcap);
+                        stage_twin = (int *)realloc(stage_twin, sizeof(int) * nstcap);
                     }
                     stage[nst] = m.pon[k];
                     stage_idx[nst] = m.pon_idx[k];
+                    stage_twin[nst] = m.pon_twin[k];
                     ++nst;
                 }
             }
@@ -987,12 +1001,18 @@ int ora_mcc(const ora_mcc_setup *st, uint64_t seed, uint32_t step,
         if (d < 0 || d >= st->ns || n[d] >= cap[d]) { rc = -1; continue; }
         po_t p = stage[k];
         p.Vz = p.Vz * invVF; p.Vr = p.Vr * invVF; p.Vt = p.Vt * invVF;
+        /* a twin of another particle of its species made by the same
+         * collision: its Vt pattern's low bit is flipped, so that the two
+         * key different streams afterwards (not in the reference, whose
+         * global random stream separates such twins by itself) */
+        if (stage_twin[k]) p.Vt = flip_low_bit(p.Vt);
         int64_t j = n[d]++;
         P[d][0][j] = p.Z; P[d][1][j] = p.R; P[d][2][j] = p.Vz; P[d][3][j] = p.Vr;
         P[d][4][j] = p.Vt; P[d][5][j] = p.Az; P[d][6][j] = p.Ar; P[d][7][j] = p.WQ;
     }
     free(stage);
     free(stage_idx);
+    free(stage_twin);
     for (int s = 0; s < st->ns && s < 64; ++s) free(keys[s]);
     return rc;
 }

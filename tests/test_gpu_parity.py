@@ -332,6 +332,14 @@ def test_capacity_overflow_raises(rng):
     hp.set_rng(1, 0)
     with pytest.raises(CapacityError):
         hp.mcc()
+    with pytest.raises(CapacityError):       # the report stays until the caller acts on it
+        hp.count(0)
+    # re-uploading smaller populations clears the report: the context recovers
+    for s in range(cfg.ns):
+        hp.upload(s, P[s][:, :1000])
+    hp.set_rng(2, 0)
+    hp.mcc()
+    assert all(hp.count(s) >= 1000 for s in range(cfg.ns))
 
 
 def test_mcc_independent_of_storage_order(rng):

@@ -299,3 +299,51 @@ def test_sigma_raw_energy_below_grid_takes_emin_coordinate(oracle, builder):
     for i in (1, 500, 1500, 2000):
         want = s[0] + (i - emin) * (s[1] - s[0]) / (x2 - emin)
         assert sig[i - 1] == pytest.approx(want, rel=1e-9)
+
+
+def _vt_bits(a):
+    return np.ascontiguousarray(a).view(np.int64)
+
+
+def test_twin_products_get_distinct_streams(oracle, rng):
+    """Dissociation into two fragments of one species (Resultant 22) appends
+    two products that differ only in the low bit of Vt's pattern, so that
+    their state-keyed streams differ afterwards; the same for 113 giving back
+    the parent's species.  Fragments of different species are left as the
+    reference makes them."""
+    e = xsec.log_grid()
+    def electron(res):
+        r = xsec.Reaction(0, 15, res, 1.0, e, np.where(e > 1.0, 1e-19, 0.0))
+        gas = dict(mass=88.0043128 * AMU, temperature=300.0, density=1e23, natom=5.0, beta_max=3.0)
+        b = xsec.build_bundle(xsec.SigmaFile(1, [r]), ME, 1.0, gas, DT)
+        b.collision_ratio = 1.0
+        return b
+    n = 3000
+    P = np.zeros((8, n))
+    P[0], P[1], P[7] = 3.5, 2.5, 1.0
+    d = rng.standard_normal((3, n))
+    P[2:5] = d / np.linalg.norm(d, axis=0) * (math.sqrt(2 * 20.0 * QE / ME) / (DX / DT))
+    for res, twin in ((22, True), (21, False)):
+        st = oracle.MccSetup(DX, DT, [-QE, QE, -QE], [ME, 69 * AMU, 19 * AMU], {(0, 0): electron(res)})
+        out, cnt = oracle.mcc(st, 6, 0, [P, np.zeros((8, 0)), np.zeros((8, 0))], counts=True)
+        k = int(cnt[0, 0, 1])
+        assert k > 100
+        if twin:
+            A, B = out[2][:, 0::2], out[2][:, 1::2]
+            assert A.shape[1] == B.shape[1] == k
+            np.testing.assert_array_equal(np.delete(A, 4, axis=0), np.delete(B, 4, axis=0))
+            assert np.all((_vt_bits(A[4]) ^ _vt_bits(B[4])) == 1)
+        else:
+            np.testing.assert_array_equal(out[1], out[2])      # the same target in both species
+    # 113 with Resultant = the parent's species: the product is the parent but for Vt's low bit
+    b = _reactive_table(0.0, beta_max=1.0)
+    b.rtype = np.array([101, 113], np.int32)
+    b.resultant = np.array([1, 1], np.int32)
+    st = oracle.MccSetup(DX, DT, [-QE, -QE], [ME, 19 * AMU], {(1, 0): b})
+    Q = P.copy()
+    Q[2:5] *= 0.05
+    out, cnt = oracle.mcc(st, 7, 0, [np.zeros((8, 0)), Q], counts=True)
+    assert cnt[1, 0, 2] == n and out[1].shape[1] == 2 * n
+    A, B = out[1][:, :n], out[1][:, n:]
+    np.testing.assert_array_equal(np.delete(A, 4, axis=0), np.delete(B, 4, axis=0))
+    assert np.all((_vt_bits(A[4]) ^ _vt_bits(B[4])) == 1)
