@@ -81,7 +81,7 @@ def cache_key(hp, nparticles):
 
 
 def autotune(hp, nparticles, cache_path=None, bpc_candidates=(0, 2, 4, 6, 8),
-             sort_candidates=(5, 10, 20, 40), steps=6, collide_candidates=(2, 4, 6)):
+             sort_candidates=(5, 10, 20, 40, 60, 80), steps=6, collide_candidates=(2, 4, 6)):
     """Tune (or load from cache) and apply; returns the chosen settings."""
     key = cache_key(hp, nparticles)
     if cache_path and os.path.exists(cache_path):

@@ -14,7 +14,9 @@ Poisson solve is not part of the particle path).  Prints one JSON line.
 
 ``value`` times the step as one captured graph replay (``ipm_step_graph``),
 after a pass that launches the same kernels one by one with HIP events
-around each (``eager_value``, and the kernel times the roofline uses).
+around each (``eager_value``, and the kernel times the roofline uses).  The
+cell sort enters every step at its amortised cost: the sort time measured
+after a full sort interval of drift, divided by the interval.
 """
 import argparse
 import json
@@ -46,18 +48,20 @@ def stream_ceiling(n, reps=5):
     lib.ipm_probe_stream.restype = C.c_int
     arrs = [torch.zeros(n, dtype=torch.float64, device="cuda") for _ in range(8)]
     ptrs = (C.c_void_p * 8)(*[a.data_ptr() for a in arrs])
-    ms = (C.c_double * 2)()
+    ms = (C.c_double * 3)()
     torch.cuda.synchronize()
     rc = lib.ipm_probe_stream(ptrs, n, reps, C.c_void_p(torch.cuda.current_stream().cuda_stream), ms)
     torch.cuda.synchronize()
     del arrs
     if rc != 0:
         raise RuntimeError(f"ipm_probe_stream failed with HIP error {rc}")
-    return {"value": 120.0 * n / (ms[0] * 1e-3) / 1e9, "unit": "GB/s",
+    plain = 120.0 * n / (ms[0] * 1e-3) / 1e9
+    nt = 120.0 * n / (ms[2] * 1e-3) / 1e9
+    return {"value": max(plain, nt), "unit": "GB/s", "plain_GBps": plain, "nontemporal_GBps": nt,
             "copy_GBps": 16.0 * n / (ms[1] * 1e-3) / 1e9,
             "what": f"median of {reps} passes of a kernel that reads 8 and writes 7 arrays of {n} doubles "
-                    f"(the particle pass's streams, no compute), one element per thread; copy_GBps: one-array "
-                    f"device copy (read + write)"}
+                    f"(the particle pass's streams, no compute), one element per thread, with plain and with "
+                    f"non-temporal accesses (value: the faster); copy_GBps: one-array device copy (read + write)"}
 
 
 def live_total(hp, ns):
@@ -65,8 +69,13 @@ def live_total(hp, ns):
 
 
 def timed_steps(hp, cfg, steps, sort_interval, graph=False):
+    """K steps with the cell sort every sort_interval steps.  Returns the
+    wall time, particle-steps, populations before/after and the wall time
+    spent in the sorts (each sort starts by synchronizing the stream, so its
+    own time is bracketed by two synchronizations)."""
     hp.synchronize()
     n0 = live_total(hp, cfg.ns)
+    t_sort = 0.0
     t0 = time.perf_counter()
     for k in range(steps):
         if graph:
@@ -74,11 +83,73 @@ def timed_steps(hp, cfg, steps, sort_interval, graph=False):
         else:
             hp.one_step()
         if sort_interval and (k + 1) % sort_interval == 0:
+            hp.synchronize()
+            ts = time.perf_counter()
             hp.sort(-1)
+            t_sort += time.perf_counter() - ts
     hp.synchronize()
     t1 = time.perf_counter()
     n1 = live_total(hp, cfg.ns)
-    return t1 - t0, 0.5 * (n0 + n1) * steps, n0, n1
+    return t1 - t0, 0.5 * (n0 + n1) * steps, n0, n1, t_sort
+
+
+def sort_after_full_interval(hp, sort_interval):
+    """Wall time of one sort of every species after sort_interval steps of
+    drift from a sorted state (what each sort of a long run costs)."""
+    hp.sort(-1)
+    for _ in range(sort_interval):
+        hp.one_step_graph()
+    hp.synchronize()
+    t = time.perf_counter()
+    hp.sort(-1)
+    return time.perf_counter() - t
+
+
+def host_cpus():
+    """Host CPU description: logical CPUs, the ones this process may use, the
+    cgroup CPU quota (if any) and the model name."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except Exception:
+        info["affinity"] = None
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+            if q != "max":
+                quota = float(q) / float(per)
+    except Exception:
+        pass
+    info["cgroup_cpu_quota"] = quota
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.lower().startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except Exception:
+        pass
+    info["model"] = model
+    return info
+
+
+def native_oracle():
+    """Compile the oracle with -O3 -march=native for this host (same source,
+    contraction off) into a temporary directory; None if that fails."""
+    import subprocess
+    import tempfile
+    src = os.path.join(ROOT, "oracle", "ipm_oracle.c")
+    out = os.path.join(tempfile.mkdtemp(prefix="ipm_oracle_"), "libipm_oracle_native.so")
+    cmd = ["gcc", "-shared", "-fPIC", "-std=c99", "-ffp-contract=off", "-fno-fast-math", "-O3", "-march=native",
+           src, "-o", out, "-lm"]
+    try:
+        subprocess.run(cmd, check=True, capture_output=True, timeout=120)
+        return out
+    except Exception as e:
+        log(f"[bench] -march=native oracle build failed ({e}); using the prebuilt -O3 oracle")
+        return None
 
 
 def plain_steps(hp, cfg, steps):
@@ -131,14 +202,20 @@ def _oracle_worker(w):
 
 def cpu_baseline(cfg, geom, tables, particles, seconds, workers):
     """The oracle (statement-by-statement restatement of the reference
-    routines) on the host: one core on a ~2M-particle sample, then `workers`
-    processes each owning an interleaved 1/workers share of the whole
-    workload (the reference's MPI ranks each own their particles; the
-    per-step grid reduction between ranks is not included).  Runs before any
-    GPU initialisation, so the workers can be forked."""
+    routines), compiled -O3 -march=native on this host, timed on the host:
+    one core on a ~2M-particle sample, then `workers` processes each owning
+    an interleaved 1/workers share of the whole workload (the reference's
+    MPI ranks each own their particles; the per-step grid reduction between
+    ranks is not included).  Runs before any GPU initialisation, so the
+    workers can be forked."""
     global _SHARE
     import multiprocessing as mp
+    from oracle import ipm_oracle as O
     from ipm2d import workload as W
+    native = native_oracle()
+    if native:
+        O.use_timed_library(native)
+    build = "-O3 -march=native oracle" if native else "-O3 -march=x86-64-v2 oracle"
     total = sum(P.shape[1] for P in particles)
     ez, er = W.fields(cfg, 0.0)
     stride = max(1, total // 2_000_000)
@@ -146,7 +223,7 @@ def cpu_baseline(cfg, geom, tables, particles, seconds, workers):
     ps1, n1, el1 = _oracle_run(cfg, geom, tables, sample, ez, er, seconds, 100)
     out = dict(value=ps1 / el1, unit="particle-steps/s", cores=1, kind="port",
                sample=f"every {stride}th particle of the same workload ({sum(P.shape[1] for P in sample)} "
-                      f"particles), {n1} steps of push+absorb+deposit+MCC in {el1:.1f} s, -O3 oracle, one core")
+                      f"particles), {n1} steps of push+absorb+deposit+MCC in {el1:.1f} s, {build}, one core")
     if workers > 1:
         _SHARE = (cfg, geom, tables, particles, ez, er, seconds, workers)
         with mp.get_context("fork").Pool(workers) as pool:
@@ -157,7 +234,7 @@ def cpu_baseline(cfg, geom, tables, particles, seconds, workers):
         out = dict(value=ps / el, unit="particle-steps/s", cores=workers, kind="port",
                    sample=f"the whole workload ({total} particles) split over {workers} processes, "
                           f"{min(r[1] for r in res)}-{max(r[1] for r in res)} steps each of "
-                          f"push+absorb+deposit+MCC in {el:.1f} s, -O3 oracle",
+                          f"push+absorb+deposit+MCC in {el:.1f} s, {build}",
                    single_core_value=ps1 / el1, single_core_sample=out["sample"])
     return out
 
@@ -265,10 +342,18 @@ def main():
     log(f"[bench] {cfg.name}: {cfg.nz}x{cfg.nr} nodes, {ntot} particles, host load {time.perf_counter() - t:.1f}s")
     cpu = None
     if not args.no_cpu_baseline:
-        workers = args.cpu_workers or min(16, os.cpu_count() or 1)
-        log(f"[bench] cpu baseline (1 core, then {workers} processes) ...")
+        host = host_cpus()
+        usable = host["affinity"] or host["nproc"] or 1
+        if host["cgroup_cpu_quota"]:
+            usable = min(usable, max(1, int(host["cgroup_cpu_quota"])))
+        # one GPU's share of a shared host is 16 cores
+        workers = args.cpu_workers or min(16, usable)
+        log(f"[bench] cpu baseline (1 core, then {workers} processes) on {host}")
         cpu = cpu_baseline(cfg, geom, tables, particles, args.cpu_seconds, workers)
-        log(f"[bench] cpu {cpu['value']:.4g} particle-steps/s on {cpu['cores']} cores")
+        cpu["host"] = host
+        cpu["cores_of_host"] = (f"{cpu['cores']} of {host['nproc']} logical CPUs ({host['affinity']} in this "
+                                f"process's affinity, cgroup quota {host['cgroup_cpu_quota']}), {host['model']}")
+        log(f"[bench] cpu {cpu['value']:.4g} particle-steps/s on {cpu['cores_of_host']}")
     hp, _, _, _ = ipm2d.from_config(cfg, geom=geom, tables=tables, particles=particles, capacity_factor=1.25)
     tuned = None
     if not args.no_autotune and not args.traffic_probe and args.sort_interval < 0 and args.blocks_per_cu < 0:
@@ -294,21 +379,31 @@ def main():
     # per-kernel launch times: the step launched kernel by kernel with HIP
     # events around each launch on the library's stream
     hp.set_timing(True)
-    el_e, psteps, n0, n1 = timed_steps(hp, cfg, args.steps, sort_interval)
+    el_e, psteps, n0, n1, ts_e = timed_steps(hp, cfg, args.steps, sort_interval)
     tm = hp.take_timing()
     hp.set_timing(False)
-    eager_value = psteps / el_e
-    log(f"[bench] eager: {args.steps} steps in {el_e:.3f}s -> {eager_value:.4g} particle-steps/s; kernels {tm}")
+    log(f"[bench] eager: {args.steps} steps in {el_e:.3f}s ({ts_e:.4f}s of it sorting); kernels {tm}")
     if args.eager or args.traffic_probe:
-        el, value = el_e, eager_value
+        el, ps_g, ts_g = el_e, psteps, ts_e
     else:
         # the headline: the whole step replayed as one captured graph, the
         # capture and instantiation done before the timed region
         hp.one_step_graph()
         hp.synchronize()
-        el, ps_g, _, _ = timed_steps(hp, cfg, args.steps, sort_interval, graph=True)
-        value = ps_g / el
-        log(f"[bench] graph: {args.steps} steps in {el:.3f}s -> {value:.4g} particle-steps/s")
+        el, ps_g, _, _, ts_g = timed_steps(hp, cfg, args.steps, sort_interval, graph=True)
+        log(f"[bench] graph: {args.steps} steps in {el:.3f}s ({ts_g:.4f}s of it sorting)")
+    # The cell sort runs every sort_interval steps; a timed region of K steps
+    # holds floor(K / interval) of them, which over- or under-weights the
+    # sort unless K is a multiple of the interval.  The step time reported is
+    # therefore the timed steps without their sorts plus the cost of one sort
+    # after a full interval of drift, divided by the interval.
+    t_sort = sort_after_full_interval(hp, sort_interval) if (sort_interval and not args.traffic_probe) else 0.0
+    amort = t_sort / sort_interval if sort_interval else 0.0
+    step_s = (el - ts_g) / args.steps + amort
+    value = ps_g / args.steps / step_s
+    eager_value = psteps / args.steps / ((el_e - ts_e) / args.steps + amort)
+    log(f"[bench] sort after {sort_interval} steps of drift {t_sort * 1e3:.3f} ms; step {step_s * 1e3:.4f} ms "
+        f"-> {value:.4g} particle-steps/s (eager {eager_value:.4g})")
 
     pk_ms, pk_n = tm["particle"]
     avg_s = pk_ms / pk_n / 1e3
@@ -352,7 +447,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": el / args.steps * 1e3,
+        "ms_per_step": step_s * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -384,6 +479,18 @@ def main():
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "stream_ceiling": ceiling,
         },
+        "step_roofline": {
+            "achieved": bytes_per_launch / step_s / 1e9,
+            "frac": bytes_per_launch / step_s / 1e9 / HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "what": "the whole step (particle pass, collision pass, grid epilogue, launch gaps and the amortised "
+                    "sort): the particle pass's algorithmic bytes over ms_per_step",
+        },
+        "sort": {"interval": sort_interval, "ms_after_full_interval": t_sort * 1e3,
+                 "ms_per_step_amortised": amort * 1e3,
+                 "sorts_in_timed_region": (args.steps // sort_interval) if sort_interval else 0,
+                 "timed_region_ms_per_step": el / args.steps * 1e3,
+                 "timed_region_sort_ms": ts_g * 1e3},
         "kernel_ms_per_step": {k: (v[0] / v[1] if v[1] else None) for k, v in tm.items()},
         "cpu_baseline": cpu,
         "adjoint_over_forward": adj["electrode_flux"]["adjoint_over_forward"] if adj else None,

@@ -39,10 +39,18 @@ class OraSetup(C.Structure):
 
 
 _libs = {}
+_timed_path = None
+
+
+def use_timed_library(path):
+    """Point lib(fast=True) -- the build the CPU baseline times -- at another
+    build of the same source (bench.py compiles one for the host it runs on)."""
+    global _timed_path
+    _timed_path = path
 
 
 def lib(fast=False):
-    path = LIB_FAST if fast else LIB
+    path = (_timed_path or LIB_FAST) if fast else LIB
     if path in _libs:
         return _libs[path]
     if not os.path.exists(path):
