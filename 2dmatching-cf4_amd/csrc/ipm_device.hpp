@@ -148,21 +148,41 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
 }
 
 // Per-particle random stream.  The counter carries 96 bits folded from the
-// particle's phase-space state at the start of the MCC pass plus the draw
+// particle's phase-space state at the start of the MCC pass plus the block
 // index; the key carries the seed, the step and the (species, gas) tag.  The
 // stream therefore depends on what the particle is, not on where it is
 // stored: sorting, compaction and the order in which products are appended
-// do not change any draw.
+// do not change any draw.  One Philox block gives two draws: draw k is the
+// top 52 bits of words (1,0) of block k/2 for even k and of words (3,2) for
+// odd k, so a run of draws costs one block per two draws (the target
+// sampler's rejection loop draws in pairs).
 struct Rng {
     uint32_t c0, c1, c2, k;
     uint32_t key0, key1;
+    uint64_t spare;     // draw k's 52 bits when k is odd and have is set
+    bool have;
     __device__ __forceinline__ double next()
     {
-        uint32_t c[4] = {c0, c1, c2, k};
+        uint64_t x;
+        if ((k & 1u) && have) {
+            x = spare;
+        } else {
+            uint32_t c[4] = {c0, c1, c2, k >> 1};
+            philox4x32_10(c, key0, key1);
+            const uint64_t lo = ((((uint64_t)c[1]) << 32) | c[0]) >> 12;   // 52 bits
+            const uint64_t hi = ((((uint64_t)c[3]) << 32) | c[2]) >> 12;
+            x = (k & 1u) ? hi : lo;
+            spare = hi;
+            have = true;
+        }
         ++k;
-        philox4x32_10(c, key0, key1);
-        uint64_t x = ((((uint64_t)c[1]) << 32) | c[0]) >> 12;   // 52 bits
         return ((double)x + 0.5) * 0x1p-52;                     // (0,1)
+    }
+    // continue the stream at draw k (nothing of the current block is kept)
+    __device__ __forceinline__ void seek(uint32_t draw)
+    {
+        k = draw;
+        have = false;
     }
 };
 
@@ -188,6 +208,7 @@ __device__ __forceinline__ Rng make_rng(const StateKey &sk, unsigned long long s
 {
     Rng g;
     g.c0 = sk.c0; g.c1 = sk.c1; g.c2 = sk.c2; g.k = 0;
+    g.spare = 0; g.have = false;
     g.key0 = (uint32_t)seed + step * 0x9E3779B9u;
     g.key1 = (uint32_t)(seed >> 32) ^ (tag * 0x85EBCA6Bu);
     return g;

@@ -668,7 +668,7 @@ __device__ __forceinline__ void finish_records(const DevState &st, DevCounters *
         StateKey sk;
         sk.c0 = (uint32_t)w0; sk.c1 = (uint32_t)(w0 >> 32); sk.c2 = (uint32_t)w1;
         Rng rg = make_rng(sk, seed, step, (uint32_t)s | ((uint32_t)g << 8));
-        rg.k = (uint32_t)(w1 >> 32);
+        rg.seek((uint32_t)(w1 >> 32));
         Part p;
         mcc_finish(st, ctr, s, g, t0, m, rg, idx, p, &sink);
         for (int g2 = g + 1; g2 < st.ng; ++g2) {
@@ -802,7 +802,7 @@ collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
                     if (tb.model == 0) continue;
                     Rng rg = make_rng(sk, seed, step, (uint32_t)s | ((uint32_t)g << 8));
                     if (st.ng == 1) {
-                        rg.k = 1;   // the particle pass drew this same uniform and selected it
+                        rg.seek(1);   // the particle pass drew this same uniform and selected it
                     } else if (!(rg.next() < tb.ratio)) {
                         continue;
                     }

@@ -68,11 +68,34 @@ def live_total(hp, ns):
     return sum(hp.count(s) for s in range(ns))
 
 
-def timed_steps(hp, cfg, steps, sort_interval, graph=False):
-    """K steps with the cell sort every sort_interval steps.  Returns the
-    wall time, particle-steps, populations before/after and the wall time
-    spent in the sorts (each sort starts by synchronizing the stream, so its
-    own time is bracketed by two synchronizations)."""
+def sort_phase(steps, sort_interval):
+    """Steps after a sort at which a timed region of `steps` steps starts, so
+    that the region sits in the middle of the sort cycle: the particle pass
+    slows as particles drift out of cell order between sorts, so a region
+    that began right after a sort would time the fastest part of the cycle
+    only (and one that began late, the slowest)."""
+    if not sort_interval:
+        return 0
+    return (sort_interval - steps % sort_interval) // 2 if steps % sort_interval else 0
+
+
+def start_phase(hp, phase, graph=False):
+    """Sort, then advance `phase` untimed steps."""
+    hp.sort(-1)
+    for _ in range(phase):
+        if graph:
+            hp.one_step_graph()
+        else:
+            hp.one_step()
+    hp.synchronize()
+
+
+def timed_steps(hp, cfg, steps, sort_interval, graph=False, phase=0):
+    """K steps with the cell sort every sort_interval steps, the first step
+    being step `phase` after a sort.  Returns the wall time, particle-steps,
+    populations before/after and the wall time spent in the sorts (each sort
+    starts by synchronizing the stream, so its own time is bracketed by two
+    synchronizations)."""
     hp.synchronize()
     n0 = live_total(hp, cfg.ns)
     t_sort = 0.0
@@ -82,7 +105,7 @@ def timed_steps(hp, cfg, steps, sort_interval, graph=False):
             hp.one_step_graph()
         else:
             hp.one_step()
-        if sort_interval and (k + 1) % sort_interval == 0:
+        if sort_interval and (phase + k + 1) % sort_interval == 0:
             hp.synchronize()
             ts = time.perf_counter()
             hp.sort(-1)
@@ -378,8 +401,11 @@ def main():
 
     # per-kernel launch times: the step launched kernel by kernel with HIP
     # events around each launch on the library's stream
+    phase = sort_phase(args.steps, sort_interval) if not args.traffic_probe else 0
+    if not args.traffic_probe:
+        start_phase(hp, phase)
     hp.set_timing(True)
-    el_e, psteps, n0, n1, ts_e = timed_steps(hp, cfg, args.steps, sort_interval)
+    el_e, psteps, n0, n1, ts_e = timed_steps(hp, cfg, args.steps, sort_interval, phase=phase)
     tm = hp.take_timing()
     hp.set_timing(False)
     log(f"[bench] eager: {args.steps} steps in {el_e:.3f}s ({ts_e:.4f}s of it sorting); kernels {tm}")
@@ -390,7 +416,8 @@ def main():
         # capture and instantiation done before the timed region
         hp.one_step_graph()
         hp.synchronize()
-        el, ps_g, _, _, ts_g = timed_steps(hp, cfg, args.steps, sort_interval, graph=True)
+        start_phase(hp, phase, graph=True)
+        el, ps_g, _, _, ts_g = timed_steps(hp, cfg, args.steps, sort_interval, graph=True, phase=phase)
         log(f"[bench] graph: {args.steps} steps in {el:.3f}s ({ts_g:.4f}s of it sorting)")
     # The cell sort runs every sort_interval steps; a timed region of K steps
     # holds floor(K / interval) of them, which over- or under-weights the
@@ -488,7 +515,8 @@ def main():
         },
         "sort": {"interval": sort_interval, "ms_after_full_interval": t_sort * 1e3,
                  "ms_per_step_amortised": amort * 1e3,
-                 "sorts_in_timed_region": (args.steps // sort_interval) if sort_interval else 0,
+                 "sorts_in_timed_region": ((phase + args.steps) // sort_interval) if sort_interval else 0,
+                 "timed_region_starts_steps_after_sort": phase,
                  "timed_region_ms_per_step": el / args.steps * 1e3,
                  "timed_region_sort_ms": ts_g * 1e3},
         "kernel_ms_per_step": {k: (v[0] / v[1] if v[1] else None) for k, v in tm.items()},

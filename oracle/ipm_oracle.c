@@ -86,16 +86,18 @@ void ora_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint3
 }
 
 /* Draw `draw` of the stream whose 96-bit counter prefix is (c0, c1, c2):
- * counter {c0, c1, c2, draw}, key {seed_lo + step*0x9E3779B9,
- * seed_hi ^ tag*0x85EBCA6B}, 52 random bits -> (k + 1/2) 2^-52 in (0,1). */
+ * block {c0, c1, c2, draw/2}, key {seed_lo + step*0x9E3779B9,
+ * seed_hi ^ tag*0x85EBCA6B}; the top 52 bits of words (1,0) for an even
+ * draw, of words (3,2) for an odd one -> (k + 1/2) 2^-52 in (0,1). */
 double ora_uniform(uint64_t seed, uint32_t step, uint32_t tag, uint32_t c0, uint32_t c1, uint32_t c2,
                    uint32_t draw)
 {
-    uint32_t ctr[4] = {c0, c1, c2, draw};
+    uint32_t ctr[4] = {c0, c1, c2, draw >> 1};
     uint32_t key[2] = {(uint32_t)seed + step * 0x9E3779B9u, (uint32_t)(seed >> 32) ^ (tag * 0x85EBCA6Bu)};
     uint32_t o[4];
     ora_philox4x32_10(ctr, key, o);
-    uint64_t x = ((((uint64_t)o[1]) << 32) | o[0]) >> 12;   /* 52 bits */
+    uint64_t x = (draw & 1u) ? ((((uint64_t)o[3]) << 32) | o[2]) >> 12    /* 52 bits */
+                             : ((((uint64_t)o[1]) << 32) | o[0]) >> 12;
     return ((double)x + 0.5) * 0x1p-52;                    /* in (0,1) */
 }
 

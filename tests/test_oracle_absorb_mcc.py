@@ -244,3 +244,18 @@ def test_state_keys_distinguish_parent_and_secondary(oracle):
     a = oracle.state_key(1.5, 2.5, 0.1, 0.2, 0.3)
     b = oracle.state_key(1.5, 2.5, 0.1 * 0.7, 0.2 * 0.7, 0.3 * 0.7)
     assert a != b
+
+
+def test_paired_draws_are_independent_uniforms(oracle):
+    """Draws 2j and 2j+1 come from one Philox block (words 1,0 and 3,2): the
+    pairs fill the unit square evenly, and consecutive blocks differ."""
+    n = 40000
+    a = np.array([oracle.uniform(11, 2, 3, i, 7, 9, 0) for i in range(n)])
+    b = np.array([oracle.uniform(11, 2, 3, i, 7, 9, 1) for i in range(n)])
+    c = np.array([oracle.uniform(11, 2, 3, i, 7, 9, 2) for i in range(n)])
+    h, _, _ = np.histogram2d(a, b, bins=10, range=[[0, 1], [0, 1]])
+    exp = n / 100
+    assert ((h - exp) ** 2 / exp).sum() < 180.0             # chi2(99) far tail
+    assert abs(np.corrcoef(a, b)[0, 1]) < 6 / math.sqrt(n)
+    assert abs(np.corrcoef(b, c)[0, 1]) < 6 / math.sqrt(n)
+    assert not np.any(a == b) and not np.any(b == c)
