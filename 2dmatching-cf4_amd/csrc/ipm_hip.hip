@@ -1674,6 +1674,28 @@ int ipm_upload_species(ipm_ctx *c, int s, int64_t n, const double *po, int layou
     return IPM_OK;
 }
 
+int ipm_upload_species_device(ipm_ctx *c, int s, int64_t offset, int64_t n, const double *po_dev, int64_t ld)
+{
+    if (!c || s < 0 || s >= c->ns || !c->species_set[s]) return fail(IPM_EINVAL, "invalid species %d", s);
+    if (offset < 0 || n < 0 || ld < n) return fail(IPM_EINVAL, "invalid offset %lld / count %lld / leading dimension %lld",
+                                                 (long long)offset, (long long)n, (long long)ld);
+    if (offset + n > c->h.cap[s])
+        return fail(IPM_ECAPACITY, "upload of slots [%lld, %lld) exceeds capacity %lld", (long long)offset,
+                    (long long)(offset + n), (long long)c->h.cap[s]);
+    if (n > 0 && !po_dev) return fail(IPM_EINVAL, "null particle array");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int f = 0; f < 8 && n > 0; ++f)
+        HIPCHK(hipMemcpyAsync(c->h.f[s][f] + offset, po_dev + (size_t)f * (size_t)ld, sizeof(double) * (size_t)n,
+                              hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    DevCounters ctr;
+    HIPCHK(hipMemcpy(&ctr, c->d_ctr, sizeof ctr, hipMemcpyDeviceToHost));
+    ctr.n[s] = (unsigned long long)(offset + n); ctr.n_new[s] = 0; ctr.n_dead[s] = 0;
+    ctr.overflow = 0;
+    HIPCHK(hipMemcpy(c->d_ctr, &ctr, sizeof ctr, hipMemcpyHostToDevice));
+    return IPM_OK;
+}
+
 int ipm_download_species(ipm_ctx *c, int s, int64_t *n_out, double *po, int64_t cap, int layout)
 {
     if (!c || s < 0 || s >= c->ns || !c->species_set[s] || !n_out) return fail(IPM_EINVAL, "invalid species %d", s);

@@ -44,6 +44,7 @@ SIGNATURES = {
     "ipm_set_species": [_vp, C.c_int, C.c_double, C.c_double, C.c_int64],
     "ipm_set_geometry": [_vp, _dp, _ip, _ip, _ip, C.c_int, C.c_int, _ip],
     "ipm_upload_species": [_vp, C.c_int, C.c_int64, _dp, C.c_int],
+    "ipm_upload_species_device": [_vp, C.c_int, C.c_int64, C.c_int64, _vp, C.c_int64],
     "ipm_download_species": [_vp, C.c_int, _i64p, _dp, C.c_int64, C.c_int],
     "ipm_count": [_vp, C.c_int, _i64p],
     "ipm_set_fields": [_vp, _dp, _dp],

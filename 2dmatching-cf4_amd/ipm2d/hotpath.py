@@ -123,6 +123,12 @@ class HotPath:
         assert aos.ndim == 2 and aos.shape[1] == 8
         N.call("ipm_upload_species", self._h, s, aos.shape[0], _d(aos), N.LAYOUT_AOS)
 
+    def upload_device(self, s, offset, n, ptr, ld):
+        """Slots [offset, offset + n) of species s from device memory: 8 SoA
+        arrays of leading dimension ld at device address ptr (a float64
+        tensor's data_ptr(); the producer's stream must be synchronized)."""
+        N.call("ipm_upload_species_device", self._h, s, int(offset), int(n), C.c_void_p(int(ptr)), int(ld))
+
     def count(self, s):
         n = C.c_int64()
         N.call("ipm_count", self._h, s, C.byref(n))
@@ -320,3 +326,30 @@ def from_config(cfg, geom=None, tables=None, capacity_factor=1.5, rng=None, part
     hp.set_fields(ez, er)
     hp.set_rng(cfg.seed, 0)
     return hp, geom, tables, particles
+
+
+def from_config_device(cfg, geom=None, tables=None, capacity_factor=1.05, seed=None):
+    """from_config with the particles made on the GPU (workload.load_species_device,
+    species s seeded by seed + s) and copied in device to device: for
+    workloads too large to build on the host.  Capacities are the expected
+    counts times capacity_factor plus a margin.  Returns (hp, geom, tables,
+    counts)."""
+    from . import workload as W
+    geom = geom if geom is not None else W.make_geometry(cfg)
+    tables = tables if tables is not None else W.build_tables(cfg)
+    seed = cfg.seed if seed is None else seed
+    hp = HotPath(cfg.nz, cfg.nr, cfg.dx, cfg.dt, cfg.ns, 1)
+    for s in range(cfg.ns):
+        q, m = W.species_params(cfg, s)
+        mean = W.expected_count(cfg, geom, s)
+        hp.set_species(s, q, m, int(mean * capacity_factor + 8.0 * np.sqrt(mean + 1.0)) + 4096)
+    hp.set_geometry(geom)
+    for s in range(cfg.ns):
+        hp.set_mcc(s, 0, tables.get((s, 0)))
+    counts = [W.load_species_device(cfg, geom, s, hp, seed * 1000 + s) for s in range(cfg.ns)]
+    import torch
+    torch.cuda.empty_cache()          # the staging chunks
+    ez, er = W.fields(cfg, 0.0)
+    hp.set_fields(ez, er)
+    hp.set_rng(cfg.seed, 0)
+    return hp, geom, tables, counts

@@ -181,6 +181,72 @@ def load_species(cfg, geom, s, rng, max_particles=None):
     return out
 
 
+def expected_count(cfg, geom, s):
+    """Mean particle count load_species gives species s (for capacities)."""
+    return int(geom.cell_load.astype(bool).sum()) * cfg.species[s].ppc
+
+
+def load_species_device(cfg, geom, s, hp, seed, chunk_cells=1 << 15):
+    """Species s loaded on the GPU straight into hp's device arrays, drawn
+    from the distribution load_species uses (cell-sorted with z fastest,
+    floor(ppc) particles per loadable cell plus one more with probability
+    frac(ppc), uniform in volume inside the cell, the cell weight
+    n_s(centre) pi dx^3 (2j-1)/ppc, Maxwellian velocities, Az = Ar = 0) with
+    torch's device generator seeded by `seed`: the way to set up workloads
+    too large to build on the host (1.0e9 particles).  Cells are filled in
+    chunks of chunk_cells, each staged in a device tensor and copied in by
+    ipm_upload_species_device.  Returns the particle count."""
+    import torch
+    dev = torch.device("cuda")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(int(seed))
+    sp = cfg.species[s]
+    dx = cfg.dx
+    nzc = cfg.nz - 1
+    load = torch.as_tensor(geom.cell_load.astype(bool).ravel(order="F"), device=dev)
+    cells = torch.nonzero(load).squeeze(1)                 # z-fastest cell index
+    base = int(math.floor(sp.ppc))
+    frac = sp.ppc - base
+    mass = sp.mass_amu * AMU
+    vth = math.sqrt(KB * sp.temperature_ev * EV_TO_K / mass) / (dx / cfg.dt)
+    L, R = cfg.length, cfg.radius
+    offset = 0
+    for c0 in range(0, cells.numel(), chunk_cells):
+        ci = cells[c0:c0 + chunk_cells]
+        iz = ci % nzc
+        ir = ci // nzc
+        counts = torch.full((ci.numel(),), base, dtype=torch.int64, device=dev)
+        if frac > 0:
+            counts += (torch.rand(ci.numel(), generator=gen, device=dev, dtype=torch.float64) < frac).to(torch.int64)
+        n = int(counts.sum().item())
+        if n == 0:
+            continue
+        cz = torch.repeat_interleave(iz, counts).to(torch.float64)
+        cr = torch.repeat_interleave(ir, counts).to(torch.float64)
+        zc = (iz.to(torch.float64) + 0.5) * dx
+        rc = (ir.to(torch.float64) + 0.5) * dx
+        if cfg.profile == "uniform":
+            prof = torch.ones_like(zc)
+        else:
+            prof = torch.cos(math.pi * rc / (2.0 * R)) ** 2 * torch.cos(math.pi * (zc / L - 0.5)) ** 2
+        wcell = cfg.n0 * sp.density * prof * math.pi * dx ** 3 * (2.0 * (ir.to(torch.float64) + 1.0) - 1.0) \
+            / max(sp.ppc, 1e-30)
+        buf = torch.empty((8, n), dtype=torch.float64, device=dev)
+        buf[0] = cz + torch.rand(n, generator=gen, device=dev, dtype=torch.float64)
+        j = cr + 1.0
+        r = torch.sqrt(torch.rand(n, generator=gen, device=dev, dtype=torch.float64) * (j * j - (j - 1.0) ** 2)
+                       + (j - 1.0) ** 2)
+        buf[1] = torch.minimum(torch.maximum(r, torch.nextafter(cr, torch.full_like(cr, math.inf))), j)
+        buf[2:5] = torch.randn((3, n), generator=gen, device=dev, dtype=torch.float64) * vth
+        buf[5:7] = 0.0
+        buf[7] = torch.repeat_interleave(wcell, counts)
+        torch.cuda.synchronize()
+        hp.upload_device(s, offset, n, buf.data_ptr(), n)
+        offset += n
+        del buf, cz, cr, r, j
+    return offset
+
+
 def species_params(cfg, s):
     sp = cfg.species[s]
     return sp.charge * QE, sp.mass_amu * AMU

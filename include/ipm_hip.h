@@ -68,6 +68,14 @@ int ipm_set_geometry(ipm_ctx *ctx, const double *inv_volume, const int *cell_mat
  * An upload clears the device's pending error reports (capacity, queue),
  * so a context recovers from IPM_ECAPACITY by re-uploading its species. */
 int ipm_upload_species(ipm_ctx *ctx, int s, int64_t n, const double *po, int layout);
+/* Copies n particles that already live in device memory -- 8 SoA arrays
+ * with leading dimension ld, field f of particle i at po_dev[f*ld + i] --
+ * into slots [offset, offset + n) of species s, and sets its count to
+ * offset + n (slots below offset keep what an earlier call put there).
+ * No counterpart in the reference: for callers whose particles are made or
+ * kept on the GPU, such as device-side loading of runs too large to build
+ * on the host.  Clears the pending error reports like ipm_upload_species. */
+int ipm_upload_species_device(ipm_ctx *ctx, int s, int64_t offset, int64_t n, const double *po_dev, int64_t ld);
 /* Downloads the live particles (dead slots dropped) into po (capacity cap). */
 int ipm_download_species(ipm_ctx *ctx, int s, int64_t *n, double *po, int64_t cap, int layout);
 /* Live particle count (ParticleBundle%NPar). */
