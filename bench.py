@@ -321,6 +321,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-workers", type=int, default=0, help="0: min(16, cores)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--device-load", action="store_true",
+                    help="make the particles on the GPU (automatic above 2.5e8 particles)")
     ap.add_argument("--no-extra", action="store_true", help="skip the plain-launch comparison")
     ap.add_argument("--eager", action="store_true",
                     help="report the kernel-by-kernel launches instead of the captured graph replay")
@@ -360,9 +362,22 @@ def main():
     geom = W.make_geometry(cfg)
     tables = W.build_tables(cfg)
     rng = np.random.default_rng(cfg.seed)
-    particles = [W.load_species(cfg, geom, s, rng) for s in range(cfg.ns)]
-    ntot = sum(P.shape[1] for P in particles)
-    log(f"[bench] {cfg.name}: {cfg.nz}x{cfg.nr} nodes, {ntot} particles, host load {time.perf_counter() - t:.1f}s")
+    expected = sum(W.expected_count(cfg, geom, s) for s in range(cfg.ns))
+    device_load = args.device_load or expected > 2.5e8
+    if device_load:
+        # too large to build on the host in reasonable time: the particles are
+        # made on the GPU; the CPU baseline runs on a host-built sample of the
+        # same distribution (every species' per-cell population / K)
+        import dataclasses
+        k = max(1, int(round(expected / 2.0e7)))
+        sample_cfg = dataclasses.replace(cfg, species=[dataclasses.replace(sp, ppc=sp.ppc / k) for sp in cfg.species])
+        particles = [W.load_species(sample_cfg, geom, s, rng) for s in range(cfg.ns)]
+        cpu_cfg = sample_cfg
+    else:
+        particles = [W.load_species(cfg, geom, s, rng) for s in range(cfg.ns)]
+        cpu_cfg = cfg
+    log(f"[bench] {cfg.name}: {cfg.nz}x{cfg.nr} nodes, {expected:.4g} particles expected, host load "
+        f"{time.perf_counter() - t:.1f}s{' (device load)' if device_load else ''}")
     cpu = None
     if not args.no_cpu_baseline:
         host = host_cpus()
@@ -372,12 +387,23 @@ def main():
         # one GPU's share of a shared host is 16 cores
         workers = args.cpu_workers or min(16, usable)
         log(f"[bench] cpu baseline (1 core, then {workers} processes) on {host}")
-        cpu = cpu_baseline(cfg, geom, tables, particles, args.cpu_seconds, workers)
+        cpu = cpu_baseline(cpu_cfg, geom, tables, particles, args.cpu_seconds, workers)
+        if device_load:
+            cpu["sample"] += f" (a 1/{k} per-cell sample of the workload, built on the host)"
         cpu["host"] = host
         cpu["cores_of_host"] = (f"{cpu['cores']} of {host['nproc']} logical CPUs ({host['affinity']} in this "
                                 f"process's affinity, cgroup quota {host['cgroup_cpu_quota']}), {host['model']}")
         log(f"[bench] cpu {cpu['value']:.4g} particle-steps/s on {cpu['cores_of_host']}")
-    hp, _, _, _ = ipm2d.from_config(cfg, geom=geom, tables=tables, particles=particles, capacity_factor=1.25)
+    if device_load:
+        del particles
+        t = time.perf_counter()
+        hp, _, _, counts = ipm2d.from_config_device(cfg, geom=geom, tables=tables, capacity_factor=1.3)
+        ntot = sum(counts)
+        log(f"[bench] device load of {ntot} particles in {time.perf_counter() - t:.1f}s")
+    else:
+        ntot = sum(P.shape[1] for P in particles)
+        hp, _, _, _ = ipm2d.from_config(cfg, geom=geom, tables=tables, particles=particles, capacity_factor=1.25)
+        del particles
     tuned = None
     if not args.no_autotune and not args.traffic_probe and args.sort_interval < 0 and args.blocks_per_cu < 0:
         # the tuner advances the live simulation, so it also warms up
@@ -458,7 +484,7 @@ def main():
 
     ceiling = None
     if not args.traffic_probe:
-        ceiling = stream_ceiling(ntot)
+        ceiling = stream_ceiling(min(ntot, 200_000_000))
         ceiling["frac_of_ceiling"] = achieved / ceiling["value"]
         log(f"[bench] streaming ceiling {ceiling['value']:.0f} GB/s (copy {ceiling['copy_GBps']:.0f}); "
             f"particle kernel at {ceiling['frac_of_ceiling']:.3f} of it")
