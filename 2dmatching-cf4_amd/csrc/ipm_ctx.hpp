@@ -69,6 +69,7 @@ struct ipm_ctx {
     int *tanc = nullptr;                   // q_ntiles deposit-window anchors (z, r)
     long long q_cap = 0, q_ntiles = 0;
     double *mom = nullptr;                 // moment sums, 5 * nz * nr (ipm_moments)
+    double *scratch = nullptr;             // this device's scratch array (ipm_moments_noalloc)
     double *phi = nullptr;                 // potential (ipm_set_potential), nz * nr
     unsigned long long *rezone = nullptr;  // split/merge tallies (ipm_split_merge), 3
     hipGraphExec_t graph = nullptr;

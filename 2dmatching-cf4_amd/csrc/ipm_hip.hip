@@ -18,6 +18,7 @@
 #include <cmath>
 #include <hipcub/block/block_radix_sort.hpp>
 #include <hipcub/block/block_scan.hpp>
+#include <mutex>
 #include "ipm_device.hpp"
 
 using namespace ipm;
@@ -1357,10 +1358,38 @@ __global__ void set_count_kernel(DevCounters *ctr, int s, const unsigned long lo
 
 }  // namespace
 
+// ------------------------------------------------- no-allocation scratch
+// Accumulators of ipm_moments_noalloc: five node grids of at most
+// kScratchNodes nodes (40 MiB).  Each GPU the library runs on holds its own
+// copy for the life of the process; every call on a GPU shares it, so calls
+// that overlap on one GPU race.  Host code reaches it only through the
+// address hipGetSymbolAddress returns for the current device.
+constexpr size_t kScratchNodes = (size_t)1 << 20;
+static __device__ __align__(256) double scratch[5 * kScratchNodes];
+
 // ====================================================================
 //                              host side
 // ====================================================================
 #include "ipm_ctx.hpp"
+
+// Device address of the scratch array on device dev (the current device),
+// looked up once per device and kept.
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[64] = {};
+static int scratch_address(int dev, double **out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (dev < 0 || dev >= 64) return fail(IPM_EINVAL, "device %d beyond the scratch table", dev);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(IPM_EHIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = (double *)g_scratch_ptr[dev];
+    return IPM_OK;
+}
 
 // Collision-queue regions: species s gets ceil(cap_s/1024) tile regions of
 // R_s = min(1024, ceil(1024 * min(1, 2 * sum_g CollisionRatio)) + 64)
@@ -1503,6 +1532,10 @@ int ipm_create(ipm_ctx **out, int nz, int nr, double dx, double dt, int ns, int 
     HIPCHK(hipGetDeviceProperties(&prop, c->device));
     c->num_cu = prop.multiProcessorCount;
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    {
+        const int rc = scratch_address(c->device, &c->scratch);
+        if (rc) { ipm_destroy(c); return rc; }
+    }
     c->nz = nz; c->nr = nr; c->ns = ns; c->ng = ng; c->dx = dx; c->dt = dt;
     DevState &h = c->h;
     h.nz = nz; h.nr = nr; h.ns = ns; h.ng = ng; h.dx = dx; h.dt = dt;
@@ -1882,6 +1915,38 @@ int ipm_moments(ipm_ctx *c, int s, int implicit_j, double *rho, double *energy_e
     double *outs[5] = {rho, energy_ev, mp, jz, jr};
     for (int q = 0; q < 5; ++q)
         if (outs[q]) HIPCHK(hipMemcpy(outs[q], c->mom + (size_t)q * nn, sizeof(double) * nn, hipMemcpyDeviceToHost));
+    return IPM_OK;
+}
+
+int ipm_moments_noalloc(ipm_ctx *c, int s, int implicit_j, double *rho, double *energy_ev, double *mp, double *jz,
+                        double *jr)
+{
+    int rc = check_ready(c, implicit_j != 0, true);
+    if (rc) return rc;
+    if (s < 0 || s >= c->ns) return fail(IPM_EINVAL, "invalid species index %d", s);
+    const size_t nn = (size_t)c->nz * c->nr;
+    if (nn > kScratchNodes)
+        return fail(IPM_EINVAL, "grid of %zu nodes exceeds the %zu nodes of the no-allocation scratch", nn,
+                    kScratchNodes);
+    if (!c->scratch) return fail(IPM_EINVAL, "no scratch address for device %d", c->device);
+    double *acc = c->scratch;
+    // the accumulators keep the previous call's contents: clear them with a
+    // kernel (no memset node, so a captured graph replays it correctly)
+    hipLaunchKernelGGL(clear_kernel, dim3((unsigned)((5 * nn + 255) / 256)), dim3(256), 0, c->stream, acc, 5 * nn);
+    const long long cap = c->h.cap[s];
+    if (cap > 0)
+        hipLaunchKernelGGL(moments_kernel, dim3((unsigned)((cap + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream,
+                           c->d_state, c->d_ctr, s, implicit_j ? 1 : 0, acc);
+    const double vf = c->h.vfactor;
+    const double ef = 0.5 * c->h.mass[s] * vf * vf;
+    hipLaunchKernelGGL(moments_scale_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream, acc,
+                       c->inv_vol, nn, ef, c->h.charge[s], vf);
+    HIPCHK(hipGetLastError());
+    double *outs[5] = {rho, energy_ev, mp, jz, jr};
+    for (int q = 0; q < 5; ++q)
+        if (outs[q])
+            HIPCHK(hipMemcpyAsync(outs[q], acc + (size_t)q * nn, sizeof(double) * nn, hipMemcpyDeviceToDevice,
+                                  c->stream));
     return IPM_OK;
 }
 

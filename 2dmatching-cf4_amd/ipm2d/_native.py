@@ -46,6 +46,7 @@ SIGNATURES = {
     "ipm_upload_species": [_vp, C.c_int, C.c_int64, _dp, C.c_int],
     "ipm_upload_species_device": [_vp, C.c_int, C.c_int64, C.c_int64, _vp, C.c_int64],
     "ipm_download_species": [_vp, C.c_int, _i64p, _dp, C.c_int64, C.c_int],
+    "ipm_moments_noalloc": [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp],
     "ipm_count": [_vp, C.c_int, _i64p],
     "ipm_set_fields": [_vp, _dp, _dp],
     "ipm_move_es": [_vp, C.c_int],

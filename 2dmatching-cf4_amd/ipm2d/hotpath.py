@@ -211,6 +211,13 @@ class HotPath:
                                                                     ("rho", "energy", "mp", "jz", "jr")))
         return {k: v.reshape(self.nr, self.nz).T for k, v in out.items()}
 
+    def moments_noalloc(self, s, implicit_j, device_outputs):
+        """ipm_moments_noalloc: the five moment grids of species s into device
+        arrays (addresses of Nz*Nr float64 buffers in the order rho, energy,
+        mp, jz, jr; None to skip).  Only enqueues: synchronize before reading."""
+        ptrs = [C.c_void_p(int(p)) if p else None for p in device_outputs]
+        N.call("ipm_moments_noalloc", self._h, s, 1 if implicit_j else 0, *ptrs)
+
     def split_merge(self, s, weight, alpha, grid_r, merge_limit=0.34, split=True, merge=True):
         """split_particle_weight then merge_particle_random on species s
         (Rezoning.F90:53-92) with CalRW's reference weight

@@ -184,6 +184,16 @@ int ipm_set_potential(ipm_ctx *ctx, const double *phi);
  * implicit_j == 0 uses the particle's own velocity. */
 int ipm_moments(ipm_ctx *ctx, int s, int implicit_j, double *rho, double *energy_ev, double *mp, double *jz,
                 double *jr);
+/* ipm_moments for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops): no counterpart in the reference.
+ * Allocates nothing -- the sums live in a static device array of at most
+ * 2^20 grid nodes (40 MiB per GPU, held until the process ends); larger
+ * grids return IPM_EINVAL.  The outputs are DEVICE arrays of Nz*Nr doubles
+ * (NULL to skip); the work is only enqueued on the library's stream, so
+ * read them after ipm_synchronize.  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU race. */
+int ipm_moments_noalloc(ipm_ctx *ctx, int s, int implicit_j, double *rho, double *energy_ev, double *mp,
+                        double *jz, double *jr);
 
 /* Weight rezoning of species s: split_particle_weight then
  * merge_particle_random (R/src/base/Rezoning.F90:53-92), the per-bundle
