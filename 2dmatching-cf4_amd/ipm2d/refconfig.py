@@ -619,25 +619,29 @@ def gases_and_species(run, root=None):
 
 def mcc_tables(run, species, gases, root=None):
     """{(s, g): MCCBundle} for every species s and 0-based gas g, from
-    input/gas/lxcat/<gas>/<species><gas>.txt.  The reference falls back to a
-    hard-sphere table for a species of another gas or without a file
-    (MCCInit.F90:107-116); those cases are refused here."""
+    input/gas/lxcat/<gas>/<species><gas>.txt, as MCCBundleElelctronInit and
+    MCCBundleIonInit build them (R/src/mc/MCCInit.F90:43-120).  An ion
+    against a gas it does not belong to, or without a file for its own gas,
+    gets the hard-sphere table (ProbilityHardShpere, MCCInit.F90:107-116,
+    172-270; xsec.hard_sphere_bundle).  The electron has no such fallback in
+    the reference (its table would be read from an unset SigmaNormalized):
+    a missing electron file is refused."""
     from . import xsec
     root = _input_root(run) if root is None else root
     out = {}
     for s, sp in enumerate(species):
         for g, gas in enumerate(gases):
-            if s > 0 and sp.gas != g + 1:
-                raise NotImplementedError(f"species {sp.name} of another gas: the hard-sphere table is not built")
+            gd = {"mass": gas.mass, "temperature": gas.temperature, "density": gas.density, "natom": gas.natom,
+                  "beta_max": gas.beta_max, "radius": gas.radius, "ns": gas.ns, "index_start": gas.index_start}
             path = os.path.join(root, "input", "gas", "lxcat", gas.name, f"{sp.name}{gas.name}.txt")
+            if s > 0 and (sp.gas != g + 1 or not os.path.exists(path)):
+                out[(s, g)] = xsec.hard_sphere_bundle(s, sp.mass, sp.radius, gd, run.dt)
+                continue
             if not os.path.exists(path):
-                raise NotImplementedError(f"{path} missing: the hard-sphere table is not built")
+                raise NotImplementedError(f"{path} missing: the reference has no fallback table for the electron")
             with open(path, encoding="latin-1") as f:
                 sf = xsec.parse_sigma_file(f.read())
-            out[(s, g)] = xsec.build_bundle(sf, sp.mass, sp.natom,
-                                            {"mass": gas.mass, "temperature": gas.temperature,
-                                             "density": gas.density, "natom": gas.natom, "beta_max": gas.beta_max},
-                                            run.dt, index_start=gas.index_start)
+            out[(s, g)] = xsec.build_bundle(sf, sp.mass, sp.natom, gd, run.dt, index_start=gas.index_start)
     return out
 
 

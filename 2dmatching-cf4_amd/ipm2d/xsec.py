@@ -254,6 +254,56 @@ def probability_reactive(sigma, species_mass, gas_mass, gas_density, beta_max, p
     return p, ratio
 
 
+def hard_sphere_sigma(species_radius, gas_radius):
+    """SigmaMax of ProbilityHardShpere: 2 pi (r_g + r_s)/2 (r_g + r_s)/2
+    (R/src/mc/MCCInit.F90:190), evaluated left to right as written."""
+    return 2.0 * PI * 0.5 * (gas_radius + species_radius) * 0.5 * (gas_radius + species_radius)
+
+
+def hard_sphere_bundle(species_index, species_mass, species_radius, gas, dt):
+    """ProbilityHardShpere (R/src/mc/MCCInit.F90:172-270): the table an ion
+    gets without a cross-section file, or against a gas it does not belong
+    to (MCCBundleIonInit, MCCInit.F90:94-116).
+
+    gas: dict(mass, temperature, density, radius, ns, index_start, beta_max).
+    A gas with one ion species (GO%Ns == 1) gives two reactions, isotropic
+    elastic 101 (resultant: the species) and charge exchange 103 (resultant:
+    the gas's first ion, IndexStart + 1), with Probility = 0.5 and 1.0 times
+    sigma v(E) n; any other gas gives the elastic one alone, 1.0 sigma v(E) n.
+    Normalised by the maximum (SigmaMax), CollisionRatio = 1 - exp(-SigmaMax dt).
+
+    The grid: NSigmaIon points with the ion energy range, log10 E from
+    IonEnergyMin = -1 to IonEnergyMax = 3.  The reference takes EnergyMin and
+    EnergyInterval from a SigmaNormalized that SigmaNormalization has just
+    reset for the missing file (intent(out): EnergyInterval back to its
+    default 0, EnergyMin undefined), which makes its later table lookup
+    divide by zero; the NSigmaIon grid is the one the table's length
+    implies."""
+    emin, eint, emax_exp, nsigma = grid_for_model(3)
+    sig = hard_sphere_sigma(species_radius, gas["radius"])
+    if gas["ns"] == 1:
+        base = [0.5, 1.0]
+        rtype = [101, 103]
+        res = [species_index, gas["index_start"] + 1]
+    else:
+        base = [1.0]
+        rtype = [101]
+        res = [species_index]
+    miu = species_mass * gas["mass"] / (species_mass + gas["mass"])
+    egrid = energy_grid(emin, eint, nsigma)
+    v = np.sqrt(2.0 * egrid * JTOEV / miu)
+    prob = np.empty((nsigma, len(base)))
+    for j, b0 in enumerate(base):
+        prob[:, j] = b0 * sig * v * gas["density"]
+    mx = float(prob.max())
+    prob = prob / mx
+    return MCCBundle(model=2, nreaction=len(base), nsigma=nsigma, emin=emin, einterval=eint, emax=10.0 ** emax_exp,
+                     collision_ratio=1.0 - math.exp(-mx * dt), sigma_max=mx,
+                     rtype=np.array(rtype, dtype=np.int32), resultant=np.array(res, dtype=np.int32),
+                     threshold=np.zeros(len(base)), prob=prob, gas_mass=gas["mass"],
+                     gas_temperature=gas["temperature"], beta_max=gas.get("beta_max", 0.0))
+
+
 def gas_density(pressure_mtorr, temperature, ratio=1.0):
     """GasInit: InitDensity = (p*mTorrtoPa)/(kB*T)*ratio (R/src/base/SpecyOne.F90:136)."""
     return (pressure_mtorr * MTORR_TO_PA) / (KB * temperature) * ratio

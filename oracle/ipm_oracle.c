@@ -34,6 +34,8 @@
  *                         CreateEnergy draw, dissociation product species;
  *                         Langevin fraction 1/beta_max^2 and the
  *                         sub-Threshold(2) shortcut; gases in loop order
+ *   ora_probability_hard_sphere  sigma = 2 pi ((r_g+r_s)/2)^2, v(E) shape,
+ *                         the 0.5/1.0 split (equal 101/103 event counts)
  *   ora_sigma_* / ora_probability_*  closed forms on synthetic tables,
  *                         including sigma linear in the log-energy index
  *                         (the dble(i) weight) and the Emin coordinate of
@@ -1133,6 +1135,35 @@ double ora_probability_reactive(int nreaction, int nsigma, double species_mass, 
             prob[id] = prob[id] + prob[id - 1];
         }
     return ratio;
+}
+
+/* ProbilityHardShpere, MCCInit.F90:172-270: Probility(j,i) = base_j *
+ * SigmaMax * V(E_i) * n with base (0.5, 1.0) for two reactions (101, 103)
+ * or (1.0) for one (101), SigmaMax = 2 pi (r_g+r_s)/2 (r_g+r_s)/2, then
+ * normalised by the table's maximum, which is returned. */
+double ora_probability_hard_sphere(int nreaction, int nsigma, double emin, double einterval, double species_mass,
+                                   double gas_mass, double gas_density, double r_species, double r_gas,
+                                   double *prob)
+{
+    double SigmaMax = 2.0 * C_PI * 0.5 * (r_gas + r_species) * 0.5 * (r_gas + r_species);
+    double Miu = species_mass * gas_mass / (species_mass + gas_mass);
+    for (int i = 1; i <= nsigma; ++i) {
+        prob[(size_t)(i - 1) * nreaction] = (nreaction == 2) ? 0.5 : 1.0;
+        if (nreaction == 2) prob[(size_t)(i - 1) * nreaction + 1] = 1.0;
+    }
+    for (int i = 1; i <= nsigma; ++i) {
+        double Energy = pow(10.0, emin + (double)(i - 1) * einterval);
+        double V = sqrt(2.0 * Energy * C_JTOEV / Miu);
+        for (int j = 0; j < nreaction; ++j) {
+            size_t id = (size_t)(i - 1) * nreaction + j;
+            prob[id] = prob[id] * SigmaMax * V * gas_density;
+        }
+    }
+    size_t nt = (size_t)nsigma * nreaction;
+    double Max = prob[0];
+    for (size_t k = 1; k < nt; ++k) if (prob[k] > Max) Max = prob[k];
+    for (size_t k = 0; k < nt; ++k) prob[k] = prob[k] / Max;
+    return Max;
 }
 
 /* ------------------------------------------------ one step, in place ----

@@ -169,6 +169,12 @@ double ora_probability_reactive(int nreaction, int nsigma, double species_mass, 
                                 double gas_density, double beta_max, double polarizability,
                                 double dt, const double *sigma, double *prob);
 
+/* ProbilityHardShpere (MCCInit.F90:172-270) for nreaction 2 (101, 103) or 1
+ * (101); returns SigmaMax */
+double ora_probability_hard_sphere(int nreaction, int nsigma, double emin, double einterval, double species_mass,
+                                   double gas_mass, double gas_density, double r_species, double r_gas,
+                                   double *prob);
+
 #ifdef __cplusplus
 }
 #endif

@@ -103,6 +103,9 @@ def lib(fast=False):
     L.ora_probability_reactive.restype = C.c_double
     L.ora_probability_reactive.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                            C.c_double, C.c_double, _dp, _dp]
+    L.ora_probability_hard_sphere.restype = C.c_double
+    L.ora_probability_hard_sphere.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                              C.c_double, C.c_double, C.c_double, _dp]
     _libs[path] = L
     return L
 
@@ -442,6 +445,12 @@ def probability_reactive(sigma, ms, mg, ng, beta_max, pol, dt):
     out = np.zeros(nsigma * nr)
     ratio = lib().ora_probability_reactive(nr, nsigma, ms, mg, ng, beta_max, pol, dt, _d(sig), _d(out))
     return out.reshape(nsigma, nr), ratio
+
+
+def probability_hard_sphere(nreaction, emin, eint, nsigma, ms, mg, ng, r_s, r_g):
+    out = np.zeros(nsigma * nreaction)
+    mx = lib().ora_probability_hard_sphere(nreaction, nsigma, emin, eint, ms, mg, ng, r_s, r_g, _d(out))
+    return out.reshape(nsigma, nreaction), mx
 
 
 class StepState:

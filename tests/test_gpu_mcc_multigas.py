@@ -152,7 +152,6 @@ def test_two_gas_fused_step_matches_oracle(oracle, rng):
     np.testing.assert_array_equal(counts, cref)
     assert counts[0, 1, 1:].sum() > 10
     _compare(hp, ref, pushed, cfg.ns, "two-gas step")
-    # the graph replay of the same step agrees with an eager twin
     hp.close()
 
 
@@ -216,4 +215,34 @@ def test_twin_products_match_oracle(oracle, rng):
     prods = got[:, P[fm].shape[1]:]
     # no two products share a bit pattern
     assert np.unique(prods.T, axis=0).shape[1] == 8 and np.unique(prods.T, axis=0).shape[0] == prods.shape[1]
+    hp.close()
+
+
+def test_hard_sphere_tables_match_oracle(oracle, rng):
+    """Ions with the hard-sphere fallback (ProbilityHardShpere,
+    R/src/mc/MCCInit.F90:172-270): the elastic/charge-exchange pair of a
+    one-ion gas on gas 1 and the elastic-only table of a several-ion gas on
+    gas 2, event for event against the oracle."""
+    cfg, tables, species = _setup(pressure=3000.0)
+    geom = W.make_geometry(cfg)
+    g1 = dict(mass=39.948 * AMU, temperature=300.0, density=xsec.gas_density(cfg.pressure_mtorr, 300.0), radius=1.9e-10,
+              ns=1, index_start=0, beta_max=0.0)
+    g2 = dict(g1, mass=20.18 * AMU, radius=1.5e-10, ns=3, density=g1["density"] / 2)
+    tabs = {(0, 0): tables[(0, 0)]}
+    for s in range(1, cfg.ns):
+        tabs[(s, 0)] = xsec.hard_sphere_bundle(s, species[s][1], 2e-10, g1, cfg.dt)
+        tabs[(s, 1)] = xsec.hard_sphere_bundle(s, species[s][1], 2e-10, g2, cfg.dt)
+    hp = _hp(cfg, geom, tabs, species, 2)
+    P = [_energetic(cfg, s, 40000, rng, 30.0 if s == 0 else 10.0) for s in range(cfg.ns)]
+    for s in range(cfg.ns):
+        hp.upload(s, P[s])
+    hp.set_rng(404, 9)
+    hp.set_event_counting(True)
+    hp.mcc()
+    counts = hp.take_mcc_counts()
+    setup = oracle.MccSetup(cfg.dx, cfg.dt, [q for q, _ in species], [m for _, m in species], tabs, ng=2)
+    ref, cref = oracle.mcc(setup, 404, 9, P, counts=True)
+    np.testing.assert_array_equal(counts, cref)
+    assert counts[1, 0, 1] > 50 and counts[1, 0, 2] > 50 and counts[1, 1, 1] > 50
+    _compare(hp, ref, P, cfg.ns, "hard sphere")
     hp.close()

@@ -418,14 +418,41 @@ def test_refused_inputs_say_what_is_missing(tmp_path):
 """)
     species, gases = RC.gases_and_species(run, root=str(tmp_path))
     assert [sp.name for sp in species] == ["Electron", "CF3+"] and gases[0].ns == 1
-    # no cross-section file for the electron: the reference's hard-sphere fallback is not built
+    # no cross-section file for the electron: the reference has no fallback for it
     with pytest.raises(NotImplementedError, match="ElectronCF4.txt"):
         RC.mcc_tables(run, species, gases, root=str(tmp_path))
-    # a species from another gas set would need the same fallback
-    species[1].gas = 2
+    # an ion without its file gets the hard-sphere table (MCCInit.F90:107-111):
+    # the gas has one ion species, so elastic 101 and charge exchange 103
     (gas / "ElectronCF4.txt").write_text(_xsec_text())
-    with pytest.raises(NotImplementedError, match="another gas"):
-        RC.mcc_tables(run, species, gases, root=str(tmp_path))
+    tabs = RC.mcc_tables(run, species, gases, root=str(tmp_path))
+    b = tabs[(1, 0)]
+    assert b.model == 2 and list(b.rtype) == [101, 103] and list(b.resultant) == [1, 1]
+    sigma = math.pi * (2e-10 + gases[0].radius) ** 2 / 2.0
+    mu = species[1].mass * gases[0].mass / (species[1].mass + gases[0].mass)
+    assert b.sigma_max == pytest.approx(sigma * math.sqrt(2e3 * 1.6022e-19 / mu) * gases[0].density, rel=1e-12)
+    # and so does a species of another gas (MCCInit.F90:113-116)
+    species[1].gas = 2
+    assert list(RC.mcc_tables(run, species, gases, root=str(tmp_path))[(1, 0)].rtype) == [101, 103]
+
+
+@needs_ref
+def test_hard_sphere_for_the_reference_be_gas(tmp_path):
+    """input/gas/lxcat/Be has Be+ as its one ion and no Be+Be.txt: the ion
+    gets the hard-sphere pair with the radii of Be.txt (112 pm each)."""
+    d = copy.deepcopy(BASE)
+    d["gas"]["name"] = ["Be"]
+    d["gas"]["ratio"] = [1.0]
+    run = _load(tmp_path, d)
+    root = os.path.dirname(REF)
+    species, gases = RC.gases_and_species(run, root=root)
+    assert [sp.name for sp in species] == ["Electron", "Be+"] and gases[0].ns == 1
+    tabs = RC.mcc_tables(run, species, gases, root=root)
+    b = tabs[(1, 0)]
+    assert list(b.rtype) == [101, 103]
+    sigma = 2.0 * math.pi * (0.5 * 224e-12) ** 2
+    mu = species[1].mass * gases[0].mass / (species[1].mass + gases[0].mass)
+    assert b.sigma_max == pytest.approx(sigma * math.sqrt(2e3 * 1.6022e-19 / mu) * gases[0].density, rel=1e-12)
+    assert tabs[(0, 0)].model == 1
 
 
 def _xsec_text():

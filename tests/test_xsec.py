@@ -152,3 +152,61 @@ SR(1)%EnergySigma         =
     sf1 = xsec.parse_sigma_file(text1)
     r = sf1.reactions[0]
     assert r.threshold == 6.2293e-6 and r.energy.size == 3 and r.sigma[2] == 6.46222e-19
+
+
+def _hs_gas(ns):
+    return dict(mass=88.0043128 * AMU, temperature=300.0, density=xsec.gas_density(50.0, 300.0), radius=1.548e-10,
+                ns=ns, index_start=0, beta_max=3.0)
+
+
+@pytest.mark.parametrize("ns", [1, 3])
+def test_hard_sphere_closed_form(oracle, ns):
+    """ProbilityHardShpere: sigma = 2 pi ((r_g + r_s)/2)^2, Probility = base *
+    sigma v(E) n / max with v = sqrt(2 E e / mu), so the top row is 1 at the
+    grid's last energy and the shape is sqrt(E / E_max); two reactions (101,
+    103) with base 0.5 and 1.0 for a one-ion gas, one (101) otherwise."""
+    ms, rs, dt = 69.0059096 * AMU, 2e-10, 5e-11
+    gas = _hs_gas(ns)
+    b = xsec.hard_sphere_bundle(1, ms, rs, gas, dt)
+    emin, eint, emax, nsig = xsec.grid_for_model(3)
+    E = xsec.energy_grid(emin, eint, nsig)
+    mu = ms * gas["mass"] / (ms + gas["mass"])
+    sigma = math.pi * (rs + gas["radius"]) ** 2 / 2.0
+    vmax = math.sqrt(2.0 * 10.0 ** emax * xsec.JTOEV / mu)
+    assert b.sigma_max == pytest.approx(sigma * vmax * gas["density"], rel=1e-12)
+    assert b.collision_ratio == pytest.approx(1.0 - math.exp(-sigma * vmax * gas["density"] * dt), rel=1e-12)
+    shape = np.sqrt(E / E[-1])
+    if ns == 1:
+        assert list(b.rtype) == [101, 103] and list(b.resultant) == [1, 1]
+        np.testing.assert_allclose(b.prob[:, 1], shape, rtol=1e-12)
+        np.testing.assert_allclose(b.prob[:, 0], 0.5 * shape, rtol=1e-12)
+    else:
+        assert list(b.rtype) == [101] and list(b.resultant) == [1]
+        np.testing.assert_allclose(b.prob[:, 0], shape, rtol=1e-12)
+    assert b.model == 2 and b.nsigma == 1001
+    p, mx = oracle.probability_hard_sphere(b.nreaction, emin, eint, nsig, ms, gas["mass"], gas["density"], rs,
+                                           gas["radius"])
+    np.testing.assert_allclose(p, b.prob, rtol=1e-13)
+    assert mx == pytest.approx(b.sigma_max, rel=1e-14)
+
+
+def test_hard_sphere_events_split_evenly(oracle, rng):
+    """With the 0.5 / 1.0 table a selected ion's elastic and charge-exchange
+    events are equally likely at every energy (SelectProbility,
+    TypeMCC.f90:173-216), and the selected fraction is the CollisionRatio."""
+    ms = 9.012182 * AMU
+    gas = dict(mass=ms, temperature=300.0, density=xsec.gas_density(5000.0, 300.0), radius=112e-12, ns=1,
+               index_start=0, beta_max=0.0)
+    b = xsec.hard_sphere_bundle(1, ms, 112e-12, gas, 5e-9)
+    st = oracle.MccSetup(1e-3, 5e-9, [-xsec.ELECTRON_CHARGE, xsec.ELECTRON_CHARGE], [ME, ms], {(1, 0): b})
+    n = 200000
+    P = np.zeros((8, n))
+    P[0], P[1], P[7] = 3.5, 2.5, 1.0
+    P[2:5] = rng.standard_normal((3, n)) * (5000.0 / (1e-3 / 5e-9))
+    _, cnt = oracle.mcc(st, 3, 0, [np.zeros((8, 0)), P], counts=True)
+    sel = cnt[1, 0].sum()
+    p = b.collision_ratio
+    assert abs(sel - n * p) < 6 * math.sqrt(n * p * (1 - p))
+    n1, n2 = cnt[1, 0, 1], cnt[1, 0, 2]
+    assert n1 + n2 > 2000
+    assert abs(n1 - n2) < 6 * math.sqrt(n1 + n2)
