@@ -74,6 +74,7 @@ struct ipm_ctx {
     unsigned long long *rezone = nullptr;  // split/merge tallies (ipm_split_merge), 3
     hipGraphExec_t graph = nullptr;
     int block = 256, blocks_per_cu = 0;
+    int pk_block = 256, pk_winr = 24;      // fused particle pass: threads per block, deposit window rows
     int collide_bpc = 6;                   // collision-pass blocks per compute unit
     // optional HIP-event timing of the kernels ipm_step enqueues
     bool timing = false;

@@ -25,14 +25,13 @@ using namespace ipm;
 
 namespace {
 
-constexpr int kBlock = 256;          // threads per block (4 wave64)
-constexpr int kItems = 8;            // particles per thread per tile
-constexpr int kTile = kBlock * kItems;
+constexpr int kBlock = 256;          // threads per block (4 wave64) of every kernel but the tuned particle pass
+constexpr int kTile = 2048;          // slots per tile: one block of the particle pass
 constexpr int kWinZ = 64;            // LDS deposit window (nodes in z)
-constexpr int kWinR = 24;            // LDS deposit window (nodes in r)
+constexpr int kWinR = 24;            // default LDS deposit window height (nodes in r)
 // A tile's window starts this many nodes before its first particle's cell.
 constexpr int kAnchorZ = 4;
-constexpr int kAnchorR = kWinR / 2 - 1;
+constexpr int anchor_rows(int winr) { return winr / 2 - 1; }
 
 enum : int { M_PUSH = 1, M_ABSORB = 2, M_DEPOSIT = 4, M_MCC = 8 };
 
@@ -300,18 +299,24 @@ __device__ __forceinline__ bool wave_segmented_sum(int key, double (&v)[4])
 // slot ranges; the tile range is split into 8 contiguous parts, one per XCD
 // (blocks are dispatched to XCDs round-robin), so neighbouring tiles -- which
 // after a cell sort touch neighbouring grid nodes -- share one L2.
-template <int MODE>
-__global__ void __launch_bounds__(kBlock)
+// BLOCK threads (BLOCK / 64 waves) take a tile of kTile slots, ITEMS
+// consecutive runs of 64 slots per wave; WINR is the deposit window's height.
+// The fused pass is instantiated for several (BLOCK, WINR) pairs, which the
+// autotuner chooses among; every other mode uses (256, 24).
+template <int MODE, int BLOCK = kBlock, int WINR = kWinR>
+__global__ void __launch_bounds__(BLOCK)
 particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr, int sel)
 {
+    constexpr int ITEMS = kTile / BLOCK;
+    constexpr int kAnchorR = anchor_rows(WINR);
     const DevState &st = *stp;
-    __shared__ double win[kWinZ * kWinR];
+    __shared__ double win[kWinZ * WINR];
     __shared__ unsigned long long dead_blk[IPM_NS_MAX];
     // Entries of the tile in the collision queue so far (two counters so
     // the next tile never waits for the reset), and each wave's staging
     // buffer of up to 64 queue records, written out with full waves.
     __shared__ unsigned q_scount[2];
-    __shared__ double q_wst[(MODE & M_MCC) ? (kBlock / 64) * 64 * kRecD : 1];
+    __shared__ double q_wst[(MODE & M_MCC) ? (BLOCK / 64) * 64 * kRecD : 1];
 
     const int ns = st.ns;
     long long T = 0;
@@ -319,7 +324,7 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
         if (sel < 0 || sel == s) T += ((long long)ctr->n[s] + kTile - 1) / kTile;
 
     if (MODE & M_DEPOSIT)
-        for (int q = threadIdx.x; q < kWinZ * kWinR; q += kBlock) win[q] = 0.0;
+        for (int q = threadIdx.x; q < kWinZ * WINR; q += BLOCK) win[q] = 0.0;
     if (threadIdx.x < IPM_NS_MAX) dead_blk[threadIdx.x] = 0;
     if (threadIdx.x < 2) q_scount[threadIdx.x] = 0;
     __syncthreads();
@@ -366,7 +371,7 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
         const int rw0 = (MODE & M_DEPOSIT) ? anc[1] : 0;
         int na0 = zw0, na1 = rw0;   // next pass's window, from slot base after this pass
 
-        // Wave w of the block takes kItems consecutive runs of 64 slots, so a
+        // Wave w of the block takes ITEMS consecutive runs of 64 slots, so a
         // wave walks a contiguous, cell-sorted stretch of the tile.  The
         // particle streams are read and written once per pass: non-temporal
         // accesses keep them from evicting the field grids and the deposit
@@ -408,8 +413,8 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
             cw = 0;
         };
 #pragma unroll 1
-        for (int k = 0; k < kItems; ++k) {
-            const long long i = base + (long long)(wv * kItems + k) * 64 + lane;
+        for (int k = 0; k < ITEMS; ++k) {
+            const long long i = base + (long long)(wv * ITEMS + k) * 64 + lane;
             const bool valid = i < n_s;
             Part p;
             if (valid) load_item(i, p);
@@ -470,7 +475,7 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
                 const bool run_end = wave_segmented_sum(key, w);
                 if (run_end && key >= 0) {
                     const int wz = NzL - zw0, wr = NrL - rw0;
-                    if (wz >= 0 && wz + 1 < kWinZ && wr >= 0 && wr + 1 < kWinR) {
+                    if (wz >= 0 && wz + 1 < kWinZ && wr >= 0 && wr + 1 < WINR) {
                         double *wp = win + wz + wr * kWinZ;
                         lds_add(wp, w[0]);
                         lds_add(wp + kWinZ, w[1]);
@@ -547,7 +552,7 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
         }
         if (MODE & M_DEPOSIT) {
             __syncthreads();
-            for (int q = threadIdx.x; q < kWinZ * kWinR; q += kBlock) {
+            for (int q = threadIdx.x; q < kWinZ * WINR; q += BLOCK) {
                 const double v = win[q];
                 if (v != 0.0) {
                     const int iz = zw0 + (q % kWinZ), ir = rw0 + (q / kWinZ);
@@ -1335,7 +1340,7 @@ __global__ void scan_add_kernel(unsigned long long *__restrict__ out, const unsi
 // tile's window is placed by its first particle, as the particle pass does
 // for the pass after it.  anc points at the species' first tile.
 __global__ void anchor_kernel(const DevCounters *__restrict__ ctr, int s, const double *__restrict__ Z,
-                              const double *__restrict__ R, int *__restrict__ anc, long long max_tiles)
+                              const double *__restrict__ R, int *__restrict__ anc, long long max_tiles, int arows)
 {
     const long long n = (long long)ctr->n[s];
     const long long nt = min((n + kTile - 1) / kTile, max_tiles);
@@ -1344,7 +1349,7 @@ __global__ void anchor_kernel(const DevCounters *__restrict__ ctr, int s, const 
     const double z = Z[t * kTile], r = R[t * kTile];
     if (is_dead(z) || !(z > 0.0) || !(r >= 0.0) || z >= 1e9 || r >= 1e9) return;
     anc[2 * t] = (int)ceil(z) - kAnchorZ;
-    anc[2 * t + 1] = (int)ceil(r) - kAnchorR;
+    anc[2 * t + 1] = (int)ceil(r) - arows;
 }
 
 __global__ void set_count_kernel(DevCounters *ctr, int s, const unsigned long long *offs,
@@ -1478,11 +1483,29 @@ static int grid_blocks(ipm_ctx *c)
     return (int)std::max<long long>(8, std::min<long long>(tmax, 0x7fffffffLL));
 }
 
+template <int MODE, int BLOCK, int WINR>
+static void launch_shape(ipm_ctx *c, int sel)
+{
+    hipLaunchKernelGGL((particle_kernel<MODE, BLOCK, WINR>), dim3(grid_blocks(c)), dim3(BLOCK), 0, c->stream,
+                       c->d_state, c->d_ctr, sel);
+}
+
+// The fused pass (every mode bit set) runs the tuned shape; the separate
+// entry points run the default one.
 template <int MODE>
 static int launch_particles(ipm_ctx *c, int sel)
 {
-    hipLaunchKernelGGL(particle_kernel<MODE>, dim3(grid_blocks(c)), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr,
-                       sel);
+    if constexpr (MODE == (M_PUSH | M_ABSORB | M_DEPOSIT | M_MCC)) {
+        const int b = c->pk_block, w = c->pk_winr;
+        if (b == 128 && w == 24) launch_shape<MODE, 128, 24>(c, sel);
+        else if (b == 512 && w == 24) launch_shape<MODE, 512, 24>(c, sel);
+        else if (b == 256 && w == 16) launch_shape<MODE, 256, 16>(c, sel);
+        else if (b == 256 && w == 32) launch_shape<MODE, 256, 32>(c, sel);
+        else if (b == 256 && w == 24) launch_shape<MODE, 256, 24>(c, sel);
+        else return fail(IPM_EINVAL, "no particle-pass variant for %d threads and a %d-row window", b, w);
+    } else {
+        launch_shape<MODE, kBlock, kWinR>(c, sel);
+    }
     HIPCHK(hipGetLastError());
     return IPM_OK;
 }
@@ -2187,7 +2210,7 @@ int ipm_sort(ipm_ctx *c, int s)
             if (c->h.q_cbase[sp] + max_tiles <= c->q_ntiles) {
                 const unsigned ablk = (unsigned)((max_tiles + kBlock - 1) / kBlock);
                 hipLaunchKernelGGL(anchor_kernel, dim3(ablk), dim3(kBlock), 0, c->stream, c->d_ctr, sp, dst[0], dst[1],
-                                   c->tanc + 2 * c->h.q_cbase[sp], max_tiles);
+                                   c->tanc + 2 * c->h.q_cbase[sp], max_tiles, anchor_rows(c->pk_winr));
             }
         }
         HIPCHK(hipGetLastError());
@@ -2202,9 +2225,28 @@ int ipm_sort(ipm_ctx *c, int s)
 int ipm_set_launch(ipm_ctx *c, int block, int blocks_per_cu)
 {
     if (!c) return fail(IPM_EINVAL, "null context");
-    if (block != 0 && block != kBlock) return fail(IPM_EINVAL, "block size is fixed at %d", kBlock);
+    if (block != 0 && block != 128 && block != 256 && block != 512)
+        return fail(IPM_EINVAL, "block must be 0 (256), 128, 256 or 512 threads, got %d", block);
     if (blocks_per_cu < 0 || blocks_per_cu > 64) return fail(IPM_EINVAL, "invalid blocks_per_cu %d", blocks_per_cu);
+    const int b = block ? block : kBlock;
+    if (b != kBlock && c->pk_winr != kWinR)
+        return fail(IPM_EINVAL, "a %d-thread block runs with the %d-row window only", b, kWinR);
+    c->pk_block = b;
     c->blocks_per_cu = blocks_per_cu;
+    if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    return IPM_OK;
+}
+
+int ipm_set_deposit_window(ipm_ctx *c, int rows)
+{
+    if (!c) return fail(IPM_EINVAL, "null context");
+    if (rows != 16 && rows != 24 && rows != 32) return fail(IPM_EINVAL, "window rows must be 16, 24 or 32, got %d", rows);
+    if (rows != kWinR && c->pk_block != kBlock)
+        return fail(IPM_EINVAL, "a %d-row window runs with %d-thread blocks only", rows, kBlock);
+    c->pk_winr = rows;
     if (c->graph) {
         (void)hipGraphExecDestroy(c->graph);
         c->graph = nullptr;

@@ -7,6 +7,9 @@ live simulation: every candidate advances the real state by a few steps.
 
 * ``blocks_per_cu``: 0 for one block per tile, else persistent blocks per
   compute unit of the fused particle kernel; chosen by timing a few steps each.
+* ``tile_shape``: threads per block of the fused pass (128, 256 or 512, i.e.
+  16, 8 or 4 particles per thread of a 2048-slot tile) and the height of its
+  LDS deposit window (16, 24 or 32 node rows).
 * ``collide_blocks_per_cu``: grid of the collision pass (blocks per compute
   unit); chosen by timing a few whole steps each, since it also changes
   where the pass leaves the caches for the next particle pass.
@@ -48,6 +51,29 @@ def tune_blocks_per_cu(hp, candidates=(0, 2, 4, 6, 8), steps=6, timer=None):
     return best, res
 
 
+# (threads per block, deposit-window rows) of the fused particle pass
+TILE_SHAPES = ((256, 24), (128, 24), (512, 24), (256, 16), (256, 32))
+
+
+def tune_tile_shape(hp, candidates=TILE_SHAPES, steps=6, lead=10, timer=None):
+    """Threads per block (16, 8 or 4 particles per thread of a 2048-slot
+    tile) and deposit-window height of the fused pass.  Each candidate is
+    timed over `steps` steps that start `lead` steps after a sort: the
+    window's height matters once particles have drifted out of cell order.
+    Returns (best, {(block, rows): seconds per step})."""
+    timer = timer or (lambda: _timed(hp, steps))
+    res = {}
+    for shape in candidates:
+        hp.set_tile_shape(*shape)
+        hp.sort(-1)
+        for _ in range(lead):
+            hp.one_step()
+        res[tuple(shape)] = timer()
+    best = min(res, key=res.get)
+    hp.set_tile_shape(*best)
+    return best, res
+
+
 def tune_collide_grid(hp, candidates=(2, 4, 6), steps=6, timer=None):
     """Return (best, {candidate: seconds per step})."""
     timer = timer or (lambda: _timed(hp, steps))
@@ -81,7 +107,8 @@ def cache_key(hp, nparticles):
 
 
 def autotune(hp, nparticles, cache_path=None, bpc_candidates=(0, 2, 4, 6, 8),
-             sort_candidates=(5, 10, 20, 40, 60, 80), steps=6, collide_candidates=(2, 4, 6)):
+             sort_candidates=(5, 10, 20, 40, 60, 80), steps=6, collide_candidates=(2, 4, 6),
+             tile_candidates=TILE_SHAPES, tile_lead=10):
     """Tune (or load from cache) and apply; returns the chosen settings."""
     key = cache_key(hp, nparticles)
     if cache_path and os.path.exists(cache_path):
@@ -89,12 +116,15 @@ def autotune(hp, nparticles, cache_path=None, bpc_candidates=(0, 2, 4, 6, 8),
             cache = json.load(f)
         if key in cache:
             hp.set_launch(cache[key]["blocks_per_cu"])
+            hp.set_tile_shape(*cache[key].get("tile_shape", (256, 24)))
             hp.set_collide_grid(cache[key].get("collide_blocks_per_cu", 6))
             return cache[key]
     bpc, tb = tune_blocks_per_cu(hp, bpc_candidates, steps)
+    shape, tt = tune_tile_shape(hp, tile_candidates, steps, tile_lead) if tile_candidates else ((256, 24), {})
     cbpc, tc = tune_collide_grid(hp, collide_candidates, steps) if collide_candidates else (6, {})
     si, ts = tune_sort_interval(hp, sort_candidates)
-    out = {"blocks_per_cu": bpc, "collide_blocks_per_cu": cbpc, "sort_interval": si,
+    out = {"blocks_per_cu": bpc, "tile_shape": list(shape), "collide_blocks_per_cu": cbpc, "sort_interval": si,
+           "step_seconds_by_tile_shape": {f"{k[0]}x{k[1]}": v for k, v in tt.items()},
            "step_seconds_by_blocks_per_cu": {str(k): v for k, v in tb.items()},
            "step_seconds_by_collide_blocks_per_cu": {str(k): v for k, v in tc.items()},
            "step_seconds_by_sort_interval": {str(k): v for k, v in ts.items()}}

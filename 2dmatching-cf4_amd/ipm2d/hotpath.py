@@ -46,6 +46,7 @@ class HotPath:
         N.call("ipm_create", C.byref(h), nz, nr, float(dx), float(dt), ns, ng)
         self._h = h
         self.nmetal = 0
+        self._block, self._bpc, self._winr = 256, 0, 24
         self.capacity = [0] * ns
         self.species_params = [None] * ns     # (charge, mass) per species
 
@@ -105,8 +106,27 @@ class HotPath:
     def set_event_counting(self, on=True):
         N.call("ipm_set_event_counting", self._h, 1 if on else 0)
 
-    def set_launch(self, blocks_per_cu=0):
-        N.call("ipm_set_launch", self._h, 0, int(blocks_per_cu))
+    def set_launch(self, blocks_per_cu=0, block=None):
+        """Grid of the fused pass (0: one block per tile; else persistent
+        blocks per CU) and, if given, its threads per block (128/256/512)."""
+        block = self._block if block is None else int(block)
+        N.call("ipm_set_launch", self._h, block, int(blocks_per_cu))
+        self._block, self._bpc = block, int(blocks_per_cu)
+
+    def set_deposit_window(self, rows=24):
+        N.call("ipm_set_deposit_window", self._h, int(rows))
+        self._winr = int(rows)
+
+    def set_tile_shape(self, block, rows):
+        """Threads per block of the fused pass and its deposit window height
+        (a pair ipm_set_launch and ipm_set_deposit_window accept together)."""
+        self.set_deposit_window(24)
+        self.set_launch(self._bpc, block)
+        self.set_deposit_window(rows)
+
+    @property
+    def tile_shape(self):
+        return self._block, self._winr
 
     def set_collide_grid(self, blocks_per_cu=6):
         N.call("ipm_set_collide_grid", self._h, int(blocks_per_cu))

@@ -410,7 +410,8 @@ def main():
         from ipm2d import autotune
         tuned = autotune.autotune(hp, ntot)
         sort_interval = tuned["sort_interval"]
-        log(f"[bench] autotuned: blocks_per_cu {tuned['blocks_per_cu']}, collision blocks per CU "
+        log(f"[bench] autotuned: blocks_per_cu {tuned['blocks_per_cu']}, tile shape {tuned['tile_shape']} "
+            f"{tuned['step_seconds_by_tile_shape']}, collision blocks per CU "
             f"{tuned['collide_blocks_per_cu']}, sort every {sort_interval} steps")
     elif args.blocks_per_cu > 0:
         hp.set_launch(args.blocks_per_cu)
@@ -518,7 +519,10 @@ def main():
             "launch": ("autotuned: " if tuned else "") + (
                 "one block per 2048-slot tile" if (tuned["blocks_per_cu"] if tuned else max(args.blocks_per_cu, 0)) == 0
                 else f"{tuned['blocks_per_cu'] if tuned else args.blocks_per_cu} persistent blocks per CU") + (
-                f", collision pass {tuned['collide_blocks_per_cu']} blocks per CU" if tuned else ""),
+                f", {tuned['tile_shape'][0]} threads per tile ({2048 // tuned['tile_shape'][0]} particles per "
+                f"thread), {tuned['tile_shape'][1]}-row deposit window, collision pass "
+                f"{tuned['collide_blocks_per_cu']} blocks per CU" if tuned else ""),
+            "autotune": ({k: v for k, v in tuned.items() if k.startswith("step_seconds")} if tuned else None),
         },
         "roofline": {
             "bound": "hbm",

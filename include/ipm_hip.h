@@ -150,11 +150,18 @@ int ipm_take_timing(ipm_ctx *ctx, double *ms, int64_t *launches);
 /* Cell sort (counting sort by cell; drops dead slots). s = -1: all. */
 int ipm_sort(ipm_ctx *ctx, int s);
 
-/* Launch shape of the particle pass: block must be 0 or 256 (threads per
- * block); blocks_per_cu = 0 launches one block per tile of 2048 particles
- * (grid sized from the capacities, the default), 1..64 a persistent grid of
- * that many blocks per compute unit looping over the tiles. */
+/* Launch shape of the fused particle pass (ipm_step, ipm_step_graph):
+ * block = threads per block of a tile of 2048 slots, 128, 256 or 512 (16, 8
+ * or 4 particles per thread; 0 = 256); blocks_per_cu = 0 launches one block
+ * per tile (grid sized from the capacities, the default), 1..64 a
+ * persistent grid of that many blocks per compute unit looping over the
+ * tiles.  A scheduling choice with no counterpart in the reference: results
+ * do not depend on it. */
 int ipm_set_launch(ipm_ctx *ctx, int block, int blocks_per_cu);
+/* Height (nodes in r: 16, 24 or 32, default 24) of the LDS window the fused
+ * pass deposits into before flushing to HBM (64 nodes in z); heights other
+ * than 24 run with 256-thread blocks.  Results do not depend on it. */
+int ipm_set_deposit_window(ipm_ctx *ctx, int rows);
 
 /* Grid of the collision pass: blocks_per_cu (1..6, default 6) blocks per
  * compute unit.  Like ipm_set_launch, a scheduling choice with no
