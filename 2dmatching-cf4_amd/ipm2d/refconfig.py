@@ -545,6 +545,8 @@ class GasDef:
     temperature: float
     density: float           # m^-3
     index_start: int         # species of this gas are index_start + 1 .. index_start + ns
+    cs_model: int = 1        # 1 lxcat, 2 Pegasus (GO%CSModel)
+    sop_index: Optional[List[int]] = None   # Pegasus: the Index of each species line
 
 
 _NML_ITEM = re.compile(r"GP%(\w+)(?:\((\d+)\))?\s*=", re.IGNORECASE)
@@ -590,25 +592,210 @@ def read_gas_physics(path):
     return out
 
 
+# ------------------------------------------------------------ Pegasus gas sets
+class _Records:
+    """Sequential formatted reads of a text file as Fortran does them: each
+    READ starts on a new line, a list-directed READ takes its items from as
+    many lines as it needs (blanks and commas separate them, '/' ends the
+    list) and the rest of its last line is skipped."""
+
+    def __init__(self, text, path):
+        self.lines = text.splitlines()
+        self.pos = 0
+        self.path = path
+
+    def skip_comments(self):
+        """skip_comments (R/src/base/numerical/Numrical.F90:5-15): lines whose
+        first character is '#'."""
+        while self.pos < len(self.lines) and self.lines[self.pos][:1] == "#":
+            self.pos += 1
+
+    def read(self, n):
+        toks = []
+        while len(toks) < n:
+            if self.pos >= len(self.lines):
+                raise ConfigError(f"{self.path}: end of file after {len(toks)} of {n} items")
+            line = self.lines[self.pos].split("/", 1)[0]
+            self.pos += 1
+            toks += [t for t in re.split(r"[\s,]+", line) if t]
+        return toks[:n]
+
+
+def read_gas_physics_pegasus(path):
+    """input/gas/pegasus/<gas>/<gas>.txt as LoadGasPhysicalPegasus reads it
+    (R/src/mc/ReactionOnePegasus.f90:450-486): name, MCModel, NAtom, the
+    number of particle lines NSOP, then NSOP lines 'Index Mass Charge Radius
+    Polarizability Name' (LoadSpecyOnePegasus, R/src/base/SpecyOne.F90:287-294),
+    the first the gas itself.  The gas takes that first line's name, mass
+    and radius (Angstrom, times 1e-10: ConvertSpecyOnePegasus2GasPhysical),
+    Ns = NSOP - 1; species j its line's name, charge, mass and radius
+    (NAtom keeps its default 1, SpecyOnePhysical); BetaMax is not read (0).
+    Returns the dict read_gas_physics returns, plus 'SOP_index', the Index of
+    each species line (the numbers the reaction files name particles by)."""
+    with open(path, encoding="latin-1") as f:
+        r = _Records(f.read(), path)
+    name = r.read(1)[0]
+    mc_model = int(r.read(1)[0])
+    natom = int(r.read(1)[0])
+    nsop = int(r.read(1)[0])
+    sop = []
+    for _ in range(nsop):
+        t = r.read(6)
+        sop.append((int(t[0]), _num(t[1]), _num(t[2]), _num(t[3]), _num(t[4]), t[5]))
+    if nsop < 1:
+        raise ConfigError(f"{path}: no particle lines")
+    gas = sop[0]
+    out = {"Name": gas[5], "FileName": name, "MCModel": mc_model, "NAtom": natom, "Ns": nsop - 1,
+           "Mass": gas[1], "Radius": gas[3] * 1e-10, "BetaMax": 0.0, "SP": {},
+           "SOP_index": [x[0] for x in sop[1:]]}
+    for j, x in enumerate(sop[1:], start=1):
+        out["SP"][j] = (x[5], 1, int(round(x[2])), x[1], x[3] * 1e-10)
+    return out
+
+
+def _num(tok):
+    from .restart import _value
+    return _value(tok)
+
+
+def read_pegasus_reaction(path):
+    """One reaction file of a Pegasus set as LoadReactionOnePegasus reads it
+    (R/src/mc/ReactionOnePegasus.f90:105-145, LoadReactionOneBase 93-103):
+    comment lines, Index, Name, comment lines, Threshold, NSpecy, SpecyIndex(1:NSpecy),
+    NumberChange(1:NSpecy), ReactionType, Ncrosssection, then that many
+    (energy [eV], cross section [cm^2]) lines.  Returns a dict."""
+    with open(path, encoding="latin-1") as f:
+        r = _Records(f.read(), path)
+    r.skip_comments()
+    index = int(r.read(1)[0])
+    name = r.read(1)[0]
+    r.skip_comments()
+    th = _num(r.read(1)[0])
+    nsp = int(r.read(1)[0])
+    spidx = [int(t) for t in r.read(nsp)]
+    nchg = [int(t) for t in r.read(nsp)]
+    rtype = int(r.read(1)[0])
+    ncs = int(r.read(1)[0])
+    if ncs > 500:
+        raise ConfigError(f"{path}: {ncs} cross-section points exceed NcrosssectionMax = 500")
+    cs = np.array([[_num(t) for t in r.read(2)] for _ in range(ncs)], dtype=np.float64).reshape(ncs, 2)
+    return {"index": index, "name": name, "threshold": th, "species": spidx, "change": nchg, "type": rtype,
+            "energy": cs[:, 0].copy(), "sigma_cm2": cs[:, 1].copy()}
+
+
+def pegasus_reaction_code(rx, sop_index, reactant):
+    """(ReactionType, Resultant) of ConvertReactionOneBase2ReactionOne
+    (R/src/mc/ReactionOnePegasus.f90:241-404) for a reaction read by
+    read_pegasus_reaction, among the gas's species whose Index numbers are
+    sop_index (species j = position j, 1-based; particle number 1 is the
+    electron, 0; unknown numbers -1, ConvertReactionOneIndex 424-442).
+
+    The source sets its electron flag in a loop over the first two
+    particles that keeps only the second one's test; that is kept.  Where
+    the source leaves Resultant unassigned (ion types 1 and 6, whose
+    collisions make no product), the reactant's own index is used; reaction
+    codes the source leaves unassigned are refused."""
+    idx = []
+    for t in rx["species"]:
+        if t == 1:
+            idx.append(0)
+        else:
+            idx.append(sop_index.index(t) + 1 if t in sop_index else -1)
+    is_electron = len(idx) >= 2 and idx[1] == 0
+    products = [k for k in idx[2:] if k > 0]
+    n = len(products)
+    rt = rx["type"]
+    where = f"reaction {rx['index']} ({rx['name']})"
+    if is_electron:
+        if rt == 1:
+            return 11, 0
+        if rt == 100:
+            return 12, 0
+        if rt == 2:
+            if n < 1:
+                raise ConfigError(f"{where}: ionization without a product")
+            return 13, products[0]
+        if rt in (3, 4, 255):
+            if n == 0:
+                return 12, 0
+            if n == 2:
+                return 15, products[0] * 10 + products[1]
+            if n == 1:
+                if rt == 3:
+                    return 14, products[0]
+                ch = [rx["change"][k] for k in range(2) if idx[k] == 0]
+                if not ch:
+                    raise ConfigError(f"{where}: no electron among the first two particles")
+                c = ch[-1]
+                if rt == 255 and c == 1:
+                    return 13, products[0]
+                if c == 0:
+                    return 12, (products[0] if rt == 4 else 0)
+                if c == -1:
+                    return 14, products[0]
+        raise ConfigError(f"{where}: electron reaction type {rt} with {n} products is not converted by the source")
+    if rt == 1:
+        return 101, reactant
+    if rt == 6:
+        return 103, reactant
+    if rt == 255 and n == 1:
+        return 112, products[0]
+    raise ConfigError(f"{where}: ion reaction type {rt} with {n} products is not converted here")
+
+
+def pegasus_sigma_file(root, gas, species_name, species_index):
+    """The SigmaFile SigmaNormalizationPegasus builds for one (species, gas)
+    (R/src/mc/ReactionOnePegasus.f90:36-88): input/gas/pegasus/<gas>/<species><gas>.txt
+    holds Model and NReaction, then one line per reaction naming its file
+    (the type and threshold on that line are not used); cross sections are
+    converted from cm^2 to m^2 (ConvertReactionOnePegasus2SigmaRaw, 443-449).
+    None if the species file is missing."""
+    from . import xsec
+    d = os.path.join(root, "input", "gas", "pegasus", gas.name)
+    path = os.path.join(d, f"{species_name}{gas.name}.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path, encoding="latin-1") as f:
+        r = _Records(f.read(), path)
+    model, nreaction = (int(t) for t in r.read(2))
+    reactions = []
+    for _ in range(nreaction):
+        cs_index = r.read(3)[0]
+        rx = read_pegasus_reaction(os.path.join(d, cs_index))
+        rtype, res = pegasus_reaction_code(rx, gas.sop_index, species_index)
+        reactions.append(xsec.Reaction(species_index, rtype, res, rx["threshold"], rx["energy"],
+                                       rx["sigma_cm2"] / 1.0e4))
+    return xsec.SigmaFile(model, reactions)
+
+
 def _input_root(run):
     return os.path.dirname(os.path.abspath(run.source))
 
 
 def gases_and_species(run, root=None):
-    """(species, gases) of the run's gas list, GasInitPegasus order."""
+    """(species, gases) of the run's gas list, GasInitPegasus order
+    (R/src/mc/ReactionOnePegasus.f90:521-600): lxcat sets
+    (collision_section_type 1) from input/gas/lxcat/<gas>/<gas>.txt,
+    Pegasus sets (2) from input/gas/pegasus/<gas>/<gas>.txt."""
     from . import xsec
-    if run.gas.collision_section_type != 1:
-        raise NotImplementedError("only lxcat gas sets (collision_section_type 1) are read")
+    if run.gas.collision_section_type not in (1, 2):
+        raise NotImplementedError(f"collision_section_type {run.gas.collision_section_type}: only lxcat (1) and "
+                                  f"Pegasus (2) gas sets exist")
+    pegasus = run.gas.collision_section_type == 2
     root = _input_root(run) if root is None else root
     species = [SpeciesDef("Electron", 1, float(-1) * xsec.ELECTRON_CHARGE,
                           (xsec.ELECTRON_MASS / xsec.ATOMIC_MASS) * xsec.ATOMIC_MASS, 0.0, 0)]
     gases = []
     index = 0
     for g, (name, ratio) in enumerate(zip(run.gas.names, run.gas.ratios), start=1):
-        gp = read_gas_physics(os.path.join(root, "input", "gas", "lxcat", name, f"{name}.txt"))
+        if pegasus:
+            gp = read_gas_physics_pegasus(os.path.join(root, "input", "gas", "pegasus", name, f"{name}.txt"))
+        else:
+            gp = read_gas_physics(os.path.join(root, "input", "gas", "lxcat", name, f"{name}.txt"))
         gases.append(GasDef(gp["Name"], gp.get("MCModel", 0), gp["NAtom"], gp["Ns"], gp["Mass"] * xsec.ATOMIC_MASS,
                             gp.get("Radius", 0.0), gp.get("BetaMax", 0.0), run.gas.temperature,
-                            xsec.gas_density(run.gas.pressure, run.gas.temperature, ratio), index))
+                            xsec.gas_density(run.gas.pressure, run.gas.temperature, ratio), index,
+                            2 if pegasus else 1, gp.get("SOP_index")))
         for j in range(1, gp["Ns"] + 1):
             nm, natom, z, m_amu, radius = gp["SP"][j]
             species.append(SpeciesDef(nm, natom, float(z) * xsec.ELECTRON_CHARGE, m_amu * xsec.ATOMIC_MASS,
@@ -633,14 +820,23 @@ def mcc_tables(run, species, gases, root=None):
         for g, gas in enumerate(gases):
             gd = {"mass": gas.mass, "temperature": gas.temperature, "density": gas.density, "natom": gas.natom,
                   "beta_max": gas.beta_max, "radius": gas.radius, "ns": gas.ns, "index_start": gas.index_start}
-            path = os.path.join(root, "input", "gas", "lxcat", gas.name, f"{sp.name}{gas.name}.txt")
-            if s > 0 and (sp.gas != g + 1 or not os.path.exists(path)):
+            if gas.cs_model == 2:
+                # electron species index 0; reactions name products by
+                # their position among the gas's species
+                sf = pegasus_sigma_file(root, gas, sp.name, s - gas.index_start if s > 0 else 0) \
+                    if (s == 0 or sp.gas == g + 1) else None
+                path = os.path.join(root, "input", "gas", "pegasus", gas.name, f"{sp.name}{gas.name}.txt")
+            else:
+                path = os.path.join(root, "input", "gas", "lxcat", gas.name, f"{sp.name}{gas.name}.txt")
+                sf = None
+                if os.path.exists(path) and (s == 0 or sp.gas == g + 1):
+                    with open(path, encoding="latin-1") as f:
+                        sf = xsec.parse_sigma_file(f.read())
+            if s > 0 and sf is None:
                 out[(s, g)] = xsec.hard_sphere_bundle(s, sp.mass, sp.radius, gd, run.dt)
                 continue
-            if not os.path.exists(path):
+            if sf is None:
                 raise NotImplementedError(f"{path} missing: the reference has no fallback table for the electron")
-            with open(path, encoding="latin-1") as f:
-                sf = xsec.parse_sigma_file(f.read())
             out[(s, g)] = xsec.build_bundle(sf, sp.mass, sp.natom, gd, run.dt, index_start=gas.index_start)
     return out
 

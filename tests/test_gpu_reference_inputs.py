@@ -38,7 +38,35 @@ def _hashes(P):
     return np.sort(h)
 
 
-def _write_inputs(root, controls=False):
+def _write_pegasus_ar(root):
+    """A Pegasus-format Ar set (input/gas/pegasus/Ar/): the gas file, the
+    electron's three reactions (elastic, ionization into Ar(+), an
+    excitation), Ar(+)'s charge exchange and elastic scattering, and no file
+    for Ar(2+), which therefore gets the hard-sphere table."""
+    d = root / "input" / "gas" / "pegasus" / "Ar"
+    d.mkdir(parents=True)
+    (d / "Ar.txt").write_text("Ar\n1\n1\n3\n180  39.95  0.0  3.68  93.30  Ar\n"
+                              "181  39.95  +1.0  3.33  -1  Ar(+)\n182  39.95  +2.0  3.00  -1  Ar(2+)\n")
+    (d / "ElectronAr.txt").write_text("1  3\n180000  Elastic 0.0\n180001  Ionization 15.76\n"
+                                      "180006  Excitation 11.828\n")
+    (d / "Ar(+)Ar.txt").write_text("2  2\n180011  Exchange 0.0\n180012  Elastic 0.0\n")
+    e = np.logspace(-2, 3, 60)
+
+    def rx(index, name, th, species, change, rtype, sigma_cm2):
+        rows = "\n".join(f"  {a:.6e}  {b:.6e}" for a, b in zip(e, sigma_cm2))
+        (d / str(index)).write_text(f"# synthetic\n{index}\n{name}\n{th}\n{len(species)}\n"
+                                    f"{' '.join(map(str, species))}\n{' '.join(f'{x:+d}' for x in change)}\n"
+                                    f"{rtype}\n{len(e)}\n{rows}\n")
+    rx(180000, "e(-)+Ar->Ar+e(-)", 0.0, (180, 1), (0, 0), 1, np.full(e.size, 1.0e-15))
+    rx(180001, "e(-)+Ar->Ar(+)+e(-)+e(-)", 15.76, (180, 1, 181), (-1, 1, 1), 2,
+       np.where(e > 15.76, 3e-16 * (1 - np.exp(-(e - 15.76) / 30.0)), 0.0))
+    rx(180006, "e(-)+Ar->Ar(*)+e(-)", 11.828, (180, 1, 186), (-1, 0, 1), 100,
+       np.where(e > 11.828, 1e-16, 0.0))
+    rx(180011, "Ar(+)+Ar->Ar+Ar(+)", 0.0, (180, 181), (0, 0), 6, np.full(e.size, 6e-15))
+    rx(180012, "Ar(+)+Ar->Ar(+)+Ar", 0.0, (180, 181), (0, 0), 1, np.full(e.size, 2e-14))
+
+
+def _write_inputs(root, controls=False, pegasus=False):
     (root / "input" / "gas" / "lxcat" / "CF4").mkdir(parents=True)
     cm = np.zeros((NZ - 1, NR - 1), dtype=np.int32)      # indices into the materials list
     cm[:2, :] = 1
@@ -94,6 +122,9 @@ def _write_inputs(root, controls=False):
                 "collision_section_type": 1, "is_open_recombination": False},
         "circuit": {"circuits_number": 1, "circuits_list": [{"type": 1, "metal_index": 2, "name": "ground"}]},
     }
+    if pegasus:
+        _write_pegasus_ar(root)
+        cfg["gas"].update({"name": ["Ar"], "collision_section_type": 2})
     if controls:                            # the shipped configuration's controls
         cfg["plasma"]["is_open_split_merge"] = True
         cfg["diag"]["diag_preiod"] = 4
@@ -129,6 +160,38 @@ def test_reference_inputs_to_gpu_steps(tmp_path):
         np.testing.assert_array_equal(_hashes(a.download(s)), _hashes(b.download(s)), err_msg=species[s].name)
     metal, _ = a.take_tallies()
     assert np.any(metal != 0.0), "nothing reached the metals"
+    a.close()
+    b.close()
+
+
+def test_pegasus_inputs_to_gpu_steps(tmp_path):
+    """The same cell with a Pegasus-format Ar set (collision_section_type 2,
+    R/src/mc/ReactionOnePegasus.f90): the tables come from the reaction files,
+    Ar(2+) without a file gets the hard-sphere table, and five eager steps
+    equal five graph steps bit for bit."""
+    run = RC.load(_write_inputs(tmp_path, pegasus=True))
+    species, gases = RC.gases_and_species(run)
+    tabs = RC.mcc_tables(run, species, gases)
+    assert [sp.name for sp in species] == ["Electron", "Ar(+)", "Ar(2+)"]
+    assert list(tabs[(0, 0)].rtype) == [11, 13, 12] and tabs[(0, 0)].resultant[1] == 1
+    assert list(tabs[(1, 0)].rtype) == [103, 101] and list(tabs[(2, 0)].rtype) == [101]
+    a, _, _, _ = RC.build_hotpath(run, seed=5, capacity=60000)
+    b, _, _, _ = RC.build_hotpath(run, seed=5, capacity=60000)
+    n0 = [a.count(s) for s in range(3)]
+    z = np.arange(NZ)[:, None] * np.ones((1, NR))
+    phi = 300.0 * (1.0 - z / (NZ - 1))
+    for hp in (a, b):
+        hp.set_potential(phi)
+        hp.set_event_counting(True)
+    for _ in range(5):
+        a.one_step()
+        b.one_step_graph()
+    b.synchronize()
+    for s in range(3):
+        np.testing.assert_array_equal(_hashes(a.download(s)), _hashes(b.download(s)), err_msg=species[s].name)
+    counts = a.take_mcc_counts()
+    assert counts[0, 0, 1] > 100                      # electron elastic events from file 180000
+    assert all(a.count(s) > 0.9 * n0[s] for s in range(3))
     a.close()
     b.close()
 

@@ -403,9 +403,9 @@ def test_build_hotpath_from_the_reference_inputs():
 
 def test_refused_inputs_say_what_is_missing(tmp_path):
     d = copy.deepcopy(BASE)
-    d["gas"]["collision_section_type"] = 2
+    d["gas"]["collision_section_type"] = 3
     run = _load(tmp_path, d)
-    with pytest.raises(NotImplementedError, match="lxcat"):
+    with pytest.raises(NotImplementedError, match="collision_section_type 3"):
         RC.gases_and_species(run, root=str(tmp_path))
     d["gas"]["collision_section_type"] = 1
     run = _load(tmp_path, d)
@@ -458,3 +458,88 @@ def test_hard_sphere_for_the_reference_be_gas(tmp_path):
 def _xsec_text():
     from ipm2d import xsec
     return xsec.write_sigma_file(1, xsec.synthetic_electron_reactions(ion_product=1, att_product=1))
+
+
+# ------------------------------------------------------------ Pegasus gas sets
+@needs_ref
+def test_pegasus_ar_set_from_the_reference(tmp_path):
+    """input/gas/pegasus/Ar: the gas line 180 (39.95 amu, 3.68 A), one ion
+    Ar(+) (Index 181); the electron's five reactions 180000/1/6/7/8 become
+    elastic 11, ionization 13 into Ar(+), and three excitations 12 with the
+    files' thresholds; Ar(+) gets charge exchange 103 (type 6) and elastic
+    101 (type 1); cross sections are the files' values / 1e4 (cm^2 -> m^2)."""
+    d = copy.deepcopy(BASE)
+    d["gas"]["name"] = ["Ar"]
+    d["gas"]["ratio"] = [1.0]
+    d["gas"]["collision_section_type"] = 2
+    run = _load(tmp_path, d)
+    root = os.path.dirname(REF)
+    species, gases = RC.gases_and_species(run, root=root)
+    assert [sp.name for sp in species] == ["Electron", "Ar(+)"]
+    g = gases[0]
+    assert g.name == "Ar" and g.ns == 1 and g.sop_index == [181] and g.cs_model == 2
+    assert g.mass == pytest.approx(39.95 * 1.660539040e-27, rel=1e-15) and g.radius == pytest.approx(3.68e-10)
+    assert species[1].charge == pytest.approx(1.6022e-19) and species[1].radius == pytest.approx(3.33e-10)
+    sf = RC.pegasus_sigma_file(root, g, "Electron", 0)
+    assert sf.model == 1
+    assert [(r.rtype, r.resultant) for r in sf.reactions] == [(11, 0), (13, 1), (12, 0), (12, 0), (12, 0)]
+    assert [r.threshold for r in sf.reactions] == [0.0, 15.76, 11.828, 12.900, 14.090]
+    ion = sf.reactions[1]
+    assert ion.energy[13] == 40.0 and ion.sigma[13] == pytest.approx(2.39e-20, rel=1e-14)
+    sfi = RC.pegasus_sigma_file(root, g, "Ar(+)", 1)
+    assert sfi.model == 2 and [r.rtype for r in sfi.reactions] == [103, 101]
+    tabs = RC.mcc_tables(run, species, gases, root=root)
+    assert tabs[(0, 0)].model == 1 and tabs[(0, 0)].nreaction == 5
+    assert tabs[(1, 0)].model == 2 and list(tabs[(1, 0)].rtype) == [103, 101]
+
+
+def _pegasus_reaction(path, index, species, change, rtype, th=1.0):
+    path.write_text(f"""# a comment
+#
+{index}
+A+B->C
+{th}
+{len(species)}
+{' '.join(str(x) for x in species)}
+{' '.join(f'{x:+d}' for x in change)}
+{rtype}
+3
+ 1.0  1.0E-16
+ 10.0  2.0E-16
+ 100.0  3.0E-16
+""")
+
+
+@pytest.mark.parametrize("species,change,rtype,want", [
+    ((7, 1), (0, 0), 1, (11, 0)),                 # elastic
+    ((7, 1, 8), (-1, 0, 1), 100, (12, 0)),        # excitation
+    ((7, 1, 8), (-1, 1, 1), 2, (13, 1)),          # ionization into the first species
+    ((7, 1, 9), (-1, -1, 1), 3, (14, 2)),         # attachment (type 3, one product)
+    ((7, 1, 8, 9), (-1, 0, 1, 1), 3, (15, 12)),   # dissociation into two
+    ((7, 1, 9), (-1, -1, 1), 4, (14, 2)),         # type 4, the electron lost
+    ((7, 1, 9), (-1, 0, 1), 4, (12, 2)),          # type 4, the electron kept
+    ((7, 1, 8), (-1, 1, 1), 255, (13, 1)),        # type 255, an electron gained
+    ((7, 1), (0, 0), 255, (12, 0)),               # type 255, no product
+    ((7, 8), (0, 0), 1, (101, 1)),                # ion elastic
+    ((7, 8), (0, 0), 6, (103, 1)),                # ion charge exchange
+    ((7, 8, 9), (-1, -1, 1), 255, (112, 2)),      # ion type 255 with a product
+])
+def test_pegasus_reaction_codes(tmp_path, species, change, rtype, want):
+    """ConvertReactionOneBase2ReactionOne (R/src/mc/ReactionOnePegasus.f90:241-404)
+    on hand-made reaction files: particle 1 is the electron, 7 the gas, 8
+    and 9 the gas's species 1 and 2."""
+    f = tmp_path / "r"
+    _pegasus_reaction(f, 99, species, change, rtype)
+    rx = RC.read_pegasus_reaction(str(f))
+    assert rx["index"] == 99 and rx["threshold"] == 1.0 and rx["type"] == rtype
+    np.testing.assert_array_equal(rx["sigma_cm2"], [1e-16, 2e-16, 3e-16])
+    assert RC.pegasus_reaction_code(rx, [8, 9], 1) == want
+
+
+def test_pegasus_electron_flag_follows_the_second_particle(tmp_path):
+    """The source's electron test keeps only the second particle's answer:
+    a file listing the electron first and the gas second reads as an ion
+    reaction."""
+    f = tmp_path / "r"
+    _pegasus_reaction(f, 5, (1, 7), (0, 0), 1)
+    assert RC.pegasus_reaction_code(RC.read_pegasus_reaction(str(f)), [8], 1) == (101, 1)
