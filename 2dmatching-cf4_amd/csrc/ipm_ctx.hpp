@@ -71,6 +71,9 @@ struct ipm_ctx {
     double *mom = nullptr;                 // moment sums, 5 * nz * nr (ipm_moments)
     double *scratch = nullptr;             // this device's scratch array (ipm_moments_noalloc)
     double *phi = nullptr;                 // potential (ipm_set_potential), nz * nr
+    double *eps_cell = nullptr;            // cell_epsilon, (nz-1) * (nr-1) (ipm_set_solver_geometry)
+    int *ptype = nullptr;                  // point_type, nz * nr
+    double *sol = nullptr;                 // ipm_solver_system: 6 nz*nr outputs, 2 nz*nr inputs, nz*nr ints
     unsigned long long *rezone = nullptr;  // split/merge tallies (ipm_split_merge), 3
     hipGraphExec_t graph = nullptr;
     int block = 256, blocks_per_cu = 0;

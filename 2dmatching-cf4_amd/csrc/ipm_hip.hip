@@ -1093,6 +1093,57 @@ __global__ void set_e_kernel(const double *__restrict__ phi, double *__restrict_
     }
 }
 
+// FieldSolver%SetCoe (with_chi = 1) or %SetCoeWN (0), then %SetSrc and
+// %SetBound (R/src/pic/field/FieldSolver.f90:136-301), one thread per node
+// (i, j).  The five coefficients multiply phi(i-1, j), phi(i, j), phi(i+1, j),
+// phi(i, j-1) and phi(i, j+1).  Each product and sum is evaluated in the
+// reference's left-to-right order.  cell_epsilon is read at the Fortran
+// linear address (a-1) + (b-1)(Nz-1) of its (Nz-1, Nr-1) array, as the
+// reference reads it: on the axis next to the z = L boundary (i = Nz-1) CoeC
+// names cell (Nz, 1), which that address places at (1, 2).  The host checks
+// Nz, Nr >= 3, so every address stays inside the array.
+__global__ void solver_system_kernel(const double *__restrict__ chi, const double *__restrict__ eps,
+                                     const int *__restrict__ ptype, const double *__restrict__ rho,
+                                     const double *__restrict__ sigma, const double *__restrict__ inv_vol,
+                                     const double *__restrict__ bound, const int *__restrict__ mask, int with_chi,
+                                     double scale, double *__restrict__ out, int nz, int nr)
+{
+    const long long nn = (long long)nz * nr;
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nn) return;
+    const int i = (int)(k % nz) + 1, j = (int)(k / nz) + 1;
+    auto X = [&](int a, int b) { return chi[(size_t)a + (size_t)b * (nz + 2)]; };          // Chi(0:Nz+1, 0:Nr+1)
+    auto EP = [&](int a, int b) { return eps[(size_t)(a - 1) + (size_t)(b - 1) * (nz - 1)]; };
+    auto F = [&](int a, int b) { return with_chi ? 1.0 + 0.5 * (X(i, j) + X(a, b)) : 1.0; };
+    double A = 0.0, B = 0.0, C = 0.0, D = 0.0, E = 0.0;
+    if (ptype[k] > 0) {                       // metal node: phi is the electrode's
+        B = 1.0;
+    } else if (j == nr || i == 1 || i == nz) {
+        // outer boundary: the solver takes these rows from the boundary values
+    } else if (j == 1) {                      // axis
+        A = F(i - 1, j) * 0.5 * (EP(i, j) + EP(i - 1, j));
+        C = F(i + 1, j) * 0.5 * (EP(i, j) + EP(i + 1, j));
+        E = 4.0 * F(i, j + 1) * 0.5 * (EP(i, j) + EP(i, j + 1));
+        B = -(A + C + D + E);
+    } else {
+        const double jd = (double)j;
+        A = F(i - 1, j) * 0.5 * (EP(i - 1, j - 1) + EP(i - 1, j));
+        C = F(i + 1, j) * 0.5 * (EP(i, j - 1) + EP(i, j));
+        D = F(i, j - 1) * 0.5 * (EP(i - 1, j - 1) + EP(i, j - 1)) * (jd - 1.5) / (jd - 1.0);
+        E = F(i, j + 1) * 0.5 * (EP(i - 1, j) + EP(i, j)) * (jd - 0.5) / (jd - 1.0);
+        B = -(A + C + D + E);
+    }
+    out[k] = A;
+    out[nn + k] = B;
+    out[2 * nn + k] = C;
+    out[3 * nn + k] = D;
+    out[4 * nn + k] = E;
+    const double sg = sigma ? sigma[k] : 0.0;
+    double src = scale * (rho[k] + sg * inv_vol[k]);
+    if (mask && mask[k] != 0) src = bound[k];
+    out[5 * nn + k] = src;
+}
+
 __global__ void clear_kernel(double *a, size_t n)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1621,7 +1672,7 @@ int ipm_destroy(ipm_ctx *c)
                     c->metal_charge, c->sigma, c->Ez, c->Er, c->mcc_counts, c->hist, c->offs,
                     c->cursor, c->partial, c->partial2, c->d_dst, c->qbuf, c->cbuf, c->q_tile_count, c->tanc, c->p_stage,
                     c->p_meta, c->p_cnt,
-                    c->mom, c->phi, c->rezone};
+                    c->mom, c->phi, c->eps_cell, c->ptype, c->sol, c->rezone};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1815,6 +1866,50 @@ int ipm_set_potential(ipm_ctx *c, const double *phi)
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     c->fields_set = true;
+    return IPM_OK;
+}
+
+int ipm_set_solver_geometry(ipm_ctx *c, const double *cell_epsilon, const int *point_type)
+{
+    if (!c || !cell_epsilon || !point_type) return fail(IPM_EINVAL, "null argument");
+    if (c->nz < 3 || c->nr < 3) return fail(IPM_EINVAL, "the field-solver system needs Nz, Nr >= 3");
+    const size_t nn = (size_t)c->nz * c->nr, ncell = (size_t)(c->nz - 1) * (c->nr - 1);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!c->eps_cell) HIPCHK(hipMalloc(&c->eps_cell, sizeof(double) * ncell));
+    if (!c->ptype) HIPCHK(hipMalloc(&c->ptype, sizeof(int) * nn));
+    HIPCHK(hipMemcpy(c->eps_cell, cell_epsilon, sizeof(double) * ncell, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->ptype, point_type, sizeof(int) * nn, hipMemcpyHostToDevice));
+    return IPM_OK;
+}
+
+int ipm_solver_system(ipm_ctx *c, int with_chi, const double *sigma, const double *bound, const int *bound_mask,
+                      double *coe, double *source)
+{
+    if (!c) return fail(IPM_EINVAL, "null context");
+    if (!c->eps_cell || !c->ptype) return fail(IPM_EINVAL, "ipm_set_solver_geometry has not been called");
+    if (!c->geometry_set) return fail(IPM_EINVAL, "ipm_set_geometry has not been called");
+    if ((bound == nullptr) != (bound_mask == nullptr))
+        return fail(IPM_EINVAL, "bound and bound_mask are given together or not at all");
+    const size_t nn = (size_t)c->nz * c->nr;
+    if (!c->sol) HIPCHK(hipMalloc(&c->sol, sizeof(double) * 8 * nn + sizeof(int) * nn));
+    double *d_sigma = c->sol + 6 * nn, *d_bound = c->sol + 7 * nn;
+    int *d_mask = reinterpret_cast<int *>(c->sol + 8 * nn);
+    if (sigma) HIPCHK(hipMemcpyAsync(d_sigma, sigma, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+    if (bound) {
+        HIPCHK(hipMemcpyAsync(d_bound, bound, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(d_mask, bound_mask, sizeof(int) * nn, hipMemcpyHostToDevice, c->stream));
+    }
+    // FS%Source = -1.d0 * dx * dx / Epsilon * (...): Fortran's unary minus
+    // applies to the whole product, which rounds like this form
+    const double scale = -(c->dx * c->dx / kEPS0);
+    hipLaunchKernelGGL(solver_system_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream, c->chi_tot,
+                       c->eps_cell, c->ptype, c->rho_tot, sigma ? d_sigma : nullptr, c->inv_vol,
+                       bound ? d_bound : nullptr, bound ? d_mask : nullptr, with_chi ? 1 : 0, scale, c->sol, c->nz,
+                       c->nr);
+    HIPCHK(hipGetLastError());
+    if (coe) HIPCHK(hipMemcpyAsync(coe, c->sol, sizeof(double) * 5 * nn, hipMemcpyDeviceToHost, c->stream));
+    if (source) HIPCHK(hipMemcpyAsync(source, c->sol + 5 * nn, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return IPM_OK;
 }
 

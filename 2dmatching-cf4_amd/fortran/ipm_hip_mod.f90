@@ -184,6 +184,22 @@ module ipm_hip
             type(c_ptr), value :: ctx
             real(c_double), intent(in) :: phi(*)
         end function
+        integer(c_int) function ipm_set_solver_geometry(ctx, cell_epsilon, point_type) &
+                bind(C, name="ipm_set_solver_geometry")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(in) :: cell_epsilon(*)
+            integer(c_int), intent(in) :: point_type(*)
+        end function
+        integer(c_int) function ipm_solver_system(ctx, with_chi, sigma, bound, bound_mask, coe, source) &
+                bind(C, name="ipm_solver_system")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: with_chi
+            real(c_double), intent(in) :: sigma(*), bound(*)
+            integer(c_int), intent(in) :: bound_mask(*)
+            real(c_double), intent(inout) :: coe(*), source(*)
+        end function
         integer(c_int) function ipm_moments(ctx, s, implicit_j, rho, energy_ev, mp, jz, jr) &
                 bind(C, name="ipm_moments")
             import :: c_ptr, c_int, c_double
@@ -311,6 +327,28 @@ contains
         type(c_ptr), intent(in) :: ctx
         real(c_double), intent(in) :: Phi(:, :)
         call ipm_check(ipm_set_potential(ctx, Phi), "SetE")
+    end subroutine
+
+    ! FS%SetCoe(FT, Geom), FS%SetSrc(FT, MT, Geom) and FS%SetBound(boundary)
+    ! (R/src/pic/field/FieldSolver.f90:136-248) assembled on the device from
+    ! the Rho and Chi of the last FieldSourceSum.  sigma = MT%sigma, bound and
+    ! bound_mask = boundary%bound / %bound_mask; every array is (1:Nz, 1:Nr).
+    ! Geom%cell_epsilon and Geom%point_type go down once, through
+    ! ipm_set_solver_geometry.
+    subroutine ipm_SetSolverSystem(ctx, sigma, bound, bound_mask, CoeA, CoeB, CoeC, CoeD, CoeE, Source)
+        type(c_ptr), intent(in) :: ctx
+        real(c_double), intent(in) :: sigma(:, :), bound(:, :)
+        integer(c_int), intent(in) :: bound_mask(:, :)
+        real(c_double), intent(inout) :: CoeA(:, :), CoeB(:, :), CoeC(:, :), CoeD(:, :), CoeE(:, :), Source(:, :)
+        real(c_double), allocatable :: coe(:, :, :)
+        allocate(coe(size(CoeA, 1), size(CoeA, 2), 5))
+        call ipm_check(ipm_solver_system(ctx, 1_c_int, sigma, bound, bound_mask, coe, Source), "SetCoe")
+        CoeA = coe(:, :, 1)
+        CoeB = coe(:, :, 2)
+        CoeC = coe(:, :, 3)
+        CoeD = coe(:, :, 4)
+        CoeE = coe(:, :, 5)
+        deallocate(coe)
     end subroutine
 
     ! WeightingParticleField for species k: R/src/diag/diagParticleField.F90:173.

@@ -14,7 +14,10 @@
 !      calls MoveES for species 0 only and writes the result to post.bin;
 !   2. runs [steps] full steps (MoveES + absorption per species, P2CES per
 !      species, FieldSource%Sum, MCC) and prints NPar, the metal charge and
-!      the deposited charge against the charge carried by the particles.
+!      the deposited charge against the charge carried by the particles;
+!   3. after the last step's FieldSource%Sum, assembles the field solver's
+!      system (SetCoe, SetSrc, SetBound) on the device and writes Rho, Chi,
+!      CoeA..CoeE and Source to <outdir>/system.bin.
 program ipm_onestep_driver
     use iso_c_binding
     use ipm_hip
@@ -32,6 +35,9 @@ program ipm_onestep_driver
     real(8), allocatable :: vfac(:, :), vol(:, :), inv_vol(:, :)
     real(8), allocatable :: PO(:, :), phi(:, :), Ez(:, :), Er(:, :)
     real(8), allocatable :: RhoOne(:, :), ChiOne(:, :), Rho(:, :), Chi(:, :), sigma(:, :)
+    real(8), allocatable :: eps(:, :), bound(:, :), CoeA(:, :), CoeB(:, :), CoeC(:, :), CoeD(:, :), CoeE(:, :)
+    real(8), allocatable :: Source(:, :)
+    integer(c_int), allocatable :: point_type(:, :), bound_mask(:, :)
     real(8) :: metal_charge(2), wq, r, z, u1, u2, vth, vf, q_particles, q_grid, q_abs
     integer(c_int64_t) :: nget, npar
     integer(c_int) :: rt(1), res(1)
@@ -87,6 +93,22 @@ program ipm_onestep_driver
         end do
     end do
     call ipm_check(ipm_set_geometry(ctx, inv_vol, cell_material, hedge, vedge, 0, 2, mtype), "set_geometry")
+    ! field-solve geometry: relative permittivity growing with the cell
+    ! indices (distinct per cell), metal nodes on the electrode faces; the
+    ! powered electrode held at 100 V, the grounded one at 0
+    allocate(eps(Nz-1, Nr-1), point_type(Nz, Nr), bound(Nz, Nr), bound_mask(Nz, Nr))
+    do j = 1, Nr - 1
+        do i = 1, Nz - 1
+            eps(i, j) = 1.d0 + 0.01d0 * dble(i) + 0.1d0 * dble(j)
+        end do
+    end do
+    point_type = 0
+    point_type(1:2, :) = 1
+    point_type(Nz-1:Nz, :) = 2
+    bound = 0.d0
+    bound(1:2, :) = 100.d0
+    bound_mask = merge(1, 0, point_type > 0)
+    call ipm_check(ipm_set_solver_geometry(ctx, eps, point_type), "set_solver_geometry")
 
     ! electron elastic table (Model 1, one reaction, constant sigma) as
     ! ProbilityNonReactive builds it: P = sigma v n / max over the log grid
@@ -215,6 +237,14 @@ program ipm_onestep_driver
             q_abs = q_abs + abs(charge(s)) * sum(PO(8, 1:nget))
             deallocate(PO)
         end do
+        if (it == nsteps) then
+            allocate(CoeA(Nz, Nr), CoeB(Nz, Nr), CoeC(Nz, Nr), CoeD(Nz, Nr), CoeE(Nz, Nr), Source(Nz, Nr))
+            call ipm_SetSolverSystem(ctx, sigma, bound, bound_mask, CoeA, CoeB, CoeC, CoeD, CoeE, Source)
+            open(newunit=k, file=trim(outdir)//"/system.bin", access="stream", form="unformatted", &
+                 status="replace")
+            write(k) Rho, Chi, sigma, inv_vol, CoeA, CoeB, CoeC, CoeD, CoeE, Source
+            close(k)
+        end if
         call ipm_MCCStep(ctx)
         write(*, '(a, i3, a, i8, a, i8, a, es24.16, a, es24.16, a, es24.16, a, 2es12.4)') "step", it, &
             " NPar(e)", ipm_NPar(ctx, 0), " NPar(CF3+)", ipm_NPar(ctx, 1), &

@@ -72,6 +72,8 @@ SIGNATURES = {
     "ipm_synchronize": [_vp],
     "ipm_species_device_ptrs": [_vp, C.c_int, C.POINTER(_dp)],
     "ipm_set_potential": [_vp, _dp],
+    "ipm_set_solver_geometry": [_vp, _dp, _ip],
+    "ipm_solver_system": [_vp, C.c_int, _dp, _dp, _ip, _dp, _dp],
     "ipm_moments": [_vp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp],
     "ipm_recombine": [_vp, C.c_int, _ip, _dp, C.c_double, _i64p],
     "ipm_split_merge": [_vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _i64p],

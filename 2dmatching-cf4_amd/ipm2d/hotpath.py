@@ -7,6 +7,7 @@ the reference's names and argument meaning:
     material_absorption(s)         MaterialAborptionParticle   ParticleMaterial.F90:10
     p2ces(s) -> RhoOne, ChiOne     FO(s)%P2CES(PB(s), Geom)    FieldOne.f90:132
     field_source_sum() -> Rho, Chi FT%Sum(FO)                  FieldSource.f90:72
+    solver_system() -> Coe, Source FS%SetCoe/%SetSrc/%SetBound FieldSolver.f90:136-301
     mcc()                          MCC(Ns, Ng, PB, ...)        MCCPublic.F90:38
     one_step()                     OneStep's particle calls    onestep.F90:141-177
     moments(s)                     WeightingParticleField      diagParticleField.F90:173
@@ -82,6 +83,37 @@ class HotPath:
         mt = np.ascontiguousarray(geom.material_type, dtype=np.int32)
         N.call("ipm_set_geometry", self._h, _d(inv), _i(cm), _i(he), _i(ve), geom.ndiel, geom.nmetal, _i(mt))
         self.nmetal = geom.nmetal
+        if self.nz >= 3 and self.nr >= 3:
+            self.set_solver_geometry(geom.cell_epsilon, geom.point_type)
+
+    def set_solver_geometry(self, cell_epsilon, point_type):
+        """Geom%cell_epsilon (nz-1, nr-1) and Geom%point_type (nz, nr) of the field solve."""
+        eps = np.ascontiguousarray(fortran(np.asarray(cell_epsilon, dtype=np.float64)))
+        pt = np.ascontiguousarray(fortran(np.asarray(point_type)).astype(np.int32))
+        assert eps.size == (self.nz - 1) * (self.nr - 1) and pt.size == self.nz * self.nr
+        N.call("ipm_set_solver_geometry", self._h, _d(eps), _i(pt))
+
+    def solver_system(self, with_chi=True, sigma=None, bound=None, bound_mask=None):
+        """FS%SetCoe (or %SetCoeWN when with_chi is False), %SetSrc and
+        %SetBound (FieldSolver.f90:136-301) on the device, from the Rho and
+        Chi of the last field_source_sum / one_step.  sigma = MT%sigma,
+        bound / bound_mask = boundary%bound / %bound_mask, each (nz, nr) or
+        None.  Returns ({"A".."E": (nz, nr)}, source (nz, nr))."""
+        nn = self.nz * self.nr
+
+        def host(a, dt):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(fortran(np.asarray(a)).astype(dt))
+            assert a.size == nn
+            return a
+        sg, bd, mk = host(sigma, np.float64), host(bound, np.float64), host(bound_mask, np.int32)
+        coe = np.zeros(5 * nn)
+        src = np.zeros(nn)
+        N.call("ipm_solver_system", self._h, 1 if with_chi else 0, None if sg is None else _d(sg),
+               None if bd is None else _d(bd), None if mk is None else _i(mk), _d(coe), _d(src))
+        grids = coe.reshape(5, self.nr, self.nz).transpose(0, 2, 1)
+        return dict(zip("ABCDE", grids)), src.reshape(self.nr, self.nz).T
 
     def set_mcc(self, s, g, b):
         """Upload one MCCBundle (ipm2d.xsec.MCCBundle or model 0 = none)."""

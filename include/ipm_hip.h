@@ -181,6 +181,29 @@ int ipm_species_device_ptrs(ipm_ctx *ctx, int s, double **fields8);
  * the potential: one grid crosses PCIe instead of two. */
 int ipm_set_potential(ipm_ctx *ctx, const double *phi);
 
+/* Geometry of the field solve (R/src/base/Geometry.F90:21-45):
+ * cell_epsilon = Geom%cell_epsilon ((Nz-1)*(Nr-1) relative permittivities)
+ * and point_type = Geom%point_type (Nz*Nr, > 0 on metal nodes).  Needs
+ * Nz, Nr >= 3. */
+int ipm_set_solver_geometry(ipm_ctx *ctx, const double *cell_epsilon, const int *point_type);
+/* The linear system of the host Poisson solve, assembled on the device from
+ * the Rho and Chi of the last ipm_field_sum / ipm_step / ipm_step_graph:
+ * FS%SetCoe(FT, Geom) (with_chi != 0) or FS%SetCoeWN(Geom) (with_chi == 0),
+ * then FS%SetSrc(FT, MT, Geom) and FS%SetBound(boundary)
+ * (R/src/pic/field/FieldSolver.f90:136-301).
+ *   coe    (5*Nz*Nr, NULL to skip): CoeA, CoeB, CoeC, CoeD, CoeE, each Nz*Nr,
+ *          the five-point coefficients of phi(i-1,j), phi(i,j), phi(i+1,j),
+ *          phi(i,j-1), phi(i,j+1) at node (i, j);
+ *   source (Nz*Nr, NULL to skip): -dx^2/Epsilon * (Rho + sigma *
+ *          point_inv_volume_weighting), then bound where bound_mask != 0;
+ *   sigma  (Nz*Nr) = MT%sigma, NULL for zero; bound (Nz*Nr doubles) and
+ *          bound_mask (Nz*Nr ints) = boundary%bound / %bound_mask, both
+ *          NULL to merge nothing.
+ * These are the arrays poisson_solve_ receives (FieldSolver.f90:307-316):
+ * the host's pass over the grid is replaced by one device kernel. */
+int ipm_solver_system(ipm_ctx *ctx, int with_chi, const double *sigma, const double *bound, const int *bound_mask,
+                      double *coe, double *source);
+
 /* Moment diagnostics of species s: WeightingParticleField
  * (R/src/diag/diagParticleField.F90:173-318).  Each output is Nz*Nr (or
  * NULL to skip): rho = RhoOne (number density), energy_ev = TOne (mean

@@ -16,6 +16,12 @@
  *   ora_deposit_*         charge conservation, on-node known answers,
  *                         uniform density vs analytic node volumes
  *   ora_node_volumes      closed-form cylindrical shell volumes
+ *   ora_solver_coe/_source  uniform medium = the cylindrical five-point
+ *                         Laplacian; a quadratic potential's stencil gives
+ *                         (2a + 4b)(1 + chi) eps; flux symmetry A(i+1,j) =
+ *                         C(i,j), (j-1) E(i,j) = j D(i,j+1) for any medium;
+ *                         hand-worked axis and aliased-cell values; Chi = 0
+ *                         form = form without Chi; source closed form
  *   ora_set_e             linear potentials give the uniform field exactly
  *                         on every face including the ghost-fed ones;
  *                         numpy restatement agrees bit for bit
@@ -574,6 +580,84 @@ void ora_set_e(int nz, int nr, double dx, const double *phi, double *Ez, double 
         }
 #undef PP
     free(p);
+}
+
+/* The r-z Poisson coefficients, R/src/pic/field/FieldSolver.f90:174-301:
+ * SetFieldSolverCoe (with_chi = 1, lines 174-248) or
+ * SetFieldSolverCoeWithoutChi (with_chi = 0, lines 251-301), restated loop
+ * for loop on a one-rank domain (zstart..zend =
+ * 1..Nz, rstart..rend = 1..Nr).  chi is Chi(0:Nz+1, 0:Nr+1); cell_epsilon is
+ * (1:Nz-1, 1:Nr-1) and read at the Fortran address the reference computes
+ * (EPSR below), so the axis term at i = Nz-1 that names cell (Nz, 1) reads
+ * the element stored there, cell (1, 2).  coe = CoeA..CoeE, each Nz*Nr. */
+void ora_solver_coe(int nz, int nr, int with_chi, const double *chi, const double *cell_epsilon,
+                    const int *point_type, double *coe)
+{
+    const size_t nn = (size_t)nz * nr;
+    double *coeA = coe, *coeB = coe + nn, *coeC = coe + 2 * nn, *coeD = coe + 3 * nn, *coeE = coe + 4 * nn;
+#define CHI(i, j) chi[(size_t)(i) + (size_t)(j) * (size_t)(nz + 2)]
+#define EPSR(i, j) cell_epsilon[(size_t)((i) - 1) + (size_t)((j) - 1) * (size_t)(nz - 1)]
+#define AT(a, i, j) a[(size_t)((i) - 1) + (size_t)((j) - 1) * (size_t)nz]
+    for (size_t k = 0; k < 5 * nn; ++k) coe[k] = 0.0;
+    for (int j = 1; j <= nr; ++j)
+        for (int i = 1; i <= nz; ++i) {
+            const int point_type_one = AT(point_type, i, j);
+            if (point_type_one > 0) {
+                AT(coeB, i, j) = 1.0;
+            } else if (nr == j || 1 == i || nz == i) {
+                /* boundary: not used */
+            } else if (1 == j) {
+                if (with_chi) {
+                    AT(coeA, i, j) = (1.0 + 0.5 * (CHI(i, j) + CHI(i - 1, j))) * 0.5 * (EPSR(i, j) + EPSR(i - 1, j));
+                    AT(coeC, i, j) = (1.0 + 0.5 * (CHI(i, j) + CHI(i + 1, j))) * 0.5 * (EPSR(i, j) + EPSR(i + 1, j));
+                    AT(coeE, i, j) =
+                        4.0 * (1.0 + 0.5 * (CHI(i, j) + CHI(i, j + 1))) * 0.5 * (EPSR(i, j) + EPSR(i, j + 1));
+                } else {
+                    AT(coeA, i, j) = 0.5 * (EPSR(i, j) + EPSR(i - 1, j));
+                    AT(coeC, i, j) = 0.5 * (EPSR(i, j) + EPSR(i + 1, j));
+                    AT(coeE, i, j) = 4.0 * 0.5 * (EPSR(i, j) + EPSR(i, j + 1));
+                }
+                AT(coeB, i, j) = -(AT(coeA, i, j) + AT(coeC, i, j) + AT(coeD, i, j) + AT(coeE, i, j));
+            } else {
+                if (with_chi) {
+                    AT(coeA, i, j) =
+                        (1.0 + 0.5 * (CHI(i, j) + CHI(i - 1, j))) * 0.5 * (EPSR(i - 1, j - 1) + EPSR(i - 1, j));
+                    AT(coeC, i, j) = (1.0 + 0.5 * (CHI(i, j) + CHI(i + 1, j))) * 0.5 * (EPSR(i, j - 1) + EPSR(i, j));
+                    AT(coeD, i, j) = (1.0 + 0.5 * (CHI(i, j) + CHI(i, j - 1))) * 0.5 *
+                                     (EPSR(i - 1, j - 1) + EPSR(i, j - 1)) * (j - 1.5) / (j - 1.0);
+                    AT(coeE, i, j) = (1.0 + 0.5 * (CHI(i, j) + CHI(i, j + 1))) * 0.5 * (EPSR(i - 1, j) + EPSR(i, j)) *
+                                     (j - 0.5) / (j - 1.0);
+                } else {
+                    AT(coeA, i, j) = 0.5 * (EPSR(i - 1, j - 1) + EPSR(i - 1, j));
+                    AT(coeC, i, j) = 0.5 * (EPSR(i, j - 1) + EPSR(i, j));
+                    AT(coeD, i, j) = 0.5 * (EPSR(i - 1, j - 1) + EPSR(i, j - 1)) * (j - 1.5) / (j - 1.0);
+                    AT(coeE, i, j) = 0.5 * (EPSR(i - 1, j) + EPSR(i, j)) * (j - 0.5) / (j - 1.0);
+                }
+                AT(coeB, i, j) = -(AT(coeA, i, j) + AT(coeC, i, j) + AT(coeD, i, j) + AT(coeE, i, j));
+            }
+        }
+#undef CHI
+#undef EPSR
+#undef AT
+}
+
+/* SetFieldSolverSource then SetSolverBoundary, FieldSolver.f90:153-171 and
+ * 136-150:
+ * Source = -1 * dx * dx / Epsilon * (Rho + sigma * inv_vol) (Fortran's
+ * unary minus applies to the whole product), then Source = bound where
+ * bound_mask /= 0.  sigma, bound and bound_mask may be NULL (zero / no
+ * merge).  Every array is Nz*Nr. */
+void ora_solver_source(int nz, int nr, double dx, const double *rho, const double *sigma, const double *inv_vol,
+                       const double *bound, const int *bound_mask, double *source)
+{
+    const size_t nn = (size_t)nz * nr;
+    for (size_t k = 0; k < nn; ++k) {
+        const double sg = sigma ? sigma[k] : 0.0;
+        source[k] = -(1.0 * dx * dx / C_EPS0 * (rho[k] + sg * inv_vol[k]));
+    }
+    if (bound && bound_mask)
+        for (size_t k = 0; k < nn; ++k)
+            if (bound_mask[k] != 0) source[k] = bound[k];
 }
 
 /* WeightingParticleField for one species, R/src/diag/diagParticleField.F90:196-257

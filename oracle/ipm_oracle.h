@@ -72,6 +72,16 @@ void ora_deposit_finish(int nz, int nr, double dt, double charge, double mass,
  * PICCom.F90:516-578): phi (1:Nz,1:Nr) -> Ez (0:Nz,1:Nr), Er (1:Nz,0:Nr). */
 void ora_set_e(int nz, int nr, double dx, const double *phi, double *Ez, double *Er);
 
+/* The host Poisson system, R/src/pic/field/FieldSolver.f90:136-301:
+ * SetFieldSolverCoe (with_chi = 1) / SetFieldSolverCoeWithoutChi (0) into
+ * coe = CoeA..CoeE (5*Nz*Nr; chi is Chi(0:Nz+1, 0:Nr+1), cell_epsilon
+ * (Nz-1)*(Nr-1), point_type Nz*Nr), and SetFieldSolverSource +
+ * SetSolverBoundary into source (Nz*Nr; sigma, bound, bound_mask may be NULL). */
+void ora_solver_coe(int nz, int nr, int with_chi, const double *chi, const double *cell_epsilon,
+                    const int *point_type, double *coe);
+void ora_solver_source(int nz, int nr, double dx, const double *rho, const double *sigma, const double *inv_vol,
+                       const double *bound, const int *bound_mask, double *source);
+
 /* Recombination (Recombination.F90:164-273): cell densities of one bundle,
  * then the whole call on ns bundles (P: 8 arrays per species). */
 void ora_recomb_density(int nz, int nr, double dx, double dt, double mass, int want_t, int64_t n,

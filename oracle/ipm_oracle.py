@@ -67,6 +67,10 @@ def lib(fast=False):
     L.ora_deposit_raw.argtypes = [C.c_int, C.c_int, C.c_int64, _dp, _dp, _dp, _dp]
     L.ora_deposit_finish.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _dp]
     L.ora_set_e.argtypes = [C.c_int, C.c_int, C.c_double, _dp, _dp, _dp]
+    L.ora_solver_coe.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _ip, _dp]
+    L.ora_solver_coe.restype = None
+    L.ora_solver_source.argtypes = [C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp, _ip, _dp]
+    L.ora_solver_source.restype = None
     L.ora_split_merge.argtypes = [_i64p, C.POINTER(_dp), C.c_int64, C.c_int, C.c_double, C.c_double, C.c_double,
                                   C.c_double, C.c_uint64, C.c_uint32, C.c_int, _i64p]
     L.ora_split_merge.restype = C.c_int
@@ -276,6 +280,30 @@ def set_e(phi, dx):
     er = np.zeros(nz * (nr + 1))
     lib().ora_set_e(nz, nr, dx, _d(ph), _d(ez), _d(er))
     return ez, er
+
+
+def solver_coe(chi, cell_epsilon, point_type, with_chi=True):
+    """SetFieldSolverCoe / SetFieldSolverCoeWithoutChi: chi (nz+2, nr+2),
+    cell_epsilon (nz-1, nr-1), point_type (nz, nr) -> {"A".."E": (nz, nr)}."""
+    nz, nr = point_type.shape
+    assert chi.shape == (nz + 2, nr + 2) and cell_epsilon.shape == (nz - 1, nr - 1)
+    ch, ep = F(chi), F(cell_epsilon)
+    pt = np.ascontiguousarray(np.asarray(point_type).T.reshape(-1), dtype=np.int32)
+    out = np.zeros(5 * nz * nr)
+    lib().ora_solver_coe(nz, nr, 1 if with_chi else 0, _d(ch), _d(ep), pt.ctypes.data_as(_ip), _d(out))
+    return dict(zip("ABCDE", out.reshape(5, nr, nz).transpose(0, 2, 1)))
+
+
+def solver_source(dx, rho, inv_vol, sigma=None, bound=None, bound_mask=None):
+    """SetFieldSolverSource + SetSolverBoundary on (nz, nr) grids."""
+    nz, nr = rho.shape
+    opt = [None if a is None else F(a) for a in (sigma, bound)]
+    mk = None if bound_mask is None else np.ascontiguousarray(np.asarray(bound_mask).T.reshape(-1), dtype=np.int32)
+    out = np.zeros(nz * nr)
+    lib().ora_solver_source(nz, nr, dx, _d(F(rho)), None if opt[0] is None else _d(opt[0]), _d(F(inv_vol)),
+                            None if opt[1] is None else _d(opt[1]), None if mk is None else mk.ctypes.data_as(_ip),
+                            _d(out))
+    return out.reshape(nr, nz).T
 
 
 def moments(nz, nr, dx, dt, charge, mass, ez_flat, er_flat, P, inv_vol, implicit_j=True):

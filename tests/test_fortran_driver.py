@@ -46,3 +46,23 @@ def test_fortran_driver_onestep(oracle, tmp_path):
         qp = float(re.search(r"q_particles\s+(\S+)", l).group(1))
         qa = float(re.search(r"q_abs\s+(\S+)", l).group(1))
         assert qa > 0 and abs(qg - qp) <= 1e-12 * qa, l
+    # the solver system of the last step, assembled on the device through the
+    # Fortran binding, against the oracle on the Rho and Chi the driver read
+    sysf = np.fromfile(tmp_path / "system.bin", dtype=np.float64)
+    nn, nc = nz * nr, (nz + 2) * (nr + 2)
+    assert sysf.size == 9 * nn + nc
+    grid = lambda a: a.reshape(nr, nz).T
+    rho, chi = grid(sysf[:nn]), sysf[nn:nn + nc].reshape(nr + 2, nz + 2).T
+    sigma, iv = grid(sysf[nn + nc:2 * nn + nc]), grid(sysf[2 * nn + nc:3 * nn + nc])
+    coe = [grid(sysf[(3 + k) * nn + nc:(4 + k) * nn + nc]) for k in range(6)]
+    assert np.abs(chi).max() > 0
+    i, j = np.meshgrid(np.arange(1, nz), np.arange(1, nr), indexing="ij")
+    eps = 1.0 + 0.01 * i + 0.1 * j
+    pt = np.zeros((nz, nr), dtype=np.int32)
+    pt[:2, :], pt[-2:, :] = 1, 2
+    bound = np.where(pt == 1, 100.0, 0.0)
+    ref = oracle.solver_coe(chi, eps, pt, with_chi=True)
+    for k, name in enumerate("ABCDE"):
+        assert np.array_equal(coe[k], ref[name]), name
+    src = oracle.solver_source(dx, rho, iv, sigma=sigma, bound=bound, bound_mask=(pt > 0).astype(np.int32))
+    assert np.array_equal(coe[5], src)
