@@ -60,7 +60,6 @@ struct ipm_ctx {
     std::vector<void *> tab_mem[IPM_NS_MAX * IPM_NG_MAX];
     unsigned long long *hist = nullptr, *offs = nullptr, *cursor = nullptr, *partial = nullptr,
                        *partial2 = nullptr;
-    double **d_dst = nullptr;
     double *qbuf = nullptr;                // q_cap records of 8 doubles
     double *cbuf = nullptr;                // collision records, kCollFields arrays of q_cap doubles
     double *p_stage = nullptr;             // per collision block: staged products, meta, counts
@@ -76,6 +75,8 @@ struct ipm_ctx {
     double *sol = nullptr;                 // ipm_solver_system: 6 nz*nr outputs, 2 nz*nr inputs, nz*nr ints
     unsigned long long *rezone = nullptr;  // split/merge tallies (ipm_split_merge), 3
     hipGraphExec_t graph = nullptr;
+    hipGraphExec_t sort_graph = nullptr;   // cell sort of every species (ipm_step_graph's interval sorts)
+    int sort_every = 0, since_sort = 0;    // ipm_set_sort_interval
     int block = 256, blocks_per_cu = 0;
     int pk_block = 256, pk_winr = 24;      // fused particle pass: threads per block, deposit window rows
     int collide_bpc = 6;                   // collision-pass blocks per compute unit
@@ -109,6 +110,29 @@ inline int read_counters(ipm_ctx *c, DevCounters *out)
     if (out->overflow & 4) return fail(IPM_ECAPACITY, "collision queue overflow (selections were not collided)");
     if (out->overflow & 8) return fail(IPM_EINVAL, "dead-slot count disagrees with the particle arrays");
     return IPM_OK;
+}
+
+// Species s's current buffer set becomes set `cur` of its two; f[] and alt[]
+// of the host copy of DevState follow.  The device copy is updated by
+// sync_state or, after a cell sort, by the sort's own swap kernel.
+inline void set_buffers(ipm_ctx *c, int s, int cur)
+{
+    const size_t cap = ((size_t)c->h.cap[s] + 31) & ~(size_t)31;
+    c->cur[s] = cur;
+    for (int f = 0; f < 8; ++f) {
+        c->h.f[s][f] = c->buf[cur][s] + (size_t)f * cap;
+        c->h.alt[s][f] = c->buf[1 - cur][s] + (size_t)f * cap;
+    }
+}
+
+// Captured graphs bake in grid sizes derived from the capacities and the
+// launch shape: any change to those drops them (recaptured on next use).
+inline void drop_graphs(ipm_ctx *c)
+{
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    if (c->sort_graph) (void)hipGraphExecDestroy(c->sort_graph);
+    c->graph = nullptr;
+    c->sort_graph = nullptr;
 }
 
 inline int check_ready(ipm_ctx *c, bool need_fields, bool need_geom)

@@ -44,6 +44,7 @@ struct DevState {
     int nz, nr, ns, ng;
     double dx, dt, vfactor, inv_vfactor;
     double *f[IPM_NS_MAX][8];          // Z R Vz Vr Vt Az Ar WQ
+    double *alt[IPM_NS_MAX][8];        // the other buffer set: the cell sort's target
     long long cap[IPM_NS_MAX];
     double charge[IPM_NS_MAX], mass[IPM_NS_MAX], efactor[IPM_NS_MAX];
     double *raw[IPM_NS_MAX];           // raw per-species deposit (nz*nr)

@@ -1278,17 +1278,20 @@ struct SortTileLds {
 
 // SCATTER false: counts[cell] += particles of the cell (the histogram).
 // SCATTER true: counts[cell] hands out ranks within the cell (zeroed before),
-// and each particle's eight fields go to dst at offs[cell] + rank.
+// and each particle's eight fields go to the species' other buffer set at
+// offs[cell] + rank.  The grid covers the capacity; blocks past the
+// population return at once, so a captured launch fits any population.
 template <bool SCATTER>
 __global__ void __launch_bounds__(kBlock)
 sort_tile_kernel(const DevState *__restrict__ stp, const DevCounters *__restrict__ ctr, int s, int end_bit,
-                 unsigned long long *__restrict__ counts, const unsigned long long *__restrict__ offs,
-                 double *const *__restrict__ dst)
+                 unsigned long long *__restrict__ counts, const unsigned long long *__restrict__ offs)
 {
     __shared__ SortTileLds L;
+    __shared__ double stage[SCATTER ? kSortTile : 1];   // one field of the tile (scatter pass)
     const DevState &st = *stp;
     const long long n = (long long)ctr->n[s];
     const long long tbase = (long long)blockIdx.x * kSortTile;
+    if (tbase >= n) return;
     const unsigned S = (unsigned)((st.nz - 1) * (st.nr - 1));
     const int t = threadIdx.x;
     unsigned key[kSortItems];
@@ -1334,18 +1337,46 @@ sort_tile_kernel(const DevState *__restrict__ stp, const DevCounters *__restrict
         L.key[p] = key[j] < S ? (unsigned)(offs[key[j]] + L.base[start[j]] + (unsigned)(p - start[j])) : 0xffffffffu;
         L.tmp.src[p] = loc[j];
     }
-    __syncthreads();
-    // striped copy: neighbouring lanes write neighbouring slots of a run
-#pragma unroll 2
-    for (int j = 0; j < kSortItems; ++j) {
-        const int p = t + j * kBlock;
-        const unsigned d = L.key[p];
-        if (d != 0xffffffffu) {
-            const long long i = tbase + L.tmp.src[p];
+    // Field by field: the tile's values are read coalesced (non-temporal:
+    // read once) into LDS, then written in sorted order, neighbouring lanes
+    // to neighbouring slots of a run.  Reading them straight from HBM in
+    // sorted order costs a cache line per lane.  Slots past the population
+    // are not loaded; their sorted positions carry no destination.
+    const long long ntile = n - tbase;
+    for (int f = 0; f < 8; ++f) {
+        const double *sf = st.f[s][f] + tbase;
 #pragma unroll
-            for (int f = 0; f < 8; ++f) dst[f][d] = st.f[s][f][i];
+        for (int j = 0; j < kSortItems; ++j) {
+            const int l = t + j * kBlock;
+            if (l < ntile) stage[l] = __builtin_nontemporal_load(sf + l);
         }
+        __syncthreads();                    // stage (and, for f = 0, key/src) written
+        double *df = st.alt[s][f];
+#pragma unroll
+        for (int j = 0; j < kSortItems; ++j) {
+            const int p = t + j * kBlock;
+            const unsigned d = L.key[p];
+            if (d != 0xffffffffu) df[d] = stage[L.tmp.src[p]];
+        }
+        __syncthreads();                    // stage read before the next field overwrites it
     }
+}
+
+// After the scatter: the sorted set becomes species s's current buffer set.
+__global__ void swap_buffers_kernel(DevState *st, int s)
+{
+    if (threadIdx.x < 8) {
+        const int f = threadIdx.x;
+        double *a = st->f[s][f];
+        st->f[s][f] = st->alt[s][f];
+        st->alt[s][f] = a;
+    }
+}
+
+__global__ void clear_u64_kernel(unsigned long long *a, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0ull;
 }
 
 // Exclusive scan of n counts: per-block scan + block totals, then offsets.
@@ -1390,13 +1421,15 @@ __global__ void scan_add_kernel(unsigned long long *__restrict__ out, const unsi
 // Deposit-window anchors of a species' tiles in the order a sort left: each
 // tile's window is placed by its first particle, as the particle pass does
 // for the pass after it.  anc points at the species' first tile.
-__global__ void anchor_kernel(const DevCounters *__restrict__ ctr, int s, const double *__restrict__ Z,
-                              const double *__restrict__ R, int *__restrict__ anc, long long max_tiles, int arows)
+__global__ void anchor_kernel(const DevState *__restrict__ stp, const DevCounters *__restrict__ ctr, int s,
+                              int *__restrict__ anc, long long max_tiles, int arows)
 {
     const long long n = (long long)ctr->n[s];
     const long long nt = min((n + kTile - 1) / kTile, max_tiles);
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nt) return;
+    const double *__restrict__ Z = stp->f[s][0];
+    const double *__restrict__ R = stp->f[s][1];
     const double z = Z[t * kTile], r = R[t * kTile];
     if (is_dead(z) || !(z > 0.0) || !(r >= 0.0) || z >= 1e9 || r >= 1e9) return;
     anc[2 * t] = (int)ceil(z) - kAnchorZ;
@@ -1492,10 +1525,7 @@ static int ensure_queue(ipm_ctx *c)
         HIPCHK(hipMemset(c->q_tile_count, 0, sizeof(unsigned) * (size_t)ntiles));
         c->q_cap = need;
         c->q_ntiles = ntiles;
-        if (c->graph) {
-            (void)hipGraphExecDestroy(c->graph);
-            c->graph = nullptr;
-        }
+        drop_graphs(c);
     }
     if (!c->p_stage) {
         const size_t G = (size_t)c->num_cu * kCollideBlocksPerCuMax;
@@ -1586,6 +1616,71 @@ static int launch_grid(ipm_ctx *c, int s0, int s1, int sum, int fold = 0)
     return IPM_OK;
 }
 
+// Enqueues the cell sort of species s0..s1-1 (device work only, no host
+// synchronization, capturable): per species, histogram, scan, scatter into
+// the other buffer set, new count, buffer swap on the device, window anchors.
+// Grids are sized from the capacities.  The caller mirrors the swap on the
+// host with sort_mirror once the work is enqueued (or replayed).
+static int sort_enqueue(ipm_ctx *c, int s0, int s1)
+{
+    const size_t ncell = (size_t)(c->nz - 1) * (c->nr - 1);
+    const unsigned nblk = (unsigned)((ncell + kScanBlock - 1) / kScanBlock);
+    if (nblk > kScanBlock) return fail(IPM_EINVAL, "grid too large for the two-level scan");
+    int end_bit = 0;                        // bits of the sentinel key ncell
+    while (end_bit < 32 && (ncell >> end_bit) != 0) ++end_bit;
+    const unsigned cblk = (unsigned)((ncell + kBlock - 1) / kBlock);
+    for (int sp = s0; sp < s1; ++sp) {
+        if (!c->species_set[sp]) continue;
+        hipLaunchKernelGGL(clear_u64_kernel, dim3(cblk), dim3(kBlock), 0, c->stream, c->hist, ncell);
+        hipLaunchKernelGGL(clear_u64_kernel, dim3(cblk), dim3(kBlock), 0, c->stream, c->cursor, ncell);
+        const unsigned pblk = (unsigned)(((size_t)c->h.cap[sp] + kSortTile - 1) / kSortTile);
+        hipLaunchKernelGGL(sort_tile_kernel<false>, dim3(pblk), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr, sp,
+                           end_bit, c->hist, (const unsigned long long *)nullptr);
+        hipLaunchKernelGGL(scan_block_kernel, dim3(nblk), dim3(kScanBlock), 0, c->stream, c->hist, c->offs, c->partial, ncell);
+        hipLaunchKernelGGL(scan_block_kernel, dim3(1), dim3(kScanBlock), 0, c->stream, c->partial, c->partial2,
+                           c->partial + nblk, (size_t)nblk);
+        hipLaunchKernelGGL(scan_add_kernel, dim3(nblk), dim3(kScanBlock), 0, c->stream, c->offs, c->partial2, ncell);
+        hipLaunchKernelGGL(sort_tile_kernel<true>, dim3(pblk), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr, sp,
+                           end_bit, c->cursor, c->offs);
+        hipLaunchKernelGGL(set_count_kernel, dim3(1), dim3(64), 0, c->stream, c->d_ctr, sp, c->offs, c->hist, ncell);
+        hipLaunchKernelGGL(swap_buffers_kernel, dim3(1), dim3(64), 0, c->stream, c->d_state, sp);
+        if (c->tanc) {
+            const long long max_tiles = ((long long)c->h.cap[sp] + kTile - 1) / kTile;
+            if (c->h.q_cbase[sp] + max_tiles <= c->q_ntiles) {
+                const unsigned ablk = (unsigned)((max_tiles + kBlock - 1) / kBlock);
+                hipLaunchKernelGGL(anchor_kernel, dim3(ablk), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr, sp,
+                                   c->tanc + 2 * c->h.q_cbase[sp], max_tiles, anchor_rows(c->pk_winr));
+            }
+        }
+        HIPCHK(hipGetLastError());
+    }
+    return IPM_OK;
+}
+
+static void sort_mirror(ipm_ctx *c, int s0, int s1)
+{
+    for (int sp = s0; sp < s1; ++sp)
+        if (c->species_set[sp]) set_buffers(c, sp, 1 - c->cur[sp]);
+}
+
+// The sort of every species as a captured graph (captured on first use).
+static int sort_graph_launch(ipm_ctx *c)
+{
+    if (!c->sort_graph) {
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = sort_enqueue(c, 0, c->ns);
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc) return rc;
+        if (e != hipSuccess) return fail(IPM_EHIP, "graph capture failed: %s", hipGetErrorString(e));
+        HIPCHK(hipGraphInstantiate(&c->sort_graph, g, nullptr, nullptr, 0));
+        HIPCHK(hipGraphDestroy(g));
+    }
+    HIPCHK(hipGraphLaunch(c->sort_graph, c->stream));
+    sort_mirror(c, 0, c->ns);
+    return IPM_OK;
+}
+
 extern "C" {
 
 const char *ipm_last_error(void) { return g_err; }
@@ -1643,7 +1738,6 @@ int ipm_create(ipm_ctx **out, int nz, int nr, double dx, double dt, int ns, int 
     HIPCHK(hipMalloc(&c->cursor, sizeof(unsigned long long) * ncell));
     HIPCHK(hipMalloc(&c->partial, sizeof(unsigned long long) * (nblk + 1)));
     HIPCHK(hipMalloc(&c->partial2, sizeof(unsigned long long) * (nblk + 1)));
-    HIPCHK(hipMalloc(&c->d_dst, sizeof(double *) * 8));
     for (int s = 0; s < ns; ++s) h.raw[s] = c->raw_all + (size_t)s * nn;
     h.Ez = c->Ez; h.Er = c->Er; h.sigma = c->sigma; h.mcc_counts = c->mcc_counts;
     // no geometry yet: a 1-metal, all-vacuum placeholder keeps pointers valid
@@ -1660,7 +1754,7 @@ int ipm_destroy(ipm_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     (void)window_release(c);
     (void)recomb_release(c);
-    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    drop_graphs(c);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     for (int b = 0; b < 2; ++b)
         for (int s = 0; s < IPM_NS_MAX; ++s)
@@ -1670,7 +1764,7 @@ int ipm_destroy(ipm_ctx *c)
     void *ptrs[] = {c->d_state, c->d_ctr, c->raw_all, c->rho_one, c->chi_one, c->rho_tot, c->chi_tot,
                     c->d_chifac, c->inv_vol, c->cell_material, c->hedge, c->vedge, c->mat_type,
                     c->metal_charge, c->sigma, c->Ez, c->Er, c->mcc_counts, c->hist, c->offs,
-                    c->cursor, c->partial, c->partial2, c->d_dst, c->qbuf, c->cbuf, c->q_tile_count, c->tanc, c->p_stage,
+                    c->cursor, c->partial, c->partial2, c->qbuf, c->cbuf, c->q_tile_count, c->tanc, c->p_stage,
                     c->p_meta, c->p_cnt,
                     c->mom, c->phi, c->eps_cell, c->ptype, c->sol, c->rezone};
     for (void *p : ptrs)
@@ -1694,9 +1788,9 @@ int ipm_set_species(ipm_ctx *c, int s, double charge, double mass, int64_t capac
     }
     const size_t cap = ((size_t)capacity + 31) & ~(size_t)31;   // 256-byte aligned arrays
     for (int b = 0; b < 2; ++b) HIPCHK(hipMalloc(&c->buf[b][s], sizeof(double) * 8 * cap));
-    c->cur[s] = 0;
-    for (int f = 0; f < 8; ++f) c->h.f[s][f] = c->buf[0][s] + (size_t)f * cap;
     c->h.cap[s] = (long long)capacity;
+    set_buffers(c, s, 0);
+    drop_graphs(c);
     c->h.charge[s] = charge;
     c->h.mass[s] = mass;
     // MoveESIm: EFactor = Charge/Mass*dt/(dx/dt)*0.5 (ParticleBundle.f90:682)
@@ -2208,6 +2302,11 @@ int ipm_step(ipm_ctx *c)
 {
     int rc = check_ready(c, true, true);
     if (rc || (rc = step_enqueue(c, false))) return rc;
+    if (c->sort_every > 0 && ++c->since_sort >= c->sort_every) {
+        if ((rc = sort_enqueue(c, 0, c->ns))) return rc;
+        sort_mirror(c, 0, c->ns);
+        c->since_sort = 0;
+    }
     return IPM_OK;
 }
 
@@ -2255,6 +2354,10 @@ int ipm_step_graph(ipm_ctx *c)
         HIPCHK(hipGraphDestroy(g));
     }
     HIPCHK(hipGraphLaunch(c->graph, c->stream));
+    if (c->sort_every > 0 && ++c->since_sort >= c->sort_every) {
+        if ((rc = sort_graph_launch(c))) return rc;
+        c->since_sort = 0;
+    }
     return IPM_OK;
 }
 
@@ -2271,49 +2374,21 @@ int ipm_sort(ipm_ctx *c, int s)
     int rc = check_ready(c, false, false);
     if (rc || (rc = check_sel(c, s))) return rc;
     const int s0 = s < 0 ? 0 : s, s1 = s < 0 ? c->ns : s + 1;
-    const size_t ncell = (size_t)(c->nz - 1) * (c->nr - 1);
-    const unsigned nblk = (unsigned)((ncell + kScanBlock - 1) / kScanBlock);
-    if (nblk > kScanBlock) return fail(IPM_EINVAL, "grid too large for the two-level scan");
-    int end_bit = 0;                        // bits of the sentinel key ncell
-    while (end_bit < 32 && (ncell >> end_bit) != 0) ++end_bit;
     DevCounters ctr;
     if ((rc = read_counters(c, &ctr))) return rc;
-    for (int sp = s0; sp < s1; ++sp) {
-        const size_t n = (size_t)ctr.n[sp];
-        HIPCHK(hipMemsetAsync(c->hist, 0, sizeof(unsigned long long) * ncell, c->stream));
-        HIPCHK(hipMemsetAsync(c->cursor, 0, sizeof(unsigned long long) * ncell, c->stream));
-        const unsigned pblk = (unsigned)((n + kSortTile - 1) / kSortTile);
-        if (pblk > 0)
-            hipLaunchKernelGGL(sort_tile_kernel<false>, dim3(pblk), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr, sp,
-                               end_bit, c->hist, (const unsigned long long *)nullptr, (double *const *)nullptr);
-        hipLaunchKernelGGL(scan_block_kernel, dim3(nblk), dim3(kScanBlock), 0, c->stream, c->hist, c->offs, c->partial, ncell);
-        hipLaunchKernelGGL(scan_block_kernel, dim3(1), dim3(kScanBlock), 0, c->stream, c->partial, c->partial2,
-                           c->partial + nblk, (size_t)nblk);
-        hipLaunchKernelGGL(scan_add_kernel, dim3(nblk), dim3(kScanBlock), 0, c->stream, c->offs, c->partial2, ncell);
-        HIPCHK(hipGetLastError());
-        const int other = 1 - c->cur[sp];
-        const size_t cap = ((size_t)c->h.cap[sp] + 31) & ~(size_t)31;
-        double *dst[8];
-        for (int f = 0; f < 8; ++f) dst[f] = c->buf[other][sp] + (size_t)f * cap;
-        HIPCHK(hipMemcpyAsync(c->d_dst, dst, sizeof dst, hipMemcpyHostToDevice, c->stream));
-        if (pblk > 0)
-            hipLaunchKernelGGL(sort_tile_kernel<true>, dim3(pblk), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr, sp,
-                               end_bit, c->cursor, c->offs, c->d_dst);
-        hipLaunchKernelGGL(set_count_kernel, dim3(1), dim3(64), 0, c->stream, c->d_ctr, sp, c->offs, c->hist, ncell);
-        if (c->tanc && c->species_set[sp]) {
-            const long long max_tiles = ((long long)c->h.cap[sp] + kTile - 1) / kTile;
-            if (c->h.q_cbase[sp] + max_tiles <= c->q_ntiles) {
-                const unsigned ablk = (unsigned)((max_tiles + kBlock - 1) / kBlock);
-                hipLaunchKernelGGL(anchor_kernel, dim3(ablk), dim3(kBlock), 0, c->stream, c->d_ctr, sp, dst[0], dst[1],
-                                   c->tanc + 2 * c->h.q_cbase[sp], max_tiles, anchor_rows(c->pk_winr));
-            }
-        }
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(c->stream));
-        c->cur[sp] = other;
-        for (int f = 0; f < 8; ++f) c->h.f[sp][f] = dst[f];
-        if ((rc = sync_state(c))) return rc;
-    }
+    if ((rc = sort_enqueue(c, s0, s1))) return rc;
+    sort_mirror(c, s0, s1);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->since_sort = 0;
+    return IPM_OK;
+}
+
+int ipm_set_sort_interval(ipm_ctx *c, int steps)
+{
+    if (!c) return fail(IPM_EINVAL, "null context");
+    if (steps < 0) return fail(IPM_EINVAL, "sort interval must be >= 0, got %d", steps);
+    c->sort_every = steps;
+    c->since_sort = 0;
     return IPM_OK;
 }
 
@@ -2328,10 +2403,7 @@ int ipm_set_launch(ipm_ctx *c, int block, int blocks_per_cu)
         return fail(IPM_EINVAL, "a %d-thread block runs with the %d-row window only", b, kWinR);
     c->pk_block = b;
     c->blocks_per_cu = blocks_per_cu;
-    if (c->graph) {
-        (void)hipGraphExecDestroy(c->graph);
-        c->graph = nullptr;
-    }
+    drop_graphs(c);
     return IPM_OK;
 }
 
@@ -2342,10 +2414,7 @@ int ipm_set_deposit_window(ipm_ctx *c, int rows)
     if (rows != kWinR && c->pk_block != kBlock)
         return fail(IPM_EINVAL, "a %d-row window runs with %d-thread blocks only", rows, kBlock);
     c->pk_winr = rows;
-    if (c->graph) {
-        (void)hipGraphExecDestroy(c->graph);
-        c->graph = nullptr;
-    }
+    drop_graphs(c);
     return IPM_OK;
 }
 
@@ -2355,10 +2424,7 @@ int ipm_set_collide_grid(ipm_ctx *c, int blocks_per_cu)
     if (blocks_per_cu < 1 || blocks_per_cu > kCollideBlocksPerCuMax)
         return fail(IPM_EINVAL, "collision blocks per CU must be 1..%d, got %d", kCollideBlocksPerCuMax, blocks_per_cu);
     c->collide_bpc = blocks_per_cu;
-    if (c->graph) {
-        (void)hipGraphExecDestroy(c->graph);
-        c->graph = nullptr;
-    }
+    drop_graphs(c);
     if (c->num_cu * blocks_per_cu < c->ns)
         return fail(IPM_EINVAL, "collision grid of %d blocks is smaller than the %d species", c->num_cu * blocks_per_cu,
                     c->ns);

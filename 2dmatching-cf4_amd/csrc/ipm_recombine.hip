@@ -289,9 +289,8 @@ int ipm_recombine(ipm_ctx *c, int nreaction, const int *reaction_list, const dou
             HIPCHK(hipcub::DeviceRadixSort::SortPairs(r->temp, tb, r->keys[0], r->keys[1], r->vals[0], r->vals[1],
                                                       (int)n, 0, bits, c->stream));
             const int other = 1 - c->cur[s];
-            const size_t cap = ((size_t)c->h.cap[s] + 31) & ~(size_t)31;
             Ptr8 dst;
-            for (int f = 0; f < 8; ++f) dst.p[f] = c->buf[other][s] + (size_t)f * cap;
+            for (int f = 0; f < 8; ++f) dst.p[f] = c->h.alt[s][f];
             if (nlive > 0) {
                 const unsigned gblk = (unsigned)((nlive + kBlock - 1) / kBlock);
                 hipLaunchKernelGGL(rc_gather_kernel, dim3(gblk), dim3(kBlock), 0, c->stream, c->d_state, s, nlive,
@@ -302,8 +301,7 @@ int ipm_recombine(ipm_ctx *c, int nreaction, const int *reaction_list, const dou
             hipLaunchKernelGGL(rc_set_count_kernel, dim3(1), dim3(64), 0, c->stream, c->d_ctr, s, nlive);
             HIPCHK(hipGetLastError());
             HIPCHK(hipStreamSynchronize(c->stream));
-            c->cur[s] = other;
-            for (int f = 0; f < 8; ++f) c->h.f[s][f] = dst.p[f];
+            set_buffers(c, s, other);
             if ((rc = sync_state(c))) return rc;
         }
         hipLaunchKernelGGL(rc_density_kernel, dim3(cblk), dim3(kBlock), 0, c->stream, c->d_state, s, want_t[s] ? 1 : 0,

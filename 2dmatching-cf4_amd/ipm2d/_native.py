@@ -66,6 +66,7 @@ SIGNATURES = {
     "ipm_set_timing": [_vp, C.c_int],
     "ipm_take_timing": [_vp, _dp, _i64p],
     "ipm_sort": [_vp, C.c_int],
+    "ipm_set_sort_interval": [_vp, C.c_int],
     "ipm_set_launch": [_vp, C.c_int, C.c_int],
     "ipm_set_deposit_window": [_vp, C.c_int],
     "ipm_set_collide_grid": [_vp, C.c_int],

@@ -353,6 +353,11 @@ class HotPath:
     def sort(self, s=-1):
         N.call("ipm_sort", self._h, s)
 
+    def set_sort_interval(self, steps):
+        """Sort every species after each `steps` one_step / one_step_graph
+        calls (0: never); the graph form replays the sort as a graph too."""
+        N.call("ipm_set_sort_interval", self._h, int(steps))
+
     def synchronize(self):
         N.call("ipm_synchronize", self._h)
 

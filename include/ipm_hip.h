@@ -147,8 +147,16 @@ int ipm_step_graph(ipm_ctx *ctx);
  * call; cleared after read. */
 int ipm_set_timing(ipm_ctx *ctx, int on);
 int ipm_take_timing(ipm_ctx *ctx, double *ms, int64_t *launches);
-/* Cell sort (counting sort by cell; drops dead slots). s = -1: all. */
+/* Cell sort (counting sort by cell; drops dead slots). s = -1: all.  No
+ * counterpart in the reference (its bundles are unordered); nothing the
+ * path computes depends on particle order. */
 int ipm_sort(ipm_ctx *ctx, int s);
+/* Sorts every species after each `steps` calls of ipm_step / ipm_step_graph
+ * (0, the default: never).  ipm_step_graph then replays the sort as a
+ * captured graph of its own, so a run of many steps needs no host
+ * synchronization at all.  A scheduling choice: results do not depend on it
+ * (the sort changes storage order only). */
+int ipm_set_sort_interval(ipm_ctx *ctx, int steps);
 
 /* Launch shape of the fused particle pass (ipm_step, ipm_step_graph):
  * block = threads per block of a tile of 2048 slots, 128, 256 or 512 (16, 8

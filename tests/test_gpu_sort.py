@@ -67,3 +67,65 @@ def test_sort_after_drift():
                                       err_msg=f"species {s} after the sort")
     a.close()
     b.close()
+
+
+def test_sort_interval_in_graph_replays():
+    """ipm_set_sort_interval: the sort every 7 steps replayed as a graph of
+    its own inside ipm_step_graph equals explicit eager sorts at the same
+    points (particle multisets and counts exactly, the deposit to 1e-12),
+    keeps cell order, and survives a capacity change (both graphs are
+    recaptured for the new tile count)."""
+    cfg = W.make_config("c1")
+    geom = W.make_geometry(cfg)
+    tables = W.build_tables(cfg)
+    rng = np.random.default_rng(cfg.seed)
+    parts = [W.load_species(cfg, geom, s, rng) for s in range(cfg.ns)]
+    a, _, _, _ = ipm2d.from_config(cfg, geom=geom, tables=tables, particles=parts, capacity_factor=1.5)
+    b, _, _, _ = ipm2d.from_config(cfg, geom=geom, tables=tables, particles=parts, capacity_factor=1.5)
+    a.set_sort_interval(7)
+    for k in range(23):
+        a.one_step_graph()
+        b.one_step()
+        if (k + 1) % 7 == 0:
+            b.sort(-1)
+    a.synchronize()
+    for s in range(cfg.ns):
+        pa, pb = a.download(s), b.download(s)
+        assert a.count(s) == b.count(s) == pa.shape[1]
+        np.testing.assert_array_equal(_hashes(pa), _hashes(pb), err_msg=f"species {s}")
+    ra, _ = a.p2ces_all()
+    rb, _ = b.p2ces_all()
+    np.testing.assert_allclose(ra, rb, rtol=1e-12, atol=1e-12 * np.abs(rb).max())
+    # one more replay reaches the fourth sort (step 28): cell order holds
+    for _ in range(5):
+        a.one_step_graph()
+    a.synchronize()
+    for s in range(cfg.ns):
+        P = a.download(s)
+        cell = (np.ceil(P[0]) - 1) + (np.maximum(np.ceil(P[1]), 1) - 1) * (cfg.nz - 1)
+        assert np.all(np.diff(cell) >= 0), f"species {s} not in cell order after the graph sort"
+    # a larger capacity for species 1 drops both graphs; the recaptured ones
+    # cover the new tile count
+    P1 = a.download(1)
+    q, m = W.species_params(cfg, 1)
+    a.set_species(1, q, m, int(P1.shape[1] * 2.5) + 4096)
+    a.upload(1, P1)
+    a.set_sort_interval(3)
+    for _ in range(6):
+        a.one_step_graph()
+    a.synchronize()
+    P = a.download(1)
+    assert P.shape[1] == a.count(1) > 0.9 * P1.shape[1]
+    cell = (np.ceil(P[0]) - 1) + (np.maximum(np.ceil(P[1]), 1) - 1) * (cfg.nz - 1)
+    assert np.all(np.diff(cell) >= 0), "species 1 not in cell order after the recaptured sort"
+    # the eager form sorts on the same schedule
+    b.set_sort_interval(7)
+    for _ in range(7):
+        b.one_step()
+    b.synchronize()
+    for s in range(cfg.ns):
+        P = b.download(s)
+        cell = (np.ceil(P[0]) - 1) + (np.maximum(np.ceil(P[1]), 1) - 1) * (cfg.nz - 1)
+        assert np.all(np.diff(cell) >= 0), f"species {s} not in cell order after the eager interval sort"
+    a.close()
+    b.close()
