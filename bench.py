@@ -128,6 +128,23 @@ def sort_after_full_interval(hp, sort_interval):
     return time.perf_counter() - t
 
 
+def interval_in_graph(hp, sort_interval):
+    """Wall time per step of one whole sort interval replayed with the sort
+    inside the graph (ipm_set_sort_interval): sort_interval graph steps from
+    a sorted state, the last of which ends with the sort.  A cross-check of
+    the amortised step: no host synchronization inside."""
+    hp.sort(-1)
+    hp.set_sort_interval(sort_interval)
+    hp.synchronize()
+    t = time.perf_counter()
+    for _ in range(sort_interval):
+        hp.one_step_graph()
+    hp.synchronize()
+    el = time.perf_counter() - t
+    hp.set_sort_interval(0)
+    return el / sort_interval
+
+
 def host_cpus():
     """Host CPU description: logical CPUs, the ones this process may use, the
     cgroup CPU quota (if any) and the model name."""
@@ -397,12 +414,14 @@ def main():
     if device_load:
         del particles
         t = time.perf_counter()
-        hp, _, _, counts = ipm2d.from_config_device(cfg, geom=geom, tables=tables, capacity_factor=1.3)
+        # capacity for the population's growth over the tuner, warmup and
+        # timed steps (ionization outpaces losses in the synthetic field)
+        hp, _, _, counts = ipm2d.from_config_device(cfg, geom=geom, tables=tables, capacity_factor=1.5)
         ntot = sum(counts)
         log(f"[bench] device load of {ntot} particles in {time.perf_counter() - t:.1f}s")
     else:
         ntot = sum(P.shape[1] for P in particles)
-        hp, _, _, _ = ipm2d.from_config(cfg, geom=geom, tables=tables, particles=particles, capacity_factor=1.25)
+        hp, _, _, _ = ipm2d.from_config(cfg, geom=geom, tables=tables, particles=particles, capacity_factor=2.0)
         del particles
     tuned = None
     if not args.no_autotune and not args.traffic_probe and args.sort_interval < 0 and args.blocks_per_cu < 0:
@@ -458,6 +477,10 @@ def main():
     eager_value = psteps / args.steps / ((el_e - ts_e) / args.steps + amort)
     log(f"[bench] sort after {sort_interval} steps of drift {t_sort * 1e3:.3f} ms; step {step_s * 1e3:.4f} ms "
         f"-> {value:.4g} particle-steps/s (eager {eager_value:.4g})")
+    t_int = interval_in_graph(hp, sort_interval) if (sort_interval and not args.traffic_probe and not args.eager) \
+        else None
+    if t_int:
+        log(f"[bench] one sort interval in graph replays: {t_int * 1e3:.4f} ms per step")
 
     pk_ms, pk_n = tm["particle"]
     avg_s = pk_ms / pk_n / 1e3
@@ -513,6 +536,7 @@ def main():
                         f"({'/'.join(sp.name for sp in cfg.species)}), CF4 {cfg.pressure_mtorr} mTorr, "
                         f"dt {cfg.dt:g} s, cell sort every {sort_interval} steps",
             "particles": ntot,
+            "live_particles_timed_region": [n0, n1],
             "species": cfg.ns,
             "grid": [cfg.nz, cfg.nr],
             "sort_interval": sort_interval,
@@ -548,7 +572,11 @@ def main():
                  "sorts_in_timed_region": ((phase + args.steps) // sort_interval) if sort_interval else 0,
                  "timed_region_starts_steps_after_sort": phase,
                  "timed_region_ms_per_step": el / args.steps * 1e3,
-                 "timed_region_sort_ms": ts_g * 1e3},
+                 "timed_region_sort_ms": ts_g * 1e3,
+                 "interval_in_graph_ms_per_step": t_int * 1e3 if t_int else None,
+                 "interval_in_graph_what": "one whole interval of graph steps from a sorted state, the sort "
+                                           "replayed as a graph after the last (ipm_set_sort_interval): a "
+                                           "cross-check of ms_per_step"},
         "kernel_ms_per_step": {k: (v[0] / v[1] if v[1] else None) for k, v in tm.items()},
         "cpu_baseline": cpu,
         "adjoint_over_forward": adj["electrode_flux"]["adjoint_over_forward"] if adj else None,
