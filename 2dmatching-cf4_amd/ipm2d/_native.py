@@ -20,6 +20,7 @@ LAYOUT_SOA = 1
 NS_MAX = 9
 NG_MAX = 3
 NR_MAX = 128
+CAPACITY_MAX = 536870880        # IPM_CAPACITY_MAX: 32-bit byte offsets per field
 
 
 class IPMError(RuntimeError):

@@ -396,8 +396,8 @@ def from_config_device(cfg, geom=None, tables=None, capacity_factor=1.05, seed=N
     """from_config with the particles made on the GPU (workload.load_species_device,
     species s seeded by seed + s) and copied in device to device: for
     workloads too large to build on the host.  Capacities are the expected
-    counts times capacity_factor plus a margin.  Returns (hp, geom, tables,
-    counts)."""
+    counts times capacity_factor plus a margin, at most IPM_CAPACITY_MAX.
+    Returns (hp, geom, tables, counts)."""
     from . import workload as W
     geom = geom if geom is not None else W.make_geometry(cfg)
     tables = tables if tables is not None else W.build_tables(cfg)
@@ -406,7 +406,7 @@ def from_config_device(cfg, geom=None, tables=None, capacity_factor=1.05, seed=N
     for s in range(cfg.ns):
         q, m = W.species_params(cfg, s)
         mean = W.expected_count(cfg, geom, s)
-        hp.set_species(s, q, m, int(mean * capacity_factor + 8.0 * np.sqrt(mean + 1.0)) + 4096)
+        hp.set_species(s, q, m, min(int(mean * capacity_factor + 8.0 * np.sqrt(mean + 1.0)) + 4096, N.CAPACITY_MAX))
     hp.set_geometry(geom)
     for s in range(cfg.ns):
         hp.set_mcc(s, 0, tables.get((s, 0)))
