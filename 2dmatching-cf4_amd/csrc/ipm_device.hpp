@@ -29,9 +29,14 @@ constexpr int kEdgeD2V = 3;
 constexpr double kDead = -1.0e300;
 
 struct DevTable {
-    int model, nreaction, nsigma, pad;
+    int model, nreaction, nsigma;
+    int cumulative;         // every column of prob nondecreasing in j and >= 0 (checked on the host)
     double emin, einterval, emax, ratio;
     double gas_mass, gas_temperature, beta_max, pad2;
+    // per-table constants of Updater and VelMaxInit, formed on the host by
+    // the same expressions: miu = sm*gm/(sm+gm), mass_ratio = gm/(sm+gm),
+    // beta = 1/(kB*T)
+    double miu, mass_ratio, beta, pad3;
     const int *rtype;
     const int *resultant;
     const double *threshold;
@@ -406,17 +411,18 @@ __device__ __forceinline__ void vec_ran_xyz(Part &p, double V, Rng &rg)
 }
 
 // VelMaxInit, R/src/pic/particle/ParticleOne.f90:76-95
-__device__ __forceinline__ void vel_max_init(Part &p, double Mass, double Temperature, Rng &rg)
+// (Beta = 1/(kB*Temperature) comes from the caller's table.)
+__device__ __forceinline__ void vel_max_init(Part &p, double Mass, double Beta, Rng &rg)
 {
-    double Beta = 1.0 / (kKB * Temperature);
-    double FuncA = 1.0, FuncB = 0.0, R = 0.5;
+    double FuncA = 1.0, FuncB = 0.0, R = 0.5, lr = 0.0;
     while (FuncA > FuncB) {
         R = rg.next();
         FuncA = R * R;
         R = rg.next();
-        FuncB = -kEXP * R * log(R);
+        lr = log(R);
+        FuncB = -kEXP * R * lr;
     }
-    double V = sqrt(-3.0 * log(R) / Beta / Mass);
+    double V = sqrt(-3.0 * lr / Beta / Mass);
     vec_ran_xyz(p, V, rg);
 }
 
@@ -425,9 +431,9 @@ __device__ __forceinline__ void mc_update(Mc &m, double sm, const DevTable &t, R
 {
     const double gm = t.gas_mass;
     m.pot = m.poi;
-    m.miu = sm * gm / (sm + gm);
-    m.mass_ratio = gm / (sm + gm);
-    vel_max_init(m.pot, gm, t.gas_temperature, rg);
+    m.miu = t.miu;                 // sm * gm / (sm + gm)
+    m.mass_ratio = t.mass_ratio;   // gm / (sm + gm)
+    vel_max_init(m.pot, gm, t.beta, rg);
     m.gx = m.poi.Vr - m.pot.Vr;
     m.gy = m.poi.Vt - m.pot.Vt;
     m.gz = m.poi.Vz - m.pot.Vz;
@@ -508,14 +514,21 @@ __device__ __forceinline__ int select_probability(const Mc &m, const DevTable &t
     } else {
         mode = 2; a = P + (size_t)(Ns - 1) * Nr;
     }
-    for (int j = 0; j < Nr; ++j) {
+    auto value = [&](int j) {
         double v;
         if (mode == 0) v = a[j];
         else if (mode == 1) v = a[j] * S1 + b[j] * S2;
         else v = a[j] * E / t.emax;
         if (E < t.threshold[j]) v = 0.0;
-        if (R < v) return j + 1;
-    }
+        return v;
+    };
+    // A cumulative table (columns nondecreasing in j, as ProbilityNonReactive
+    // builds them) interpolates to values nondecreasing in j (the weights
+    // are nonnegative and rounding is monotone), so R >= the last value
+    // misses every reaction: the null event is known from one lookup.
+    if (t.cumulative && !(E < t.threshold[Nr - 1]) && !(R < value(Nr - 1))) return 0;
+    for (int j = 0; j < Nr; ++j)
+        if (R < value(j)) return j + 1;
     return 0;
 }
 

@@ -1480,6 +1480,23 @@ static int scratch_address(int dev, double **out)
     return IPM_OK;
 }
 
+// Per-table constants of Updater and VelMaxInit (R/src/mc/TypeMCC.f90:60-92,
+// R/src/pic/particle/ParticleOne.f90:76-95) that depend only on the species
+// and gas masses and the gas temperature, formed by the same expressions the
+// device would evaluate per collision.
+static void table_consts(ipm_ctx *c)
+{
+    for (int s = 0; s < c->ns; ++s)
+        for (int g = 0; g < c->ng; ++g) {
+            DevTable &t = c->h.tab[s * c->ng + g];
+            if (t.model == 0) continue;
+            const double sm = c->h.mass[s], gm = t.gas_mass;
+            t.miu = sm * gm / (sm + gm);
+            t.mass_ratio = gm / (sm + gm);
+            t.beta = 1.0 / (kKB * t.gas_temperature);
+        }
+}
+
 // Collision-queue regions: species s gets ceil(cap_s/1024) tile regions of
 // R_s = min(1024, ceil(1024 * min(1, 2 * sum_g CollisionRatio)) + 64)
 // entries, at least 4 standard deviations above the binomial mean of a
@@ -1799,6 +1816,7 @@ int ipm_set_species(ipm_ctx *c, int s, double charge, double mass, int64_t capac
     c->chifac[s] = 0.5 * charge / mass * c->dt * c->dt / kEPS0;
     HIPCHK(hipMemcpy(c->d_chifac, c->chifac, sizeof(double) * IPM_NS_MAX, hipMemcpyHostToDevice));
     c->species_set[s] = true;
+    table_consts(c);
     DevCounters ctr;
     HIPCHK(hipMemcpy(&ctr, c->d_ctr, sizeof ctr, hipMemcpyDeviceToHost));
     ctr.n[s] = 0; ctr.n_new[s] = 0; ctr.n_dead[s] = 0;
@@ -2213,7 +2231,15 @@ int ipm_set_mcc(ipm_ctx *c, int s, int g, int model, int nreaction, int nsigma, 
         t.emin = emin; t.einterval = einterval; t.emax = emax; t.ratio = collision_ratio;
         t.gas_mass = gas_mass; t.gas_temperature = gas_temperature; t.beta_max = beta_max;
         t.rtype = d_rt; t.resultant = d_res; t.threshold = d_th; t.prob = d_p;
+        bool cum = true;
+        for (int i = 0; i < nsigma && cum; ++i)
+            for (int j = 0; j < nreaction; ++j) {
+                const double v = prob[(size_t)i * nreaction + j];
+                if (!(v >= 0.0) || (j > 0 && v < prob[(size_t)i * nreaction + j - 1])) { cum = false; break; }
+            }
+        t.cumulative = cum ? 1 : 0;
     }
+    table_consts(c);
     int rc = ensure_queue(c);
     if (rc) return rc;
     return sync_state(c);

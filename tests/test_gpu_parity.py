@@ -6,6 +6,8 @@ tallies; MCC with the same counter-based random stream reproduces the
 oracle's events, so velocities agree to 1e-10 of |v| (device and host libm
 transcendentals may differ in the last ulp) and products agree as sets.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -140,8 +142,26 @@ def _energetic(cfg, s, n, rng, ev):
     return P
 
 
-def test_mcc_matches_oracle(oracle, rng):
+def _non_cumulative(tables):
+    """The same tables with the first reaction's column raised above the
+    others: Probility(1, i) > Probility(j, i), so the columns are no longer
+    nondecreasing in j (the null test from the last value alone would be
+    wrong; the device must walk every reaction)."""
+    out = {}
+    for k, b in tables.items():
+        if b is not None and b.model != 0 and b.nreaction >= 2:
+            pr = np.array(b.prob, copy=True)
+            pr[:, 0] = np.minimum(1.0, pr[:, -1] * 1.1)
+            b = dataclasses.replace(b, prob=pr)
+        out[k] = b
+    return out
+
+
+@pytest.mark.parametrize("cumulative", [True, False])
+def test_mcc_matches_oracle(oracle, rng, cumulative):
     cfg, tables, species = _six_species_setup()
+    if not cumulative:
+        tables = _non_cumulative(tables)
     geom = W.make_geometry(cfg)
     hp = HotPath(cfg.nz, cfg.nr, cfg.dx, cfg.dt, cfg.ns, 1)
     for s, (q, m) in enumerate(species):
