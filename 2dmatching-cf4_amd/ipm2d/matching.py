@@ -11,7 +11,10 @@ the electrode, Iconv[n] (charge collected in step n over dt, as the
 reference feeds its circuit, R/src/pic/circuit/Circuits.F90:166-170), and
 every absorption event are taken from a base run by the plain method and
 then held fixed: collisions and absorption are frozen events of the window,
-and the space-charge field is frozen in E0.
+and the space-charge field is frozen in E0.  The electrode capacitance is
+the vacuum-plate value of E1's geometry with q0 = 0, and nothing is
+differentiated through a Poisson solve: the gradients returned here are
+those of this reduced model, not of the self-consistent discharge.
 
 ``plain_coupled_run`` is that base run: per step the field from the present
 circuit voltage, PB%MoveES and MaterialAborptionParticle for every species

@@ -323,6 +323,9 @@ def adjoint_leg(reps=5):
     hp.close()
     out["workload"] = (f"c4: {cfg.nz}x{cfg.nr} nodes, {npart} particles (e/CF3+/F-), {cfg.steps} steps, "
                        f"checkpoint every 20, IMN Rs 50 Ohm Cm1 500 pF Cm2 200 pF, 200 V 13.56 MHz phase 30 deg")
+    out["model"] = ("reduced window model: the plasma part of the field and the absorption events are frozen "
+                    "over the window, the electrode capacitance is the vacuum-plate value with q0 = 0, and there "
+                    "is no adjoint through the Poisson solve; the gradients are those of this model")
     return out
 
 
