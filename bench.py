@@ -424,7 +424,12 @@ def main():
         log(f"[bench] device load of {ntot} particles in {time.perf_counter() - t:.1f}s")
     else:
         ntot = sum(P.shape[1] for P in particles)
-        hp, _, _, _ = ipm2d.from_config(cfg, geom=geom, tables=tables, particles=particles, capacity_factor=2.0)
+        # capacity for the populations' growth over the whole run (F- grows
+        # about 0.03 % per step in the synthetic field: attachment, no loss):
+        # the tuner (~600 steps), warmup, the eager and the graph regions
+        steps_total = 600 + args.warmup + 2 * args.steps
+        hp, _, _, _ = ipm2d.from_config(cfg, geom=geom, tables=tables, particles=particles,
+                                        capacity_factor=max(2.0, 1.0 + 0.0008 * steps_total))
         del particles
     tuned = None
     if not args.no_autotune and not args.traffic_probe and args.sort_interval < 0 and args.blocks_per_cu < 0:
@@ -507,6 +512,9 @@ def main():
         extra["plain_per_species_value"] = ps_p / el_p
         extra["batched_over_plain"] = value / extra["plain_per_species_value"]
         log(f"[bench] plain per-species launches {extra['plain_per_species_value']:.4g}")
+    hp_counts = [hp.count(s) for s in range(cfg.ns)]
+    capacities = list(hp.capacity)
+    log(f"[bench] live particles per species {hp_counts} of capacities {capacities}")
     hp.close()
 
     ceiling = None
@@ -540,6 +548,8 @@ def main():
                         f"dt {cfg.dt:g} s, cell sort every {sort_interval} steps",
             "particles": ntot,
             "live_particles_timed_region": [n0, n1],
+            "live_particles_per_species_after": [hp_counts[s] for s in range(cfg.ns)],
+            "capacity_per_species": capacities,
             "species": cfg.ns,
             "grid": [cfg.nz, cfg.nr],
             "sort_interval": sort_interval,
