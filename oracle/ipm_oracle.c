@@ -10,8 +10,9 @@
  * Pinning (what checks this restatement, beyond retyping the reference):
  *   ora_philox4x32_10     Random123 known-answer vectors
  *   ora_push              closed forms (E=0 straight Cartesian line, uniform
- *                         Ez quadratic z), invariants (R*Vt, Vr^2+Vt^2),
- *                         axis branch, brute-force Cartesian re-derivation
+ *                         Ez quadratic z, linear fields), invariants (R*Vt,
+ *                         Vr^2+Vt^2), axis branch, random-field gathers
+ *                         bounded by the bracketing faces' values
  *   ora_absorb            rule table on hand-built particles, exact
  *   ora_deposit_*         charge conservation, on-node known answers,
  *                         uniform density vs analytic node volumes

@@ -5,6 +5,7 @@ this module, and only as the checker or the timed CPU baseline; the product
 (2dmatching-cf4_amd/) never does.  Grids use numpy (nz, nr) arrays indexed
 [iz-1, ir-1]; particles are (8, n) float64 SoA arrays Z R Vz Vr Vt Az Ar WQ.
 """
+import contextlib
 import ctypes as C
 import os
 
@@ -40,6 +41,7 @@ class OraSetup(C.Structure):
 
 _libs = {}
 _timed_path = None
+_override_path = None
 
 
 def use_timed_library(path):
@@ -49,8 +51,21 @@ def use_timed_library(path):
     _timed_path = path
 
 
+@contextlib.contextmanager
+def library_override(path):
+    """Within the block every oracle function runs against another build of
+    the source (the mutant tests compile copies with a planted mistake)."""
+    global _override_path
+    old = _override_path
+    _override_path = path
+    try:
+        yield
+    finally:
+        _override_path = old
+
+
 def lib(fast=False):
-    path = (_timed_path or LIB_FAST) if fast else LIB
+    path = _override_path or ((_timed_path or LIB_FAST) if fast else LIB)
     if path in _libs:
         return _libs[path]
     if not os.path.exists(path):

@@ -5,6 +5,11 @@ mathematics fixes (not against a retyped formula):
 * push, linear fields: bilinear gathers on the staggered Ez/Er grids are exact
   for linear fields, so the implicit update has a closed form at each
   particle -- a wrong stagger, index or weight fails it
+* push, random fields: a particle at rest gathers a value between the
+  field values at the two face rows and the two node columns that bracket
+  it (Ez at z = i - 1/2, r = j - 1; Er at z = i - 1, r = j - 1/2), which a
+  linear field cannot show: extrapolating from the wrong faces is exact for
+  it
 * push, uniform Ez: Z' = Z + Vz + 5/2 E, Vz' = Vz + 3/2 E (Az = 0)
 * push, axis branch: R' < MinReal gives Vt = MinReal, Vr = |(Vx, Vy)|
 * deposit: total weight, first moments in z and r (bilinear weights keep
@@ -51,6 +56,36 @@ def _linear_fields(a, b, c, a2, b2, c2):
     jr2 = np.arange(0, NR + 1)[None, :] - 0.5
     er = a2 + b2 * iz2 + c2 * jr2
     return geo.fortran(ez), geo.fortran(er)
+
+
+def test_push_gather_stays_between_bracketing_faces(oracle, rng):
+    # charge/mass chosen so that EFactor = 1: a particle at rest with
+    # Az = Ar = Vt = 0 leaves with Vz = 3/2 Ez and Vr = 3/2 Er (gathered)
+    dx, dt = 1.0, 1.0
+    q, m = 2.0, 1.0
+    # |E| <= 0.3 keeps the rotated radius X = R + 5/2 Er positive (R >= 2),
+    # so the rotation leaves Vr = 3/2 Er
+    ez = rng.uniform(-0.3, 0.3, (NZ + 1) * NR)
+    er = rng.uniform(-0.3, 0.3, NZ * (NR + 1))
+    n = 20000
+    P = np.zeros((8, n))
+    P[0] = rng.uniform(2.0, NZ - 3.0, n)
+    P[1] = rng.uniform(2.0, NR - 3.0, n)
+    P[7] = 1.0
+    Q = oracle.push(NZ, NR, dx, dt, q, m, ez, er, P)
+    gz, gr = Q[2] / 1.5, Q[3] / 1.5
+    EZ = ez.reshape(NR, NZ + 1).T            # Ez(i, j) at [i, j - 1]
+    ER = er.reshape(NR + 1, NZ).T            # Er(i, j) at [i - 1, j]
+    z, r = P[0], P[1]
+    iz = np.floor(z + 0.5).astype(int)       # faces i (z = i - 1/2) and i + 1 bracket z
+    jz = np.ceil(r).astype(int)              # nodes j (r = j - 1) and j + 1 bracket r
+    cz = np.stack([EZ[iz, jz - 1], EZ[iz + 1, jz - 1], EZ[iz, jz], EZ[iz + 1, jz]])
+    ir = np.ceil(z).astype(int)              # nodes i (z = i - 1) and i + 1 bracket z
+    jr = np.floor(r + 0.5).astype(int)       # faces j (r = j - 1/2) and j + 1 bracket r
+    cr = np.stack([ER[ir - 1, jr], ER[ir, jr], ER[ir - 1, jr + 1], ER[ir, jr + 1]])
+    tol = 1e-13
+    assert np.all(gz >= cz.min(axis=0) - tol) and np.all(gz <= cz.max(axis=0) + tol)
+    assert np.all(gr >= cr.min(axis=0) - tol) and np.all(gr <= cr.max(axis=0) + tol)
 
 
 def test_push_linear_field_closed_form(oracle, rng):
