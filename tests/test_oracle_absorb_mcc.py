@@ -112,6 +112,22 @@ def test_maxwellian_mean_energy(oracle):
     assert abs(e.mean() - 1.5) < 6 * se
 
 
+def test_maxwellian_acceptance_rate(oracle):
+    """VelMaxInit's rejection loop (ParticleOne.f90:76-95) accepts a pair
+    when R1^2 <= -e R2 ln R2.  Its acceptance probability is
+    sqrt(e) * int_0^1 sqrt(-r ln r) dr = sqrt(e) Gamma(3/2) / (3/2)^(3/2)
+    = 0.7953; each try takes two draws and the direction two more, so the
+    mean draw count per sample is 2/p + 2.  (The accepted speeds do not
+    depend on the e factor -- it rescales the acceptance uniformly -- so the
+    mean-energy test cannot see it; the draw count can.)"""
+    p = math.sqrt(math.e) * math.gamma(1.5) / 1.5 ** 1.5
+    n = 30000
+    tries = np.array([(oracle.vel_maxwell(69.0 * AMU, 300.0, 11, i, 0, 0, 9)[3] - 2) / 2 for i in range(n)])
+    assert np.all(tries >= 1) and np.all(tries == np.round(tries))
+    se = math.sqrt((1.0 - p) / p ** 2 / n)
+    assert abs(tries.mean() - 1.0 / p) < 6 * se, (tries.mean(), 1.0 / p)
+
+
 @pytest.mark.parametrize("energy", [0.5, 10.0, 200.0])
 def test_vahedi_cos_chi_cdf(oracle, energy):
     c = np.array([oracle.lib().ora_cos_chi(1, energy, 3, i, 0) for i in range(20000)])

@@ -13,10 +13,15 @@ import subprocess
 import numpy as np
 import pytest
 
+import test_oracle_absorb_mcc as AM
 import test_oracle_mcc_channels as MC
+import test_oracle_moments_circuit as MO
 import test_oracle_push_deposit as PD
+import test_oracle_recombination as RC
 import test_oracle_set_e as SE
 import test_oracle_solver as SO
+import test_oracle_split_merge as SM
+import test_xsec as XS
 from oracle import ipm_oracle
 
 SRC = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "ipm_oracle.c")
@@ -60,6 +65,38 @@ MUTANTS = [
     ("sigma interpolation: dble(i - 1) for the reference's dble(i)",
      "double S1 = ((double)i - LE) / (UE - LE);", "double S1 = ((double)(i - 1) - LE) / (UE - LE);",
      lambda O: MC.test_sigma_interpolation_uses_dble_i_weight(O, "oracle")),
+    ("node volumes: axis shell with 1/4 in place of 1/3",
+     "v = f * 1.0 / 3.0 * C_PI * dx * dx * dx;", "v = f * 1.0 / 4.0 * C_PI * dx * dx * dx;",
+     lambda O: PD.test_node_volumes_tile_the_cylinder(O)),
+    ("absorption: the axis reflection dropped",
+     "R[i] = fabs(R[i]);", "R[i] = R[i];",
+     lambda O: AM.test_absorption_rule_table(O)),
+    ("Maxwellian sampler: the e factor of the acceptance test dropped",
+     "FuncB = -C_EXP * R * log(R);", "FuncB = -R * log(R);",
+     lambda O: AM.test_maxwellian_acceptance_rate(O)),
+    ("Vahedi angle: (1 + E)^R taken as E^R",
+     "return (E + 2.0 - 2.0 * pow(E + 1.0, R)) / E;", "return (E + 2.0 - 2.0 * pow(E, R)) / E;",
+     lambda O: AM.test_vahedi_cos_chi_cdf(O, 10.0)),
+    ("excitation: the threshold loss not applied to the velocity",
+     "case 12: {\n        double VF = sqrt(1.0 - th / m->energy);", "case 12: {\n        double VF = 1.0;",
+     lambda O: AM.test_excitation_removes_threshold(O, _rng())),
+    ("moments: the 1/2 of the kinetic energy dropped",
+     "const double EnergyFactor = 0.5 * mass * VFactor * VFactor;",
+     "const double EnergyFactor = mass * VFactor * VFactor;",
+     lambda O: MO.test_moments_uniform_velocity_and_speed(O, _rng())),
+    ("hard sphere: elastic share 0.25 in place of 0.5",
+     "prob[(size_t)(i - 1) * nreaction] = (nreaction == 2) ? 0.5 : 1.0;",
+     "prob[(size_t)(i - 1) * nreaction] = (nreaction == 2) ? 0.25 : 1.0;",
+     lambda O: XS.test_hard_sphere_closed_form(O, 1)),
+    ("merge: kept when the draw is above WQ/ref",
+     "if (u < WQ[i] / ref) {", "if (u > WQ[i] / ref) {",
+     lambda O: SM.test_merge_keep_fraction_and_expected_weight(O)),
+    ("merge: threshold made inclusive",
+     "if (WQ[i] < merge_limit * ref) {", "if (WQ[i] <= merge_limit * ref) {",
+     lambda O: SM.test_merge_threshold_is_strict(O)),
+    ("recombination: the drawn deletion ignores the particle's weight",
+     "if (u < recb[c] / P[7][i]) {", "if (u < recb[c]) {",
+     lambda O: RC.test_visiting_order_outcomes(O)),
     ("Poisson system: CoeC with the left cells' permittivity",
      "AT(coeC, i, j) = (1.0 + 0.5 * (CHI(i, j) + CHI(i + 1, j))) * 0.5 * (EPSR(i, j - 1) + EPSR(i, j));",
      "AT(coeC, i, j) = (1.0 + 0.5 * (CHI(i, j) + CHI(i + 1, j))) * 0.5 * (EPSR(i - 1, j - 1) + EPSR(i, j));",
