@@ -91,6 +91,11 @@ struct DevState {
     int c_grid;
     int c_pad;
     DevTable tab[IPM_NS_MAX * IPM_NG_MAX];
+    // Split/merge inside the fused step (ipm_set_step_rezone): per species
+    // the mode (1 split, 2 merge, 0 none) and CalRW's Weight, alpha, grid_r
+    // and the merge limit.
+    int rz_mode[IPM_NS_MAX];
+    double rz_weight[IPM_NS_MAX], rz_alpha[IPM_NS_MAX], rz_gridr[IPM_NS_MAX], rz_limit[IPM_NS_MAX];
 };
 
 // Mutable per-step counters (device memory).

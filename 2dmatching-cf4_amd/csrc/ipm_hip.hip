@@ -33,7 +33,11 @@ constexpr int kWinR = 24;            // default LDS deposit window height (nodes
 constexpr int kAnchorZ = 4;
 constexpr int anchor_rows(int winr) { return winr / 2 - 1; }
 
-enum : int { M_PUSH = 1, M_ABSORB = 2, M_DEPOSIT = 4, M_MCC = 8 };
+enum : int { M_PUSH = 1, M_ABSORB = 2, M_DEPOSIT = 4, M_MCC = 8, M_REZONE = 16 };
+
+// Random-stream tag of the merge draw: apart from every (species, gas) tag
+// of the collision pass, which stay below 2^16.
+constexpr uint32_t kMergeTag = 0x4D520000u;
 
 // A collision-queue record: Z R Vz Vr Vt WQ, the particle's index in its
 // tile and a pad, 64 contiguous bytes moved as four 16-byte accesses.  It
@@ -491,6 +495,63 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
                 }
             }
 
+            // Split/merge of the species' particles (particle_split_merge_control,
+            // R/src/base/Rezoning.F90:17-92) where OneStep runs it: after the
+            // deposit (which used the weight before the split) and before the
+            // collision pass (which sees the parent with its reduced weight and
+            // every copy).  The same rules as split_merge_kernel; each copy is
+            // written here and drawn for selection on its own stream.
+            unsigned ncopy = 0;            // copies of this lane's particle (split)
+            unsigned long long cbase = 0;  // their first slot
+            double cref = 0.0;             // their weight
+            bool wq_changed = false;
+            if constexpr ((MODE & M_REZONE) != 0) {
+                const int rzm = st.rz_mode[s];
+                if (alive && rzm) {
+                    const double ref = st.rz_weight[s] * ((1.0 - st.rz_alpha[s]) * p.R / st.rz_gridr[s] + st.rz_alpha[s]);
+                    if (ref > 0.0) {
+                        double wq = p.WQ;
+                        if ((rzm & 1) && wq / ref >= 2.0) {
+                            const double ratio = wq / ref;
+                            const long long capS = st.cap[s];
+                            if (!(ratio - 1.0 <= (double)(capS - n_s))) {
+                                atomicOr(&ctr->overflow, 1);
+                            } else {
+                                const unsigned long long mcp = (unsigned long long)(long long)ratio - 1ull;
+                                const unsigned long long k0 = atomicAdd(&ctr->n_new[s], mcp);
+                                const unsigned long long b0 = (unsigned long long)n_s + k0;
+                                if (b0 + mcp > (unsigned long long)capS) {
+                                    atomicOr(&ctr->overflow, 1);
+                                    unsigned long long nd = 0;
+                                    for (unsigned long long q = b0; q < (unsigned long long)capS && q < b0 + mcp; ++q, ++nd)
+                                        gp(F[0])[q] = kDead;
+                                    if (nd) atomicAdd(&ctr->n_dead[s], nd);
+                                } else {
+                                    for (unsigned long long j = 1; j <= mcp; ++j) wq = wq - ref;
+                                    ncopy = (unsigned)mcp;
+                                    cbase = b0;
+                                    cref = ref;
+                                }
+                            }
+                        }
+                        if ((rzm & 2) && wq < st.rz_limit[s] * ref) {
+                            const StateKey mk = state_key(p.Z, p.R, p.Vz, p.Vr, p.Vt);
+                            Rng g = make_rng(mk, seed, step, kMergeTag | (uint32_t)s);
+                            if (g.next() < wq / ref) {
+                                wq = ref;
+                            } else {
+                                deleted = true;
+                                selected = false;
+                            }
+                        }
+                        if (__double_as_longlong(wq) != __double_as_longlong(p.WQ)) {
+                            p.WQ = wq;
+                            wq_changed = true;
+                        }
+                    }
+                }
+            }
+
             if (MODE & M_MCC) {
                 // stage the wave's selections as queue records in LDS
                 const unsigned long long m = __ballot(selected);
@@ -509,8 +570,63 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
                     cw += nsel;
                 }
             }
+            if constexpr ((MODE & M_REZONE) != 0) {
+                // Copy j (1-based) of a split parent: the parent's state with
+                // the low bits of Vt's pattern XOR j and weight ref, written at
+                // its reserved slot; drawn for selection like any particle, its
+                // queue record names the slot relative to this tile.
+                unsigned jmax = ncopy;
+                for (int o = 32; o > 0; o >>= 1) jmax = max(jmax, (unsigned)__shfl_xor((int)jmax, o));
+                const unsigned long long vtb = (unsigned long long)__double_as_longlong(p.Vt);
+                for (unsigned j = 1; j <= jmax; ++j) {
+                    const bool mine = j <= ncopy;
+                    bool csel = false;
+                    Part cp = p;
+                    unsigned long long q = 0;
+                    if (mine) {
+                        q = cbase + j - 1;
+                        cp.Vt = __longlong_as_double((long long)(vtb ^ (unsigned long long)j));
+                        cp.WQ = cref;
+                        if (MODE & M_MCC) {
+                            const StateKey ck = state_key(cp.Z, cp.R, cp.Vz, cp.Vr, cp.Vt);
+                            for (int g = 0; g < st.ng; ++g) {
+                                const DevTable &tb = st.tab[s * st.ng + g];
+                                if (tb.model == 0) continue;
+                                Rng rg = make_rng(ck, seed, step, (uint32_t)s | ((uint32_t)g << 8));
+                                if (rg.next() < tb.ratio) csel = true;
+                            }
+                        }
+                        // a selected copy's stored velocity carries the VelRes
+                        // round trip, as a selected particle's does (one gas)
+                        double vz = cp.Vz, vr = cp.Vr, vt = cp.Vt;
+                        if ((MODE & M_MCC) && csel && st.ng == 1) {
+                            vz = vz * st.vfactor * st.inv_vfactor;
+                            vr = vr * st.vfactor * st.inv_vfactor;
+                            vt = vt * st.vfactor * st.inv_vfactor;
+                        }
+                        F[0][q] = cp.Z; F[1][q] = cp.R; F[2][q] = vz; F[3][q] = vr;
+                        F[4][q] = vt; F[5][q] = cp.Az; F[6][q] = cp.Ar; F[7][q] = cp.WQ;
+                    }
+                    if (MODE & M_MCC) {
+                        const unsigned long long m = __ballot(csel);
+                        const unsigned nsel = (unsigned)__popcll(m);
+                        if (nsel) {
+                            if (cw + nsel > 64u) flush_wave();
+                            if (csel) {
+                                const unsigned r = cw + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+                                double *d = q_wst + wv * 64 * kRecD + r * kRecD;
+                                d[0] = cp.Z; d[1] = cp.R; d[2] = cp.Vz; d[3] = cp.Vr;
+                                d[4] = cp.Vt; d[5] = cp.WQ;
+                                d[6] = __longlong_as_double((long long)q - base);
+                                d[7] = 0.0;
+                            }
+                            cw += nsel;
+                        }
+                    }
+                }
+            }
 
-            if (MODE & M_ABSORB) {
+            if (MODE & (M_ABSORB | M_REZONE)) {
                 const unsigned long long m = __ballot(deleted);
                 if (m && lane == (int)(__ffsll((long long)m) - 1))
                     atomicAdd(&dead_blk[s], (unsigned long long)__popcll(m));
@@ -536,6 +652,7 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
                     __builtin_nontemporal_store(vz, at_off(F[2], bo)); __builtin_nontemporal_store(vr, at_off(F[3], bo));
                     __builtin_nontemporal_store(vt, at_off(F[4], bo)); __builtin_nontemporal_store(p.Az, at_off(F[5], bo));
                     __builtin_nontemporal_store(p.Ar, at_off(F[6], bo));
+                    if ((MODE & M_REZONE) && wq_changed) __builtin_nontemporal_store(p.WQ, at_off(F[7], bo));
                 } else {
                     if (MODE & M_ABSORB) {
                         gp(F[1])[i] = p.R; gp(F[3])[i] = vr; gp(F[6])[i] = p.Ar;
@@ -1151,9 +1268,6 @@ __global__ void clear_kernel(double *a, size_t n)
 }
 
 // ------------------------------------------------------------ rezoning
-// Random-stream tag of the merge draw: apart from every (species, gas) tag
-// of the collision pass, which stay below 2^16.
-constexpr uint32_t kMergeTag = 0x4D520000u;
 
 // split_particle_weight followed by merge_particle_random,
 // R/src/base/Rezoning.F90:53-92, one thread per slot.  The reference weight
@@ -1593,7 +1707,7 @@ static void launch_shape(ipm_ctx *c, int sel)
 template <int MODE>
 static int launch_particles(ipm_ctx *c, int sel)
 {
-    if constexpr (MODE == (M_PUSH | M_ABSORB | M_DEPOSIT | M_MCC)) {
+    if constexpr ((MODE & ~M_REZONE) == (M_PUSH | M_ABSORB | M_DEPOSIT | M_MCC)) {
         const int b = c->pk_block, w = c->pk_winr;
         if (b == 128 && w == 24) launch_shape<MODE, 128, 24>(c, sel);
         else if (b == 512 && w == 24) launch_shape<MODE, 512, 24>(c, sel);
@@ -2053,6 +2167,26 @@ int ipm_split_merge(ipm_ctx *c, int s, int mode, double weight, double alpha, do
     return IPM_OK;
 }
 
+int ipm_set_step_rezone(ipm_ctx *c, int s, int mode, double weight, double alpha, double grid_r, double merge_limit)
+{
+    if (!c) return fail(IPM_EINVAL, "null context");
+    if (s < 0 || s >= c->ns) return fail(IPM_EINVAL, "invalid species index %d", s);
+    if (mode < 0 || mode > 3) return fail(IPM_EINVAL, "invalid split/merge mode %d", mode);
+    if (mode && (!(weight > 0.0) || !(grid_r > 0.0) || !std::isfinite(weight) || !std::isfinite(grid_r) ||
+                 !std::isfinite(alpha) || !std::isfinite(merge_limit)))
+        return fail(IPM_EINVAL, "split/merge needs finite weight > 0, grid_r > 0, alpha and merge_limit");
+    bool before = false, after = false;
+    for (int q = 0; q < c->ns; ++q) before = before || c->h.rz_mode[q] != 0;
+    c->h.rz_mode[s] = mode;
+    c->h.rz_weight[s] = weight;
+    c->h.rz_alpha[s] = alpha;
+    c->h.rz_gridr[s] = grid_r;
+    c->h.rz_limit[s] = merge_limit;
+    for (int q = 0; q < c->ns; ++q) after = after || c->h.rz_mode[q] != 0;
+    if (before != after) drop_graphs(c);   // the step launches the other particle-pass variant
+    return sync_state(c);
+}
+
 static int check_sel(ipm_ctx *c, int s)
 {
     if (s < -1 || s >= c->ns) return fail(IPM_EINVAL, "invalid species index %d", s);
@@ -2304,7 +2438,11 @@ static int step_enqueue(ipm_ctx *c, bool capturing)
     const bool t = c->timing && !capturing;
     int a = 0, b = 0, rc;
     if (t && (rc = timed_event(c, &a))) return rc;
-    if ((rc = launch_particles<M_PUSH | M_ABSORB | M_DEPOSIT | M_MCC>(c, -1))) return rc;
+    bool rezone = false;
+    for (int q = 0; q < c->ns; ++q) rezone = rezone || c->h.rz_mode[q] != 0;
+    if (rezone) rc = launch_particles<M_PUSH | M_ABSORB | M_DEPOSIT | M_MCC | M_REZONE>(c, -1);
+    else rc = launch_particles<M_PUSH | M_ABSORB | M_DEPOSIT | M_MCC>(c, -1);
+    if (rc) return rc;
     if (t) {
         if ((rc = timed_event(c, &b))) return rc;
         c->ev_pairs[0].push_back({a, b});

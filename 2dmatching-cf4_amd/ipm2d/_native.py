@@ -67,6 +67,7 @@ SIGNATURES = {
     "ipm_set_timing": [_vp, C.c_int],
     "ipm_take_timing": [_vp, _dp, _i64p],
     "ipm_sort": [_vp, C.c_int],
+    "ipm_set_step_rezone": [_vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double],
     "ipm_set_sort_interval": [_vp, C.c_int],
     "ipm_set_launch": [_vp, C.c_int, C.c_int],
     "ipm_set_deposit_window": [_vp, C.c_int],

@@ -281,6 +281,14 @@ class HotPath:
                st.ctypes.data_as(C.POINTER(C.c_int64)))
         return tuple(int(x) for x in st)
 
+    def set_step_rezone(self, s, weight=1.0, alpha=0.0, grid_r=1.0, merge_limit=0.34, split=True, merge=True):
+        """split_merge of species s folded into every later one_step /
+        one_step_graph (after the deposit, before MCC, as OneStep orders
+        them); split=merge=False switches it off."""
+        mode = (1 if split else 0) | (2 if merge else 0)
+        N.call("ipm_set_step_rezone", self._h, s, mode, float(weight), float(alpha), float(grid_r),
+               float(merge_limit))
+
     def recombine(self, reactions, rates, time_step):
         """executingRecombination2D (Recombination.F90:164-273).  reactions:
         (first, second, type) triples with 0-based species, type 0 constant

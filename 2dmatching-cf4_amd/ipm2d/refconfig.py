@@ -308,15 +308,17 @@ class RunConfig:
         d = self.dz
         return self.plasma.init_density / float(ppg) * 2.0 * G.PI * d * d * d * float(ir)
 
-    def rezoning(self, hp, ppg=None):
+    def rezoning(self, hp, ppg=None, fused_electrons=False):
         """A Rezoning control for the species of ``hp`` with the base weight
         every bundle starts from (ppg_dif(k) = ppg for every species,
-        init.F90:552-556)."""
+        init.F90:552-556); fused_electrons folds the electrons' every-step
+        split/merge into the fused step (Rezoning's docstring)."""
         from .rezoning import Rezoning
         ppg = self.plasma.ppg if ppg is None else int(ppg)
         w = self.base_weight(ppg)
         return Rezoning(hp, self.nz, self.nr, [w] * hp.ns, [ppg] * hp.ns, alpha=self.plasma.alpha,
-                        period=self.period, limits=dataclasses.replace(self.plasma.rezoning))
+                        period=self.period, limits=dataclasses.replace(self.plasma.rezoning),
+                        fused_electrons=fused_electrons)
 
     def recombination_control(self, hp):
         from .recombination import Recombination

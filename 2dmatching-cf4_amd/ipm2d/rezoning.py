@@ -8,6 +8,14 @@ weighting_update_base_particle_number (Rezoning.F90:95-119) and the ions are
 split and merged.  The per-particle work runs on the device
 (``HotPath.split_merge``); this class keeps what the reference keeps in each
 ParticleBundle (Weight, alpha, grid_r = Nr - 1) and the module's limits.
+
+With ``fused_electrons=True`` the electrons' every-step split/merge runs
+inside the fused step (``HotPath.set_step_rezone``: after the deposit and
+before the collisions, OneStep's order) instead of as a separate pass;
+``control(timer)`` then only updates the electrons' Weight every ``period``
+steps (taking effect from the next step, as in the reference) and splits and
+merges the ions as before, after the step's collisions rather than before
+them.
 """
 import dataclasses
 
@@ -26,7 +34,7 @@ class RezoningLimits:
 
 
 class Rezoning:
-    def __init__(self, hp, nz, nr, weights, ppg_ref, alpha=0.01, period=1, limits=None):
+    def __init__(self, hp, nz, nr, weights, ppg_ref, alpha=0.01, period=1, limits=None, fused_electrons=False):
         self.hp = hp
         self.nz, self.nr = int(nz), int(nr)
         self.weight = [float(w) for w in weights]          # PB(k)%Weight
@@ -39,6 +47,12 @@ class Rezoning:
             raise ValueError("one weight and one ppg_ref per species")
         if self.period < 1:
             raise ValueError("period must be >= 1")
+        self.fused_electrons = bool(fused_electrons) and self.limits.is_open_split_merge
+        if self.fused_electrons:
+            self._fold_electrons()
+
+    def _fold_electrons(self):
+        self.hp.set_step_rezone(0, self.weight[0], self.alpha, self.grid_r, self.limits.merge_limit)
 
     def reference_weight(self, s, r):
         """CalRW, R/src/pic/particle/ParticleBundle.f90:393-402."""
@@ -69,9 +83,12 @@ class Rezoning:
         out = {}
         if not self.limits.is_open_split_merge:
             return out
-        out[0] = self.split_merge(0)
+        if not self.fused_electrons:
+            out[0] = self.split_merge(0)
         if timer % self.period == 0:
             self.update_base_weight(0)
+            if self.fused_electrons:
+                self._fold_electrons()
             for k in range(1, self.hp.ns):
                 self.update_base_weight(k)
                 out[k] = self.split_merge(k)

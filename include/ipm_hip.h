@@ -147,6 +147,17 @@ int ipm_step_graph(ipm_ctx *ctx);
  * call; cleared after read. */
 int ipm_set_timing(ipm_ctx *ctx, int on);
 int ipm_take_timing(ipm_ctx *ctx, double *ms, int64_t *launches);
+/* Split/merge of species s inside every later ipm_step / ipm_step_graph:
+ * split_particle_weight and merge_particle_random (R/src/base/Rezoning.F90:
+ * 53-92) with the arguments of ipm_split_merge, applied in the fused
+ * particle pass where OneStep calls particle_split_merge_control
+ * (R/src/onestep.F90:160): after the deposit, before the collision pass,
+ * which also draws for the copies.  mode 0 switches it off again.  Weight
+ * and the other parameters may change between steps (the reference updates
+ * Weight every CF%Period) without a new graph.  Results equal ipm_move_absorb,
+ * ipm_p2c, ipm_split_merge and ipm_mcc called in that order. */
+int ipm_set_step_rezone(ipm_ctx *ctx, int s, int mode, double weight, double alpha, double grid_r,
+                        double merge_limit);
 /* Cell sort (counting sort by cell; drops dead slots). s = -1: all.  No
  * counterpart in the reference (its bundles are unordered); nothing the
  * path computes depends on particle order. */

@@ -153,6 +153,9 @@ class _Recorder:
         self.calls.append((s, weight, alpha, grid_r, merge_limit))
         return (0, 0, 0)
 
+    def set_step_rezone(self, s, weight, alpha, grid_r, merge_limit):
+        self.calls.append(("step", s, weight, alpha, grid_r, merge_limit))
+
 
 def test_controller_schedule_and_base_weight():
     nz, nr, ppg = 11, 9, 10             # 80 cells * 10 = 800 reference particles
@@ -170,3 +173,19 @@ def test_controller_schedule_and_base_weight():
     hp.calls.clear()
     Rezoning(hp, nz, nr, [1.0] * 3, [ppg] * 3, limits=RezoningLimits(is_open_split_merge=False)).control(0)
     assert hp.calls == []
+
+
+def test_controller_with_electrons_in_the_step():
+    """fused_electrons: the electrons' split/merge is handed to the step once
+    (and again with each new Weight every period, taking effect from the
+    next step), never run as a separate call; the ions keep theirs."""
+    nz, nr, ppg = 11, 9, 10
+    hp = _Recorder([900, 700, 50])
+    rz = Rezoning(hp, nz, nr, [1.0, 2.0, 3.0], [ppg] * 3, alpha=0.01, period=4, fused_electrons=True)
+    assert hp.calls == [("step", 0, 1.0, 0.01, nr - 1, 0.34)]
+    hp.calls.clear()
+    rz.control(1)
+    assert hp.calls == []
+    rz.control(4)
+    assert hp.calls[0] == ("step", 0, 1.0 * 1.01, 0.01, nr - 1, 0.34)
+    assert [c[0] for c in hp.calls[1:]] == [1, 2]
