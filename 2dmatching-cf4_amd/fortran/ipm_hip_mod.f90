@@ -169,6 +169,52 @@ module ipm_hip
             type(c_ptr), value :: ctx
             integer(c_int), value :: blocks_per_cu
         end function
+        integer(c_int) function ipm_set_deposit_window(ctx, rows) bind(C, name="ipm_set_deposit_window")
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: rows
+        end function
+        integer(c_int) function ipm_set_sort_interval(ctx, steps) bind(C, name="ipm_set_sort_interval")
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: steps
+        end function
+        integer(c_int) function ipm_set_step_rezone(ctx, s, mode, weight, alpha, grid_r, merge_limit) &
+                bind(C, name="ipm_set_step_rezone")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: s, mode
+            real(c_double), value :: weight, alpha, grid_r, merge_limit
+        end function
+        integer(c_int) function ipm_upload_species_device(ctx, s, offset, n, po_dev, ld) &
+                bind(C, name="ipm_upload_species_device")
+            import :: c_ptr, c_int, c_int64_t
+            type(c_ptr), value :: ctx, po_dev
+            integer(c_int), value :: s
+            integer(c_int64_t), value :: offset, n, ld
+        end function
+        integer(c_int) function ipm_moments_noalloc(ctx, s, implicit_j, rho, energy_ev, mp, jz, jr) &
+                bind(C, name="ipm_moments_noalloc")
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx, rho, energy_ev, mp, jz, jr
+            integer(c_int), value :: s, implicit_j
+        end function
+        integer(c_int) function ipm_window_info(ctx, nparticles, nsegments, forward_ms, adjoint_ms) &
+                bind(C, name="ipm_window_info")
+            import :: c_ptr, c_int, c_int64_t, c_float
+            type(c_ptr), value :: ctx
+            integer(c_int64_t), intent(out) :: nparticles
+            integer(c_int), intent(out) :: nsegments
+            real(c_float), intent(out) :: forward_ms, adjoint_ms
+        end function
+        integer(c_int) function ipm_window_download(ctx, checkpoint, state, absorbed_step) &
+                bind(C, name="ipm_window_download")
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: checkpoint
+            real(c_double), intent(inout) :: state(*)
+            integer(c_int), intent(inout) :: absorbed_step(*)
+        end function
         integer(c_int) function ipm_synchronize(ctx) bind(C, name="ipm_synchronize")
             import :: c_ptr, c_int
             type(c_ptr), value :: ctx
@@ -349,6 +395,18 @@ contains
         CoeD = coe(:, :, 4)
         CoeE = coe(:, :, 5)
         deallocate(coe)
+    end subroutine
+
+    ! The electrons' every-step split_particle_weight + merge_particle_random
+    ! (particle_split_merge_control, R/src/base/Rezoning.F90:33-34) folded into
+    ! every later ipm_step / ipm_step_graph; call again whenever PB(k)%Weight
+    ! changes (every CF%Period).
+    subroutine ipm_SplitMergeInStep(ctx, k, Weight, alpha, grid_r, merge_limit)
+        type(c_ptr), intent(in) :: ctx
+        integer, intent(in) :: k
+        real(c_double), intent(in) :: Weight, alpha, grid_r, merge_limit
+        call ipm_check(ipm_set_step_rezone(ctx, int(k, c_int), 3_c_int, Weight, alpha, grid_r, merge_limit), &
+                       "SplitMergeInStep")
     end subroutine
 
     ! WeightingParticleField for species k: R/src/diag/diagParticleField.F90:173.
