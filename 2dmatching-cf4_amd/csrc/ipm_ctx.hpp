@@ -61,7 +61,6 @@ struct ipm_ctx {
     unsigned long long *hist = nullptr, *offs = nullptr, *cursor = nullptr, *partial = nullptr,
                        *partial2 = nullptr;
     double *qbuf = nullptr;                // q_cap records of 8 doubles
-    double *cbuf = nullptr;                // collision records, kCollFields arrays of q_cap doubles
     double *p_stage = nullptr;             // per collision block: staged products, meta, counts
     unsigned *p_meta = nullptr, *p_cnt = nullptr;
     unsigned *q_tile_count = nullptr;      // q_ntiles

@@ -41,6 +41,14 @@ struct DevTable {
     const int *resultant;
     const double *threshold;
     const double *prob;     // Probility(j,i) -> prob[(i-1)*nreaction + (j-1)]
+    // Null-event screen (models 1 and 2; null: no screen).  nb_tab holds
+    // levels L = 0, 1, ... of (nsigma + 1) doubles: entry b of level L bounds
+    // every value SelectProbility can interpolate in energy bins b .. b+2^L-1
+    // (bin 0: N < 1; bin b < nsigma: rows b-1 and b; bin nsigma: the last row,
+    // before its E/emax factor).  nb_vmax bounds the speed VelMaxInit can
+    // draw, nb_ecoef = miu/2/JTOEV, nb_eps the error of approx_lx in bins.
+    const double *nb_tab;
+    double nb_vmax, nb_ecoef, nb_eps, nb_inv_einterval, nb_inv_emax, pad4;
 };
 
 // Device-resident description of the whole problem: one pointer is the only
@@ -60,7 +68,7 @@ struct DevState {
     double *sigma;                     // nz*nr
     unsigned long long *mcc_counts;    // optional: (s*ng+g)*128 + j
     int count_events;
-    int pad;
+    int null_screen;                   // use the tables' null-event screen
     // Queue of particles selected for a collision (null-collision pass).
     // Species s owns q_maxtiles[s] = ceil(cap/kTile) tile regions of
     // q_tile_cap[s] entries; entry k of live tile lt of species s sits at
@@ -76,12 +84,6 @@ struct DevState {
     int *tile_anchor;
     long long q_region[IPM_NS_MAX], q_cbase[IPM_NS_MAX], q_tile_cap[IPM_NS_MAX];
     double q_weight[IPM_NS_MAX];       // expected entries per tile (collision-pass split)
-    // Collisions (non-null reactions) a collision block found, kept until
-    // the same block finishes them: kCollFields arrays of c_cap doubles
-    // (structure of arrays), one record per collision, in the region of the
-    // block's queue tiles.
-    double *c_rec;
-    long long c_cap;
     // Per collision block (c_grid blocks): the products it staged
     // (kProdStage records of 8 doubles, each with species | rank << 8, and
     // per-species counts) for the append pass.
@@ -494,15 +496,15 @@ __device__ __forceinline__ void post_collision(Mc &m, bool vahedi, Rng &rg)
     m.poi.Vz = m.poi.Vz - m.mass_ratio * (gz * FcosTheta + hz * SinTheta);
 }
 
-// SelectProbility, R/src/mc/TypeMCC.f90:173-216.  The uniform R is drawn
-// before the scan (the table read does not consume draws, so the order is
-// immaterial) so that no per-reaction temporary array is needed.
-__device__ __forceinline__ int select_probability(const Mc &m, const DevTable &t, Rng &rg)
+// SelectProbility, R/src/mc/TypeMCC.f90:173-216.  R is the uniform of the
+// reaction choice, which the caller draws before Updater's target sampling
+// (draw 1 of the particle's stream for this gas): the particle pass can then
+// screen out null events before they are queued.
+__device__ __forceinline__ int select_probability(const Mc &m, const DevTable &t, double R)
 {
     const int Nr = t.nreaction, Ns = t.nsigma;
     const double E = m.energy;
     const double *__restrict__ P = t.prob;
-    const double R = rg.next();
     double lx = (log10(E) - t.emin) / t.einterval;
     int N = (E > 0.0) ? (int)ceil(lx) : 0;
     int mode;
@@ -537,16 +539,57 @@ __device__ __forceinline__ int select_probability(const Mc &m, const DevTable &t
     return 0;
 }
 
-// SelectProbilityReactive, R/src/mc/TypeMCC.f90:218-236
-__device__ __forceinline__ int select_probability_reactive(const Mc &m, const DevTable &t, Rng &rg)
+// SelectProbilityReactive, R/src/mc/TypeMCC.f90:218-236; R (the draw of
+// beta) is drawn before Updater, as SelectProbility's is, and the
+// non-reactive choice takes the stream's next draw.
+__device__ __forceinline__ int select_probability_reactive(const Mc &m, const DevTable &t, double R, Rng &rg)
 {
-    double R = rg.next();
     double beta = t.beta_max * sqrt(R);
     if (beta <= 1.0) {
         if (m.energy <= t.threshold[1]) return 1;
-        return select_probability(m, t, rg);
+        return select_probability(m, t, rg.next());
     }
     return 1;
+}
+
+// (log10(E) - emin) / einterval within t.nb_eps bins, for E > 0 normal:
+// log2 E = exponent + log2 of the mantissa in [0.5, 1), the latter from the
+// hardware single-precision log2 (error below 1e-6 there, which nb_eps
+// covers together with the rounding of the float conversion).
+__device__ __forceinline__ double approx_lx(const DevTable &t, double e)
+{
+    int ex;
+    const double mt = frexp(e, &ex);
+    const double l2 = (double)ex + (double)__log2f((float)mt);
+    return (l2 * 0.30102999566398120 - t.emin) * t.nb_inv_einterval;
+}
+
+// True only when SelectProbility (models 1 and 2) returns 0 for every target
+// velocity VelMaxInit can draw: R is at or above every value the table can
+// interpolate over the relative energies the particle can reach.  vz, vr, vt
+// are the particle's velocity after VelRes(VFactor) (m/s).  The relative
+// speed lies within nb_vmax of the particle's speed, so the energy within
+// [e_lo, e_hi]; the bins of that range come from approx_lx widened by
+// nb_eps, and the sparse table gives their bound with two loads.  The
+// threshold test of SelectProbility only lowers values, so it is left out.
+__device__ __forceinline__ bool surely_null(const DevTable &t, double vz, double vr, double vt, double R)
+{
+    const double v2 = vz * vz + vr * vr + vt * vt;
+    if (!(v2 < 1e300)) return false;
+    const double v = sqrt(v2);
+    const double glo = v - t.nb_vmax, ghi = v + t.nb_vmax;
+    const double ehi = t.nb_ecoef * ghi * ghi * (1.0 + 1e-12);
+    const double elo = glo > 0.0 ? t.nb_ecoef * glo * glo * (1.0 - 1e-12) : 0.0;
+    const int Ns = t.nsigma;
+    int lo = 0, hi = Ns;
+    if (elo > 1e-300) lo = (int)fmin(fmax(ceil(approx_lx(t, elo) - t.nb_eps), 0.0), (double)Ns);
+    if (ehi < 1e300) hi = (int)fmin(fmax(ceil(approx_lx(t, ehi) + t.nb_eps), 0.0), (double)Ns);
+    if (hi < lo) return false;
+    const int L = 31 - __clz(hi - lo + 1);
+    const double *row = t.nb_tab + (size_t)L * (size_t)(Ns + 1);
+    double bound = fmax(row[lo], row[hi - (1 << L) + 1]);
+    if (hi == Ns) bound = bound * fmax(1.0, ehi * t.nb_inv_emax) * (1.0 + 1e-12);
+    return R >= bound;
 }
 
 // Cell (ceiling(Z), ceiling(R)) of a particle, clamped into the grid's

@@ -47,11 +47,14 @@ constexpr uint32_t kMergeTag = 0x4D520000u;
 constexpr int kRecD = 8;
 constexpr int kRecParts = kRecD / 2;
 typedef double rec2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ unsigned load_rec(const double *r, Part &p)
+// Queue record: Z R Vz Vr Vt WQ, index in the tile, and the uniform of the
+// reaction choice (draw 1 of the particle's stream; with one gas only).
+__device__ __forceinline__ unsigned load_rec(const double *r, Part &p, double &u)
 {
     const auto *v = (const __attribute__((address_space(1))) rec2 *)r;
     const rec2 a = v[0], b = v[1], c = v[2], d = v[3];
     p.Z = a.x; p.R = a.y; p.Vz = b.x; p.Vr = b.y; p.Vt = c.x; p.WQ = c.y; p.Az = 0.0; p.Ar = 0.0;
+    u = d.y;
     return (unsigned)__double_as_longlong(d.x);
 }
 
@@ -123,8 +126,11 @@ __device__ __forceinline__ void ion_reactive(Mc &m, double th)
 // MCC loop, R/src/mc/MCCPublic.F90:61-97, with SelectCollisionElectron
 // (R/src/mc/MCCElectron.F90:13-147) and SelectCollisionIon (R/src/mc/MCCIon.F90:6-51),
 // in two halves.  mcc_begin: VelRes(VFactor), Updater and the reaction
-// choice; it returns the reaction index (0 = null event).
-__device__ __forceinline__ int mcc_begin(const DevState &st, int s, const DevTable &t, const Part &p, Mc &m, Rng &rg)
+// choice with the uniform u the caller drew before it (draw 1 of the
+// particle's stream for this gas; Updater's target sampling starts at draw
+// 2); it returns the reaction index (0 = null event).
+__device__ __forceinline__ int mcc_begin(const DevState &st, int s, const DevTable &t, const Part &p, Mc &m, Rng &rg,
+                                         double u)
 {
     m.poi = p;
     m.poi.Vz = m.poi.Vz * st.vfactor;
@@ -132,8 +138,8 @@ __device__ __forceinline__ int mcc_begin(const DevState &st, int s, const DevTab
     m.poi.Vt = m.poi.Vt * st.vfactor;
     mc_update(m, st.mass[s], t, rg);
     int idx = 0;
-    if (t.model == 1 || t.model == 2) idx = select_probability(m, t, rg);
-    else if (t.model == 3) idx = select_probability_reactive(m, t, rg);
+    if (t.model == 1 || t.model == 2) idx = select_probability(m, t, u);
+    else if (t.model == 3) idx = select_probability_reactive(m, t, u, rg);
     return idx;
 }
 
@@ -219,7 +225,8 @@ __device__ __forceinline__ void mcc_collide(const DevState &st, DevCounters *ctr
                                             const DevTable &t, Part &p, Rng &rg, const ProdSink *ps = nullptr)
 {
     Mc m;
-    const int idx = mcc_begin(st, s, t, p, m, rg);
+    const double u = rg.next();
+    const int idx = mcc_begin(st, s, t, p, m, rg, u);
     mcc_finish(st, ctr, s, g, t, m, rg, idx, p, ps);
 }
 
@@ -396,9 +403,37 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
         };
         // Queue records staged by this wave (wave-uniform count).  A flush
         // reserves space in the tile's region with one LDS atomic and writes
-        // the records with every lane storing 16 contiguous bytes.
+        // the records with every lane storing 16 contiguous bytes.  The flush
+        // at the end of the tile first drops the records whose collision is
+        // surely a null event (one gas, a table with a screen): lane r tests
+        // record r, and the kept ones are packed to the front.  Their
+        // velocity already carries the VelRes round trip, so nothing else is
+        // left of them but the event count.
         unsigned cw = 0;
-        auto flush_wave = [&]() {
+        auto flush_wave = [&](bool screen) {
+            if (screen) {
+                const DevTable &tb = st.tab[s];
+                double *wst = q_wst + wv * 64 * kRecD;
+                const bool has = (unsigned)lane < cw;
+                double rv[kRecD];
+                bool keep = has;
+                if (has) {
+#pragma unroll
+                    for (int f = 0; f < kRecD; ++f) rv[f] = wst[lane * kRecD + f];
+                    keep = !surely_null(tb, rv[2] * st.vfactor, rv[3] * st.vfactor, rv[4] * st.vfactor, rv[7]);
+                }
+                const unsigned long long km = __ballot(keep);
+                const unsigned nkeep = (unsigned)__popcll(km);
+                if (st.count_events && lane == 0 && cw > nkeep)
+                    atomicAdd(&st.mcc_counts[(size_t)s * 128], (unsigned long long)(cw - nkeep));
+                if (keep) {
+                    const unsigned r = (unsigned)__popcll(km & ((1ull << lane) - 1ull));
+#pragma unroll
+                    for (int f = 0; f < kRecD; ++f) wst[r * kRecD + f] = rv[f];
+                }
+                cw = nkeep;
+                if (!cw) return;
+            }
             unsigned ob = 0;
             if (lane == 0) ob = atomicAdd(&q_scount[qbuf], cw);
             ob = __shfl(ob, 0);
@@ -439,7 +474,10 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
             // draw 0 of the (slot, step, species|gas<<8) stream against
             // CollisionRatio.  Selected particles go to the collision queue
             // with their post-push state; collide_kernel finishes them.
+            // Draw 1 (the other half of the same Philox block) is the
+            // uniform of the reaction choice; it rides in the queue record.
             bool selected = false;
+            double usel = 0.0;
             if ((MODE & M_MCC) && alive) {
                 if (!(MODE & (M_PUSH | M_ABSORB))) {
                     if (!(MODE & M_DEPOSIT)) p.R = gp(F[1])[i];
@@ -450,7 +488,10 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
                     const DevTable &tb = st.tab[s * st.ng + g];
                     if (tb.model == 0) continue;
                     Rng rg = make_rng(sk, seed, step, (uint32_t)s | ((uint32_t)g << 8));
-                    if (rg.next() < tb.ratio) selected = true;
+                    if (rg.next() < tb.ratio) {
+                        selected = true;
+                        usel = rg.next();
+                    }
                 }
             }
             if (MODE & M_DEPOSIT) {
@@ -557,7 +598,7 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
                 const unsigned long long m = __ballot(selected);
                 const unsigned nsel = (unsigned)__popcll(m);
                 if (nsel) {
-                    if (cw + nsel > 64u) flush_wave();
+                    if (cw + nsel > 64u) flush_wave(false);
                     if (selected) {
                         if (!(MODE & (M_PUSH | M_ABSORB | M_DEPOSIT))) p.WQ = gp(F[7])[i];
                         const unsigned r = cw + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
@@ -565,7 +606,7 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
                         d[0] = p.Z; d[1] = p.R; d[2] = p.Vz; d[3] = p.Vr;
                         d[4] = p.Vt; d[5] = p.WQ;
                         d[6] = __longlong_as_double((long long)(i - base));
-                        d[7] = 0.0;
+                        d[7] = usel;
                     }
                     cw += nsel;
                 }
@@ -581,6 +622,7 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
                 for (unsigned j = 1; j <= jmax; ++j) {
                     const bool mine = j <= ncopy;
                     bool csel = false;
+                    double ucp = 0.0;
                     Part cp = p;
                     unsigned long long q = 0;
                     if (mine) {
@@ -593,7 +635,10 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
                                 const DevTable &tb = st.tab[s * st.ng + g];
                                 if (tb.model == 0) continue;
                                 Rng rg = make_rng(ck, seed, step, (uint32_t)s | ((uint32_t)g << 8));
-                                if (rg.next() < tb.ratio) csel = true;
+                                if (rg.next() < tb.ratio) {
+                                    csel = true;
+                                    ucp = rg.next();
+                                }
                             }
                         }
                         // a selected copy's stored velocity carries the VelRes
@@ -611,14 +656,14 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
                         const unsigned long long m = __ballot(csel);
                         const unsigned nsel = (unsigned)__popcll(m);
                         if (nsel) {
-                            if (cw + nsel > 64u) flush_wave();
+                            if (cw + nsel > 64u) flush_wave(false);
                             if (csel) {
                                 const unsigned r = cw + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
                                 double *d = q_wst + wv * 64 * kRecD + r * kRecD;
                                 d[0] = cp.Z; d[1] = cp.R; d[2] = cp.Vz; d[3] = cp.Vr;
                                 d[4] = cp.Vt; d[5] = cp.WQ;
                                 d[6] = __longlong_as_double((long long)q - base);
-                                d[7] = 0.0;
+                                d[7] = ucp;
                             }
                             cw += nsel;
                         }
@@ -665,7 +710,7 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
         }
 
         if (MODE & M_MCC) {
-            if (cw) flush_wave();
+            if (cw) flush_wave(st.null_screen && st.ng == 1 && st.tab[s].nb_tab != nullptr);
         }
         if (MODE & M_DEPOSIT) {
             __syncthreads();
@@ -709,114 +754,26 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
 // threads contend for (device-wide atomics on one address serialize, and
 // the reference's append of products, MCCAddNewParticle,
 // R/src/mc/MCCPublic.F90:209-226, would otherwise put one on every product).
-//   collide_kernel, first part: for each gas in order, the selection draw is
-//     repeated and, if it hits, VelRes(VFactor), Updater and the reaction
-//     choice of MCCPublic.F90:61-97 follow.  A null event ends there (its
-//     VelRes round trip is already in the particle's slot with one gas); a
-//     real reaction is written to a collision record in the block's own
-//     region (the region of its queue tiles).
-//   collide_kernel, second part: the block runs its records with full
-//     wavefronts (the reaction, VelRes back, the gases after it, the
-//     write-back) and stages the products in LDS, then in its own slice of
-//     a global stage.
+//   collide_kernel: for each queued particle and each gas in order, the
+//     selection draw (repeated with several gases; with one the particle
+//     pass made it), VelRes(VFactor), Updater, the reaction choice and the
+//     reaction of MCCPublic.F90:61-97, VelRes back; the changed fields are
+//     written back and products staged in LDS, then in the block's own slice
+//     of a global stage.  With one gas the particle pass has already dropped
+//     the surely null events (and stored every selected particle's VelRes
+//     round trip), so nearly every queued particle reacts and wavefronts run
+//     the reaction code full; with several gases this kernel skips the
+//     target sampling of surely null events itself.
 //   collide_append_kernel: block b appends its staged products after the
 //     products of blocks 0..b-1 of the same species.
 //   collide_fold_kernel: folds the step's products into n[] and advances the
 //     random stream's step key.
-// Most selections are null events, so the two parts keep the reaction code
-// (scattering angles, products) off the wavefronts that would mostly idle
-// in it.  collide_kernel splits its grid over the species in proportion to
-// their expected work (every species with live tiles gets a block); a block
-// takes a contiguous range of its species' tiles, scans their entry counts
-// in LDS and processes the entries flat.  All three use fixed grids, so
-// they can be captured.
+// collide_kernel splits its grid over the species in proportion to their
+// expected work (every species with live tiles gets a block); a block takes
+// a contiguous range of its species' tiles, scans their entry counts in LDS
+// and processes the entries flat.  All three use fixed grids, so they can be
+// captured.
 constexpr int kChunk = 1024;   // tile counts scanned per LDS pass
-constexpr int kCollFields = 24;
-// Record fields: 0-5 poi, 6-11 pot (Z R Vz Vr Vt WQ), 12 miu, 13 mass_ratio,
-// 14 energy, 15-19 gx gy gz gper g, 20 stream counters c0|c1, 21 c2|draw,
-// 22 slot, 23 species | gas << 8 | reaction << 16.  Az and Ar are read back
-// from the particle (the queue record does not carry them) and pot's are
-// formed from them as Updater does.
-
-__device__ __forceinline__ void put_coll(const DevState &st, unsigned long long e, const Mc &m, const Rng &rg,
-                                         long long i, int s, int g, int idx)
-{
-    double *c = st.c_rec + e;
-    const long long C = st.c_cap;
-    const double v[kCollFields] = {
-        m.poi.Z, m.poi.R, m.poi.Vz, m.poi.Vr, m.poi.Vt, m.poi.WQ,
-        m.pot.Z, m.pot.R, m.pot.Vz, m.pot.Vr, m.pot.Vt, m.pot.WQ,
-        m.miu, m.mass_ratio, m.energy, m.gx, m.gy, m.gz, m.gper, m.g,
-        __longlong_as_double((long long)((unsigned long long)rg.c0 | ((unsigned long long)rg.c1 << 32))),
-        __longlong_as_double((long long)((unsigned long long)rg.c2 | ((unsigned long long)rg.k << 32))),
-        __longlong_as_double(i),
-        __longlong_as_double((long long)((unsigned)s | ((unsigned)g << 8) | ((unsigned)idx << 16)))};
-#pragma unroll
-    for (int f = 0; f < kCollFields; ++f) gp(c)[(size_t)f * C] = v[f];
-}
-
-// The records [rbase, rbase + nrec) of one collide_kernel block, one per
-// thread: the reaction, VelRes back, the gases after it, the write-back.
-// Products are staged in the block's LDS sink and then written, with their
-// per-species counts, to block b's slice of the product stage.
-__device__ __forceinline__ void finish_records(const DevState &st, DevCounters *ctr, long long rbase, unsigned nrec,
-                                               double *prec, unsigned *pmeta, unsigned *pcount, unsigned *pspc)
-{
-    const ProdSink sink = {prec, pmeta, pcount, pspc};
-    const uint32_t step = ctr->step;
-    const unsigned long long seed = ctr->seed;
-    const long long C = st.c_cap;
-    for (unsigned e = threadIdx.x; e < nrec; e += kBlock) {
-        const double *c = st.c_rec + rbase + e;
-        double v[kCollFields];
-#pragma unroll
-        for (int f = 0; f < kCollFields; ++f) v[f] = gp(c)[(size_t)f * C];
-        const long long i = __double_as_longlong(v[22]);
-        const unsigned tag = (unsigned)__double_as_longlong(v[23]);
-        const int s = (int)(tag & 0xffu), g = (int)((tag >> 8) & 0xffu), idx = (int)(tag >> 16);
-        double *const *F = st.f[s];
-        const double az0 = gp(F[5])[i], ar0 = gp(F[6])[i];
-        const DevTable &t0 = st.tab[s * st.ng + g];
-        Mc m;
-        m.poi.Z = v[0]; m.poi.R = v[1]; m.poi.Vz = v[2]; m.poi.Vr = v[3];
-        m.poi.Vt = v[4]; m.poi.WQ = v[5]; m.poi.Az = az0; m.poi.Ar = ar0;
-        m.pot.Z = v[6]; m.pot.R = v[7]; m.pot.Vz = v[8]; m.pot.Vr = v[9];
-        m.pot.Vt = v[10]; m.pot.WQ = v[11];
-        m.pot.Ar = ar0 * st.mass[s] / t0.gas_mass;   // Updater, R/src/mc/TypeMCC.f90:90-91
-        m.pot.Az = az0 * st.mass[s] / t0.gas_mass;
-        m.miu = v[12]; m.mass_ratio = v[13]; m.energy = v[14];
-        m.gx = v[15]; m.gy = v[16]; m.gz = v[17]; m.gper = v[18]; m.g = v[19];
-        const unsigned long long w0 = (unsigned long long)__double_as_longlong(v[20]);
-        const unsigned long long w1 = (unsigned long long)__double_as_longlong(v[21]);
-        StateKey sk;
-        sk.c0 = (uint32_t)w0; sk.c1 = (uint32_t)(w0 >> 32); sk.c2 = (uint32_t)w1;
-        Rng rg = make_rng(sk, seed, step, (uint32_t)s | ((uint32_t)g << 8));
-        rg.seek((uint32_t)(w1 >> 32));
-        Part p;
-        mcc_finish(st, ctr, s, g, t0, m, rg, idx, p, &sink);
-        for (int g2 = g + 1; g2 < st.ng; ++g2) {
-            const DevTable &tb = st.tab[s * st.ng + g2];
-            if (tb.model == 0) continue;
-            Rng r2 = make_rng(sk, seed, step, (uint32_t)s | ((uint32_t)g2 << 8));
-            if (!(r2.next() < tb.ratio)) continue;
-            mcc_collide(st, ctr, s, g2, tb, p, r2, &sink);
-        }
-        __builtin_nontemporal_store(p.Vz, &F[2][i]);
-        __builtin_nontemporal_store(p.Vr, &F[3][i]);
-        __builtin_nontemporal_store(p.Vt, &F[4][i]);
-        if (__double_as_longlong(p.Az) != __double_as_longlong(az0) ||
-            __double_as_longlong(p.Ar) != __double_as_longlong(ar0)) {
-            __builtin_nontemporal_store(p.Az, &F[5][i]);
-            __builtin_nontemporal_store(p.Ar, &F[6][i]);
-        }
-    }
-    __syncthreads();
-    const unsigned np = min(*pcount, (unsigned)kProdStage);
-    double *gs = st.p_stage + (size_t)blockIdx.x * kProdStage * 8;
-    for (unsigned k = threadIdx.x; k < np * 8u; k += kBlock) gs[k] = prec[k];
-    for (unsigned k = threadIdx.x; k < np; k += kBlock) st.p_meta[(size_t)blockIdx.x * kProdStage + k] = pmeta[k];
-    if (threadIdx.x < IPM_NS_MAX) st.p_cnt[(size_t)blockIdx.x * IPM_NS_MAX + threadIdx.x] = pspc[threadIdx.x];
-}
 
 __global__ void __launch_bounds__(kBlock, 2)
 collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
@@ -824,13 +781,13 @@ collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
     const DevState &st = *stp;
     __shared__ unsigned cpre[kChunk + 1];
     __shared__ unsigned wtot[kBlock / 64];
-    __shared__ unsigned nrec;
     __shared__ double prec[kProdStage * 8];
     __shared__ unsigned pmeta[kProdStage];
     __shared__ unsigned pcount, pspc[IPM_NS_MAX];
     const int ns = st.ns;
     const unsigned G = gridDim.x;
-    if (threadIdx.x == 0) { nrec = 0; pcount = 0; }
+    if (threadIdx.x == 0) pcount = 0;
+    const ProdSink sink = {prec, pmeta, &pcount, pspc};
     if (threadIdx.x < IPM_NS_MAX) pspc[threadIdx.x] = 0;
 
     // species of this block and its tile range
@@ -868,9 +825,6 @@ collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
     }
     const uint32_t step = ctr->step;
     const unsigned long long seed = ctr->seed;
-    // this block's records go to the region of its queue tiles, which holds
-    // at least as many entries as the tiles can queue
-    const long long rbase = sb >= 0 ? st.q_region[sb] + t0 * st.q_tile_cap[sb] : 0;
     if (sb >= 0) {
         const int s = sb;
         const long long R = st.q_tile_cap[s];
@@ -916,56 +870,74 @@ collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
                 const long long lt = tc + lo;
                 const long long q = st.q_region[s] + lt * R + (e - cpre[lo]);
                 Part p;
-                const long long i = lt * kTile + load_rec(st.q_rec + (size_t)q * kRecD, p);
+                double urec;
+                const long long i = lt * kTile + load_rec(st.q_rec + (size_t)q * kRecD, p, urec);
                 const StateKey sk = state_key(p.Z, p.R, p.Vz, p.Vr, p.Vt);
+                // the queue record does not carry Az and Ar
+                p.Az = gp(F[5])[i];
+                p.Ar = gp(F[6])[i];
                 const Part p0 = p;
-                bool handed = false;
                 for (int g = 0; g < st.ng; ++g) {
                     const DevTable &tb = st.tab[s * st.ng + g];
                     if (tb.model == 0) continue;
                     Rng rg = make_rng(sk, seed, step, (uint32_t)s | ((uint32_t)g << 8));
+                    double u;
                     if (st.ng == 1) {
-                        rg.seek(1);   // the particle pass drew this same uniform and selected it
-                    } else if (!(rg.next() < tb.ratio)) {
-                        continue;
+                        // the particle pass made draws 0 (selection) and 1
+                        rg.seek(2);
+                        u = urec;
+                    } else {
+                        if (!(rg.next() < tb.ratio)) continue;
+                        u = rg.next();
+                        if (st.null_screen && tb.nb_tab &&
+                            surely_null(tb, p.Vz * st.vfactor, p.Vr * st.vfactor, p.Vt * st.vfactor, u)) {
+                            // a null event: the count and VelRes both ways
+                            if (st.count_events)
+                                atomicAdd(&st.mcc_counts[((size_t)s * st.ng + g) * 128], 1ull);
+                            p.Vz = p.Vz * st.vfactor * st.inv_vfactor;
+                            p.Vr = p.Vr * st.vfactor * st.inv_vfactor;
+                            p.Vt = p.Vt * st.vfactor * st.inv_vfactor;
+                            continue;
+                        }
                     }
                     Mc mc;
-                    const int idx = mcc_begin(st, s, tb, p, mc, rg);
-                    if (idx == 0) {
-                        mcc_finish(st, ctr, s, g, tb, mc, rg, 0, p);
-                        continue;
-                    }
-                    // a real reaction: the rest of this particle goes to the
-                    // finishing pass (one LDS atomic per wavefront)
-                    const unsigned long long bal = __ballot(1);
-                    const int leader = __ffsll((long long)bal) - 1;
-                    unsigned off = 0;
-                    if (lane == leader) off = atomicAdd(&nrec, (unsigned)__popcll(bal));
-                    off = __shfl(off, leader);
-                    const unsigned e2 = off + (unsigned)__popcll(bal & ((1ull << lane) - 1ull));
-                    put_coll(st, (unsigned long long)(rbase + e2), mc, rg, i, s, g, idx);
-                    handed = true;
-                    break;
+                    const int idx = mcc_begin(st, s, tb, p, mc, rg, u);
+                    mcc_finish(st, ctr, s, g, tb, mc, rg, idx, p, &sink);
                 }
-                // write back only what the collisions changed (null events
-                // change nothing but VelRes rounding, which the particle
-                // pass already stored when there is one gas; Az/Ar never):
-                // each field is a separate cache line of a separate array,
-                // stored non-temporally so the scattered lines do not crowd L2
-                if (!handed && st.ng != 1 && (__double_as_longlong(p.Vz) != __double_as_longlong(p0.Vz) ||
-                                __double_as_longlong(p.Vr) != __double_as_longlong(p0.Vr) ||
-                                __double_as_longlong(p.Vt) != __double_as_longlong(p0.Vt))) {
+                // write back what the collisions changed: with one gas the
+                // particle pass stored the VelRes round trip, all a null event
+                // leaves; Z, R and WQ never change.  Each field is a separate
+                // cache line of a separate array, stored non-temporally so the
+                // scattered lines do not crowd L2.
+                double vz0 = p0.Vz, vr0 = p0.Vr, vt0 = p0.Vt;
+                if (st.ng == 1) {
+                    vz0 = vz0 * st.vfactor * st.inv_vfactor;
+                    vr0 = vr0 * st.vfactor * st.inv_vfactor;
+                    vt0 = vt0 * st.vfactor * st.inv_vfactor;
+                }
+                if (__double_as_longlong(p.Vz) != __double_as_longlong(vz0) ||
+                    __double_as_longlong(p.Vr) != __double_as_longlong(vr0) ||
+                    __double_as_longlong(p.Vt) != __double_as_longlong(vt0)) {
                     __builtin_nontemporal_store(p.Vz, &F[2][i]);
                     __builtin_nontemporal_store(p.Vr, &F[3][i]);
                     __builtin_nontemporal_store(p.Vt, &F[4][i]);
+                }
+                if (__double_as_longlong(p.Az) != __double_as_longlong(p0.Az) ||
+                    __double_as_longlong(p.Ar) != __double_as_longlong(p0.Ar)) {
+                    __builtin_nontemporal_store(p.Az, &F[5][i]);
+                    __builtin_nontemporal_store(p.Ar, &F[6][i]);
                 }
             }
             __syncthreads();
         }
     }
-    // this block's real reactions, with the block's wavefronts full
+    // the block's products to its slice of the stage
     __syncthreads();
-    finish_records(st, ctr, rbase, nrec, prec, pmeta, &pcount, pspc);
+    const unsigned np = min(pcount, (unsigned)kProdStage);
+    double *gs = st.p_stage + (size_t)blockIdx.x * kProdStage * 8;
+    for (unsigned k = threadIdx.x; k < np * 8u; k += kBlock) gs[k] = prec[k];
+    for (unsigned k = threadIdx.x; k < np; k += kBlock) st.p_meta[(size_t)blockIdx.x * kProdStage + k] = pmeta[k];
+    if (threadIdx.x < IPM_NS_MAX) st.p_cnt[(size_t)blockIdx.x * IPM_NS_MAX + threadIdx.x] = pspc[threadIdx.x];
 }
 
 // Block b appends its staged products of species d after n[d], the
@@ -1608,7 +1580,44 @@ static void table_consts(ipm_ctx *c)
             t.miu = sm * gm / (sm + gm);
             t.mass_ratio = gm / (sm + gm);
             t.beta = 1.0 / (kKB * t.gas_temperature);
+            // VelMaxInit's speed sqrt(-3 log(R) / Beta / Mass) with R >= 2^-53
+            // (-log R <= 36.737), with margin
+            t.nb_vmax = sqrt(3.0 * 36.75 * kKB * t.gas_temperature / gm) * (1.0 + 1e-9);
+            t.nb_ecoef = 0.5 * t.miu / kJTOEV;
         }
+}
+
+// The null-event screen's bound table of a model 1 or 2 table (layout in
+// DevTable): per energy bin the largest probability any reaction can
+// interpolate to there, raised by 1e-12 for rounding, then the maxima over
+// runs of 2^L bins.  Tables with a value that is not finite get none.
+static std::vector<double> null_bound_table(int nreaction, int nsigma, const double *prob)
+{
+    const int Ns = nsigma, Nr = nreaction;
+    std::vector<double> rowmax(Ns);
+    for (int i = 0; i < Ns; ++i) {
+        double m = 0.0;
+        for (int j = 0; j < Nr; ++j) {
+            const double v = prob[(size_t)i * Nr + j];
+            if (!std::isfinite(v)) return {};
+            m = std::max(m, v);
+        }
+        rowmax[i] = m;
+    }
+    int levels = 1;
+    while ((2 << (levels - 1)) <= Ns + 1) ++levels;
+    std::vector<double> tab((size_t)levels * (Ns + 1), 2.0);
+    for (int b = 0; b <= Ns; ++b) {
+        const double m = b == 0 ? rowmax[0] : b == Ns ? rowmax[Ns - 1] : std::max(rowmax[b - 1], rowmax[b]);
+        tab[b] = m * (1.0 + 1e-12);
+    }
+    for (int L = 1; L < levels; ++L) {
+        const int h = 1 << (L - 1);
+        for (int b = 0; b + 2 * h - 1 <= Ns; ++b)
+            tab[(size_t)L * (Ns + 1) + b] =
+                std::max(tab[(size_t)(L - 1) * (Ns + 1) + b], tab[(size_t)(L - 1) * (Ns + 1) + b + h]);
+    }
+    return tab;
 }
 
 // Collision-queue regions: species s gets ceil(cap_s/1024) tile regions of
@@ -1642,14 +1651,11 @@ static int ensure_queue(ipm_ctx *c)
     if (need > c->q_cap || ntiles > c->q_ntiles) {
         HIPCHK(hipStreamSynchronize(c->stream));
         if (c->qbuf) HIPCHK(hipFree(c->qbuf));
-        if (c->cbuf) HIPCHK(hipFree(c->cbuf));
-        c->cbuf = nullptr;
         if (c->q_tile_count) HIPCHK(hipFree(c->q_tile_count));
         if (c->tanc) HIPCHK(hipFree(c->tanc));
         c->tanc = nullptr;
         c->qbuf = nullptr; c->q_tile_count = nullptr; c->q_cap = 0; c->q_ntiles = 0;
         HIPCHK(hipMalloc(&c->qbuf, sizeof(double) * kRecD * (size_t)need));
-        HIPCHK(hipMalloc(&c->cbuf, sizeof(double) * kCollFields * (size_t)need));
         HIPCHK(hipMalloc(&c->q_tile_count, sizeof(unsigned) * (size_t)ntiles));
         HIPCHK(hipMalloc(&c->tanc, sizeof(int) * 2 * (size_t)ntiles));
         HIPCHK(hipMemset(c->tanc, 0, sizeof(int) * 2 * (size_t)ntiles));
@@ -1670,8 +1676,6 @@ static int ensure_queue(ipm_ctx *c)
         return fail(IPM_EINVAL, "collision grid of %d blocks is smaller than the %d species", c->h.c_grid, c->ns);
     c->h.q_rec = c->qbuf;
     c->h.tile_anchor = c->tanc;
-    c->h.c_rec = c->cbuf;
-    c->h.c_cap = c->q_cap;
     c->h.p_stage = c->p_stage;
     c->h.p_meta = c->p_meta;
     c->h.p_cnt = c->p_cnt;
@@ -1841,6 +1845,7 @@ int ipm_create(ipm_ctx **out, int nz, int nr, double dx, double dt, int ns, int 
     h.nz = nz; h.nr = nr; h.ns = ns; h.ng = ng; h.dx = dx; h.dt = dt;
     h.vfactor = dx / dt;               // ParticleBundle.f90:346
     h.inv_vfactor = 1.0 / h.vfactor;   // MCCPublic.F90:90 multiplies by 1/VFactor
+    h.null_screen = 1;
     const size_t nn = (size_t)nz * nr;
     HIPCHK(hipMalloc(&c->d_state, sizeof(DevState)));
     HIPCHK(hipMalloc(&c->d_ctr, sizeof(DevCounters)));
@@ -1895,7 +1900,7 @@ int ipm_destroy(ipm_ctx *c)
     void *ptrs[] = {c->d_state, c->d_ctr, c->raw_all, c->rho_one, c->chi_one, c->rho_tot, c->chi_tot,
                     c->d_chifac, c->inv_vol, c->cell_material, c->hedge, c->vedge, c->mat_type,
                     c->metal_charge, c->sigma, c->Ez, c->Er, c->mcc_counts, c->hist, c->offs,
-                    c->cursor, c->partial, c->partial2, c->qbuf, c->cbuf, c->q_tile_count, c->tanc, c->p_stage,
+                    c->cursor, c->partial, c->partial2, c->qbuf, c->q_tile_count, c->tanc, c->p_stage,
                     c->p_meta, c->p_cnt,
                     c->mom, c->phi, c->eps_cell, c->ptype, c->sol, c->rezone};
     for (void *p : ptrs)
@@ -2372,6 +2377,19 @@ int ipm_set_mcc(ipm_ctx *c, int s, int g, int model, int nreaction, int nsigma, 
                 if (!(v >= 0.0) || (j > 0 && v < prob[(size_t)i * nreaction + j - 1])) { cum = false; break; }
             }
         t.cumulative = cum ? 1 : 0;
+        if (model == 1 || model == 2) {
+            const std::vector<double> nb = null_bound_table(nreaction, nsigma, prob);
+            if (!nb.empty()) {
+                double *d_nb;
+                HIPCHK(hipMalloc(&d_nb, sizeof(double) * nb.size()));
+                c->tab_mem[s * c->ng + g].push_back(d_nb);
+                HIPCHK(hipMemcpy(d_nb, nb.data(), sizeof(double) * nb.size(), hipMemcpyHostToDevice));
+                t.nb_tab = d_nb;
+                t.nb_eps = 0.01 + 4e-6 / einterval;
+                t.nb_inv_einterval = 1.0 / einterval;
+                t.nb_inv_emax = 1.0 / emax;
+            }
+        }
     }
     table_consts(c);
     int rc = ensure_queue(c);
@@ -2406,6 +2424,13 @@ int ipm_set_event_counting(ipm_ctx *c, int on)
 {
     if (!c) return fail(IPM_EINVAL, "null context");
     c->h.count_events = on ? 1 : 0;
+    return sync_state(c);
+}
+
+int ipm_set_null_screen(ipm_ctx *c, int on)
+{
+    if (!c) return fail(IPM_EINVAL, "null context");
+    c->h.null_screen = on ? 1 : 0;
     return sync_state(c);
 }
 

@@ -130,6 +130,11 @@ module ipm_hip
             type(c_ptr), value :: ctx
             integer(c_int), value :: on
         end function
+        integer(c_int) function ipm_set_null_screen(ctx, on) bind(C, name="ipm_set_null_screen")
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: on
+        end function
         integer(c_int) function ipm_take_mcc_counts(ctx, counts) bind(C, name="ipm_take_mcc_counts")
             import :: c_ptr, c_int, c_int64_t
             type(c_ptr), value :: ctx

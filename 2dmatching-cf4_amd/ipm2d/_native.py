@@ -61,6 +61,7 @@ SIGNATURES = {
     "ipm_set_rng": [_vp, C.c_uint64, C.c_uint32],
     "ipm_mcc": [_vp],
     "ipm_set_event_counting": [_vp, C.c_int],
+    "ipm_set_null_screen": [_vp, C.c_int],
     "ipm_take_mcc_counts": [_vp, _i64p],
     "ipm_step": [_vp],
     "ipm_step_graph": [_vp],

@@ -138,6 +138,10 @@ class HotPath:
     def set_event_counting(self, on=True):
         N.call("ipm_set_event_counting", self._h, 1 if on else 0)
 
+    def set_null_screen(self, on=True):
+        """Null-event screen of the collision pass (on by default; exact)."""
+        N.call("ipm_set_null_screen", self._h, 1 if on else 0)
+
     def set_launch(self, blocks_per_cu=0, block=None):
         """Grid of the fused pass (0: one block per tile; else persistent
         blocks per CU) and, if given, its threads per block (128/256/512)."""

@@ -125,6 +125,14 @@ int ipm_mcc(ipm_ctx *ctx);
 /* Enables (on != 0) or disables the per-event counters below (off by default:
  * counting adds one atomic per collision). */
 int ipm_set_event_counting(ipm_ctx *ctx, int on);
+/* Null-event screen (on by default; no counterpart in the reference, whose
+ * MCCPublic.F90:61-97 runs Updater and SelectProbility for every selected
+ * particle): a selected particle whose reaction-choice uniform is at or above
+ * every probability its table can reach over the relative energies the
+ * target sampler allows is a null event, known without sampling the target,
+ * and is never queued for the collision pass.  Exact: results are identical
+ * with the screen on or off (off: every selection takes the full path). */
+int ipm_set_null_screen(ipm_ctx *ctx, int on);
 /* Per-(species, gas, reaction index) event counts since the last call
  * (counts[(s*ng + g)*128 + j], j = 0 for a null event); cleared after read. */
 int ipm_take_mcc_counts(ipm_ctx *ctx, int64_t *counts);

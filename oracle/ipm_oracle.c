@@ -857,8 +857,9 @@ static void post_collision(mcpo_t *m, int vahedi, rng_t *rg)
     }
 }
 
-/* SelectProbility, TypeMCC.f90:173-216 */
-static int select_probability(const mcpo_t *m, const ora_mcc_table *t, rng_t *rg)
+/* SelectProbility, TypeMCC.f90:173-216, with its uniform R passed in: the
+ * stream's draw 1, drawn before Updater's target sampling (draws 2 on). */
+static int select_probability(const mcpo_t *m, const ora_mcc_table *t, double R)
 {
     double tmp[256];
     const int Nr = t->nreaction, Ns = t->nsigma;
@@ -878,20 +879,19 @@ static int select_probability(const mcpo_t *m, const ora_mcc_table *t, rng_t *rg
     }
     for (int j = 0; j < Nr; ++j)
         if (E < t->threshold[j]) tmp[j] = 0.0;
-    double R = rnext(rg);
     for (int j = 0; j < Nr; ++j)
         if (R < tmp[j]) return j + 1;
     return 0;
 }
 
-/* SelectProbilityReactive, TypeMCC.f90:218-236 */
-static int select_probability_reactive(mcpo_t *m, const ora_mcc_table *t, rng_t *rg)
+/* SelectProbilityReactive, TypeMCC.f90:218-236: R (beta's draw) is the
+ * stream's draw 1 as above; the non-reactive choice takes the next draw. */
+static int select_probability_reactive(mcpo_t *m, const ora_mcc_table *t, double R, rng_t *rg)
 {
-    double R = rnext(rg);
     m->beta = t->beta_max * sqrt(R);
     if (m->beta <= 1.0) {
         if (m->energy <= t->threshold[1]) return 1;
-        return select_probability(m, t, rg);
+        return select_probability(m, t, rnext(rg));
     }
     return 1;
 }
@@ -1055,10 +1055,12 @@ int ora_mcc(const ora_mcc_setup *st, uint64_t seed, uint32_t step,
                 m.poi.Z = A[0][i]; m.poi.R = A[1][i]; m.poi.Vz = A[2][i]; m.poi.Vr = A[3][i];
                 m.poi.Vt = A[4][i]; m.poi.Az = A[5][i]; m.poi.Ar = A[6][i]; m.poi.WQ = A[7][i];
                 m.poi.Vz = m.poi.Vz * VFactor; m.poi.Vr = m.poi.Vr * VFactor; m.poi.Vt = m.poi.Vt * VFactor;
+                /* the reaction choice's uniform is draw 1, the target's draws follow */
+                const double u = rnext(&rg);
                 mc_update(&m, st->mass[s], t->gas_mass, t->gas_temperature, &rg);
                 int idx = 0;
-                if (t->model == 1 || t->model == 2) idx = select_probability(&m, t, &rg);
-                else if (t->model == 3) idx = select_probability_reactive(&m, t, &rg);
+                if (t->model == 1 || t->model == 2) idx = select_probability(&m, t, u);
+                else if (t->model == 3) idx = select_probability_reactive(&m, t, u, &rg);
                 if (idx > 0) {
                     int ty = t->rtype[idx - 1];
                     if (t->model == 1)

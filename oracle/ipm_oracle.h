@@ -124,11 +124,13 @@ typedef struct {
 
 /* MCC, R/src/mc/MCCPublic.F90:38-226 and callees.  Draws come from
  * ora_uniform(seed, step, species|gas<<8, key(state at MCC start), k), k
- * counting draws per particle and gas.
+ * counting draws per particle and gas: k = 0 the selection, k = 1 the
+ * uniform of the reaction choice (SelectProbility's R, or the Langevin
+ * beta's), k >= 2 Updater's target sampling and then the collision's own.
  * Species arrays P[s][0..7] have capacity cap[s]; n[s] updated with products
- * appended at the end (in reference order).  counts[s*ng*64 + g*64 + type%64]
- * (if non-NULL) tallies collision events per species, gas and reaction type.
- * Returns 0, or -1 on capacity overflow. */
+ * appended at the end (in reference order).  counts[(s*ng + g)*128 + j]
+ * (if non-NULL) tallies events per species, gas and reaction index j
+ * (j = 0: null event).  Returns 0, or -1 on capacity overflow. */
 int ora_mcc(const ora_mcc_setup *st, uint64_t seed, uint32_t step,
             int64_t *n, const int64_t *cap, double *(*P)[8], int64_t *counts);
 
