@@ -54,6 +54,7 @@ struct ipm_ctx {
     double *rho_tot = nullptr, *chi_tot = nullptr;
     double *inv_vol = nullptr;
     int *cell_material = nullptr, *hedge = nullptr, *vedge = nullptr, *mat_type = nullptr;
+    int2 *row_vac = nullptr;               // nr + 2 vacuum runs of cell rows (DevState::row_vac)
     double *metal_charge = nullptr, *sigma = nullptr;
     double *Ez = nullptr, *Er = nullptr;
     unsigned long long *mcc_counts = nullptr;

@@ -63,6 +63,11 @@ struct DevState {
     double *raw[IPM_NS_MAX];           // raw per-species deposit (nz*nr)
     const double *Ez, *Er;
     const int *cell_material, *hedge, *vedge, *mat_type;
+    // Per cell row j (0-based, nr + 2 entries, the last three empty): a run [x, y) of
+    // 0-based z cell indices whose cells are neither metal nor dielectric,
+    // so absorption needs no material lookup there (formed on the host from
+    // cell_material and mat_type).
+    const int2 *row_vac;
     int ndiel, nmetal;
     double *metal_charge;              // nmetal
     double *sigma;                     // nz*nr
@@ -300,9 +305,13 @@ __device__ __forceinline__ void push_core(const FS &fs, int nz, int nr, double E
     const Gather g = gather_at(nz, nr, z, r);
     const size_t kr = g.kr, kz0 = g.kz0, kz1 = g.kz1, ldr = g.ldr;
     const double Z1 = g.Z1, Z2 = g.Z2, Z1H = g.Z1H, Z2H = g.Z2H, R1 = g.R1, R2 = g.R2, R1H = g.R1H, R2H = g.R2H;
-    double E1 = fs.er(kr) * Z1 * R1H + fs.er(kr + 1) * Z2 * R1H + fs.er(kr + ldr) * Z1 * R2H +
-                fs.er(kr + ldr + 1) * Z2 * R2H;
-    double E2 = fs.ez(kz0) * Z1H * R1 + fs.ez(kz0 + 1) * Z2H * R1 + fs.ez(kz1) * Z1H * R2 + fs.ez(kz1 + 1) * Z2H * R2;
+    // the eight field values are read before any is used, so that their
+    // loads are in flight together (one wait for all of them)
+    const double er0 = fs.er(kr), er1 = fs.er(kr + 1), er2 = fs.er(kr + ldr), er3 = fs.er(kr + ldr + 1);
+    const double ez0 = fs.ez(kz0), ez1 = fs.ez(kz0 + 1), ez2 = fs.ez(kz1), ez3 = fs.ez(kz1 + 1);
+    __builtin_amdgcn_sched_barrier(0);
+    double E1 = er0 * Z1 * R1H + er1 * Z2 * R1H + er2 * Z1 * R2H + er3 * Z2 * R2H;
+    double E2 = ez0 * Z1H * R1 + ez1 * Z2H * R1 + ez2 * Z1H * R2 + ez3 * Z2H * R2;
     E1 = E1 * EFactor;
     E2 = E2 * EFactor;
 
@@ -342,27 +351,26 @@ __device__ __forceinline__ void push_one(const DevState &st, double EFactor, Par
 }
 
 // ---------------------------------------------------------- absorption
-// MaterialAborptionParticle for one particle,
-// R/src/pic/particleboundary/ParticleMaterial.F90:22-85.
-// Returns true if the particle is deleted.  Reflection edits p.R/Vr/Ar.
-__device__ __forceinline__ bool absorb_one(const DevState &st, double charge, Part &p)
+// MaterialAborptionParticle, R/src/pic/particleboundary/ParticleMaterial.F90:22-85,
+// for a particle inside the domain (0 < Z < Nz-1, 0 < R < Nr-1) in cell
+// (NzL, NrL): metal takes its charge, a dielectric cell puts it on the edge
+// the reference chooses; either deletes the particle (returns true).
+// metal_acc (LDS, kMetalLds entries, or null): per-block sums of the charge
+// of metals 1..kMetalLds, added to metal_charge once per block by the
+// caller (every absorbed particle of a metal would otherwise add to one
+// address, and atomics on one address serialize).
+constexpr int kMetalLds = 16;
+__device__ __forceinline__ bool absorb_material(const DevState &st, double charge, const Part &p, int NzL, int NrL,
+                                                double *metal_acc = nullptr)
 {
-    const int nz = st.nz, nr = st.nr;
-    if (p.Z <= 0.0 || p.Z >= (double)(nz - 1) || p.R >= (double)(nr - 1)) return true;
-    if (p.R <= 0.0) {
-        p.R = fabs(p.R);
-        p.Vr = fabs(p.Vr);
-        p.Ar = fabs(p.Ar);
-        return false;
-    }
-    // 0 < Z < nz-1 and 0 < R < nr-1 here, so the clamps only matter for a
-    // non-finite position, which must not read outside the geometry arrays.
-    int NzL = min(max((int)ceil(p.Z), 1), nz - 1);
-    int NrL = min(max((int)ceil(p.R), 1), nr - 1);
+    const int nz = st.nz;
     int m = gp(st.cell_material)[(size_t)(NzL - 1) + (size_t)(NrL - 1) * (nz - 1)];
     int t = gp(st.mat_type)[m + st.ndiel];
     if (t == kMatMetal) {
-        global_add(&st.metal_charge[m - 1], p.WQ * charge);
+        if (metal_acc && m <= kMetalLds)
+            __hip_atomic_fetch_add(&metal_acc[m - 1], p.WQ * charge, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else
+            global_add(&st.metal_charge[m - 1], p.WQ * charge);
         return true;
     }
     if (t == kMatDiel) {
@@ -393,6 +401,43 @@ __device__ __forceinline__ bool absorb_one(const DevState &st, double charge, Pa
         return true;
     }
     return false;
+}
+
+// Vacuum runs of cell rows r0 .. r0 + 3 as row_vac holds them (n = 0: none).
+struct RowRuns {
+    int r0 = 0, n = 0;
+    int2 v0{0, 0}, v1{0, 0}, v2{0, 0}, v3{0, 0};
+};
+
+// MaterialAborptionParticle for one particle,
+// R/src/pic/particleboundary/ParticleMaterial.F90:10-90: deleted outside
+// the domain, reflected at the axis, else the material rules of the cell.
+// Returns true if the particle is deleted; reflection edits p.R/Vr/Ar.  A
+// particle whose cell lies in one of the vacuum runs `rr` is known to be in
+// vacuum without the two dependent material loads.  Only the material
+// lookup is a branch; the rest is selects.  metal_acc: as for absorb_material.
+__device__ __forceinline__ bool absorb_one(const DevState &st, double charge, Part &p, const RowRuns rr = RowRuns{},
+                                           double *metal_acc = nullptr)
+{
+    const int nz = st.nz, nr = st.nr;
+    const bool out = p.Z <= 0.0 || p.Z >= (double)(nz - 1) || p.R >= (double)(nr - 1);
+    const bool refl = !out && p.R <= 0.0;
+    p.R = refl ? fabs(p.R) : p.R;
+    p.Vr = refl ? fabs(p.Vr) : p.Vr;
+    p.Ar = refl ? fabs(p.Ar) : p.Ar;
+    // inside the domain the clamps only matter for a non-finite position,
+    // which must not read outside the geometry arrays
+    const int NzL = min(max((int)ceil(p.Z), 1), nz - 1);
+    const int NrL = min(max((int)ceil(p.R), 1), nr - 1);
+    const int jr = NrL - 1 - rr.r0;
+    // run of row r0 + jr, picked by the bits of jr (two levels of selects)
+    const bool odd = (jr & 1) != 0, high = (jr & 2) != 0;
+    const int lo = high ? (odd ? rr.v3.x : rr.v2.x) : (odd ? rr.v1.x : rr.v0.x);
+    const int hi = high ? (odd ? rr.v3.y : rr.v2.y) : (odd ? rr.v1.y : rr.v0.y);
+    const bool vac = jr >= 0 && jr < rr.n && NzL - 1 >= lo && NzL - 1 < hi;
+    bool del = out;
+    if (!out && !refl && !vac) del = absorb_material(st, charge, p, NzL, NrL, metal_acc);
+    return del;
 }
 
 // ---------------------------------------------------------------- MCC

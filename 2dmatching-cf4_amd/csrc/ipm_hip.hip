@@ -306,16 +306,24 @@ __device__ __forceinline__ bool wave_segmented_sum(int key, double (&v)[4])
 }
 
 // ------------------------------------------------------- particle kernel
-// Persistent launch over every (species, tile) pair.  Tiles are contiguous
-// slot ranges; the tile range is split into 8 contiguous parts, one per XCD
-// (blocks are dispatched to XCDs round-robin), so neighbouring tiles -- which
-// after a cell sort touch neighbouring grid nodes -- share one L2.
+// Launch over every (species, tile) pair.  Tiles are contiguous slot
+// ranges taken in slot order, block b first taking tile b (blocks are
+// dispatched to the XCDs round-robin): the whole device then streams through
+// one narrow stretch of every array at a time.  Cutting the tile range into
+// eight contiguous parts, one per XCD, kept neighbouring tiles on one L2 but
+// streamed eight distant stretches at once and made the pass 5 % slower.
 // BLOCK threads (BLOCK / 64 waves) take a tile of kTile slots, ITEMS
 // consecutive runs of 64 slots per wave; WINR is the deposit window's height.
-// The fused pass is instantiated for several (BLOCK, WINR) pairs, which the
-// autotuner chooses among; every other mode uses (256, 24).
+// Each wave loads the next run's particles while it finishes the current
+// one; the loads are issued after the current run's last dependent read (the
+// field gather and the material lookup), because a wait for any later load
+// would also wait for them.  The register budget is held at five waves per
+// SIMD (four with split/merge folded in, which needs more registers than
+// that budget leaves without spilling).  The fused pass is instantiated for
+// several (BLOCK, WINR) pairs, which the autotuner chooses among; every
+// other mode uses (256, 24).
 template <int MODE, int BLOCK = kBlock, int WINR = kWinR>
-__global__ void __launch_bounds__(BLOCK)
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu((MODE & M_REZONE) ? 4 : 5)))
 particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr, int sel)
 {
     constexpr int ITEMS = kTile / BLOCK;
@@ -323,6 +331,7 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
     const DevState &st = *stp;
     __shared__ double win[kWinZ * WINR];
     __shared__ unsigned long long dead_blk[IPM_NS_MAX];
+    __shared__ double metal_blk[kMetalLds];     // this block's metal charge tallies
     // Entries of the tile in the collision queue so far (two counters so
     // the next tile never waits for the reset), and each wave's staging
     // buffer of up to 64 queue records, written out with full waves.
@@ -337,22 +346,12 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
     if (MODE & M_DEPOSIT)
         for (int q = threadIdx.x; q < kWinZ * WINR; q += BLOCK) win[q] = 0.0;
     if (threadIdx.x < IPM_NS_MAX) dead_blk[threadIdx.x] = 0;
+    if (threadIdx.x < kMetalLds) metal_blk[threadIdx.x] = 0.0;
     if (threadIdx.x < 2) q_scount[threadIdx.x] = 0;
     __syncthreads();
     int qbuf = 0;
 
-    const unsigned G = gridDim.x;
-    long long t0, t1, tstride, tfirst;
-    if (G >= 8) {
-        const unsigned x = blockIdx.x & 7u;
-        const unsigned nb = (G - x + 7u) / 8u;        // blocks living on XCD x
-        t0 = T * x / 8;
-        t1 = T * (x + 1) / 8;
-        tfirst = t0 + blockIdx.x / 8u;
-        tstride = nb;
-    } else {
-        t0 = 0; t1 = T; tfirst = blockIdx.x; tstride = G;
-    }
+    const long long t1 = T, tfirst = blockIdx.x, tstride = gridDim.x;
 
     const uint32_t step = ctr->step;
     const unsigned long long seed = ctr->seed;
@@ -451,18 +450,41 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
             }
             cw = 0;
         };
+        Part pn;   // the wave's next run of particles, loaded ahead
+        {
+            const long long i0 = base + (long long)(wv * ITEMS) * 64 + lane;
+            if (i0 < n_s) load_item(i0, pn);
+        }
 #pragma unroll 1
         for (int k = 0; k < ITEMS; ++k) {
             const long long i = base + (long long)(wv * ITEMS + k) * 64 + lane;
             const bool valid = i < n_s;
-            Part p;
-            if (valid) load_item(i, p);
+            Part p = pn;
             bool live = false, deleted = false;
             if (valid) live = !is_dead(p.Z);
             if (live) {
+                // Vacuum runs of four cell rows around the first live lane's
+                // row before the push, read as scalars while the push's
+                // gather is in flight (scalar loads do not wait on vector
+                // loads); a particle moves less than a cell per step.
+                RowRuns rr;
+                if (MODE & M_ABSORB) {
+                    const int row = min(max((int)ceil(p.R), 1), st.nr - 1) - 1;
+                    rr.r0 = max(__builtin_amdgcn_readfirstlane(row) - 1, 0);
+                    rr.n = 4;
+                    const auto *rv = (const __attribute__((address_space(4))) int *)st.row_vac + 2 * rr.r0;
+                    rr.v0 = int2{rv[0], rv[1]};
+                    rr.v1 = int2{rv[2], rv[3]};
+                    rr.v2 = int2{rv[4], rv[5]};
+                    rr.v3 = int2{rv[6], rv[7]};
+                }
                 if (MODE & M_PUSH) push_one(st, efactor, p);
-                if (MODE & M_ABSORB) deleted = absorb_one(st, charge, p);
+                if (MODE & M_ABSORB) deleted = absorb_one(st, charge, p, rr, metal_blk);
             }
+            // every dependent read of this run is done: load the next one
+            __builtin_amdgcn_sched_barrier(0);
+            if (k + 1 < ITEMS && i + 64 < n_s) load_item(i + 64, pn);
+            __builtin_amdgcn_sched_barrier(0);
             const bool alive = live && !deleted;
             if ((MODE & M_DEPOSIT) && threadIdx.x == 0 && k == 0 && alive && p.Z > 0.0 && p.R >= 0.0 &&
                 p.Z < 1e9 && p.R < 1e9) {
@@ -741,10 +763,12 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
 
     if (MODE & M_ABSORB) {
         // The last tile's barrier (deposit flush or queue append) already
-        // ordered every wave's dead_blk update before this read.
+        // ordered every wave's dead_blk and metal_blk updates before these reads.
         if (!(MODE & (M_DEPOSIT | M_MCC))) __syncthreads();
         if (threadIdx.x < (unsigned)ns && dead_blk[threadIdx.x])
             atomicAdd(&ctr->n_dead[threadIdx.x], dead_blk[threadIdx.x]);
+        if (threadIdx.x < (unsigned)min(st.nmetal, kMetalLds) && metal_blk[threadIdx.x] != 0.0)
+            global_add(&st.metal_charge[threadIdx.x], metal_blk[threadIdx.x]);
     }
 
 }
@@ -1898,7 +1922,7 @@ int ipm_destroy(ipm_ctx *c)
     for (auto &v : c->tab_mem)
         for (void *p : v) (void)hipFree(p);
     void *ptrs[] = {c->d_state, c->d_ctr, c->raw_all, c->rho_one, c->chi_one, c->rho_tot, c->chi_tot,
-                    c->d_chifac, c->inv_vol, c->cell_material, c->hedge, c->vedge, c->mat_type,
+                    c->d_chifac, c->inv_vol, c->cell_material, c->hedge, c->vedge, c->mat_type, c->row_vac,
                     c->metal_charge, c->sigma, c->Ez, c->Er, c->mcc_counts, c->hist, c->offs,
                     c->cursor, c->partial, c->partial2, c->qbuf, c->q_tile_count, c->tanc, c->p_stage,
                     c->p_meta, c->p_cnt,
@@ -1957,7 +1981,7 @@ int ipm_set_geometry(ipm_ctx *c, const double *inv_volume, const int *cell_mater
         if (cell_material[k] < -ndiel || cell_material[k] > nmetal)
             return fail(IPM_EINVAL, "cell_material[%zu]=%d outside [-%d, %d]", k, cell_material[k], ndiel, nmetal);
     HIPCHK(hipStreamSynchronize(c->stream));
-    void *old[] = {c->inv_vol, c->cell_material, c->hedge, c->vedge, c->mat_type, c->metal_charge};
+    void *old[] = {c->inv_vol, c->cell_material, c->hedge, c->vedge, c->mat_type, c->row_vac, c->metal_charge};
     for (void *p : old)
         if (p) HIPCHK(hipFree(p));
     const size_t nn = (size_t)nz * nr;
@@ -1973,6 +1997,28 @@ int ipm_set_geometry(ipm_ctx *c, const double *inv_volume, const int *cell_mater
     HIPCHK(hipMemcpy(c->mat_type, material_type, sizeof(int) * (ndiel + nmetal + 1), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&c->metal_charge, sizeof(double) * (nmetal + 1)));
     HIPCHK(hipMemset(c->metal_charge, 0, sizeof(double) * (nmetal + 1)));
+    // Per cell row, the longest run of cells that are neither metal nor
+    // dielectric (absorb_one's fast path); the entries of rows nr - 1 .. nr + 1
+    // stay empty (the particle pass reads four rows from any row below nr - 1).
+    std::vector<int2> rv((size_t)nr + 2, int2{0, 0});
+    for (int j = 0; j < nr - 1; ++j) {
+        int best0 = 0, best1 = 0, run0 = 0;
+        for (int i = 0; i <= nz - 1; ++i) {
+            bool wall = true;
+            if (i < nz - 1) {
+                const int t = material_type[cell_material[(size_t)i + (size_t)j * (nz - 1)] + ndiel];
+                wall = t == kMatMetal || t == kMatDiel;
+            }
+            if (wall) {
+                if (i - run0 > best1 - best0) { best0 = run0; best1 = i; }
+                run0 = i + 1;
+            }
+        }
+        rv[(size_t)j] = int2{best0, best1};
+    }
+    HIPCHK(hipMalloc(&c->row_vac, sizeof(int2) * rv.size()));
+    HIPCHK(hipMemcpy(c->row_vac, rv.data(), sizeof(int2) * rv.size(), hipMemcpyHostToDevice));
+    c->h.row_vac = c->row_vac;
     c->h.cell_material = c->cell_material;
     c->h.hedge = c->hedge;
     c->h.vedge = c->vedge;

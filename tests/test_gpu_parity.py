@@ -105,6 +105,54 @@ def test_absorption_matches_oracle(oracle, rng):
     assert not mc2.any() and not sg2.any()
 
 
+def test_absorption_vacuum_runs_after_push(oracle, rng):
+    """The particle pass skips the material lookup for a cell inside its
+    row's vacuum run (the longest run of cells that are neither metal nor
+    dielectric, read for the four rows around each wave's first particle).
+    Cell-sorted particles, so whole waves take that path, over a geometry
+    whose rows hold two or three vacuum runs (a dielectric block and a metal
+    block inside the domain, the dielectric ring, both electrodes): the
+    survivors after push + absorption and every tally equal the oracle's."""
+    nz, nr = 65, 33
+    cm = np.zeros((nz - 1, nr - 1), dtype=np.int32)
+    cm[:, nr - 4:] = -1
+    cm[20:26, 6:12] = -1          # dielectric block: rows 6-11 hold two vacuum runs
+    cm[40:43, 9:20] = 2           # metal block: rows 9-11 hold three
+    cm[0, :] = 1
+    cm[nz - 2, :] = 2
+    from ipm2d import geometry as geo
+    geom = geo.build_geometry(nz, nr, 1e-3, cm, nmetal=2, ndiel=1, permittivity={-1: 2.1})
+    hp = make_hp(geom)
+    ez, er = random_fields(nz, nr, rng, scale=3e3)
+    hp.set_fields(ez, er)
+    P = []
+    for s, n in ((0, 60000), (1, 20000)):
+        X = random_particles(nz, nr, n, rng, vscale=0.3)
+        X[0] = rng.uniform(0.2, nz - 1.2, n)
+        X[1] = rng.uniform(0.0, nr - 1.5, n)
+        key = (np.ceil(X[0]) - 1) + (np.ceil(X[1]) - 1) * (nz - 1)
+        P.append(np.ascontiguousarray(X[:, np.argsort(key, kind="stable")]))
+    for s in range(2):
+        hp.upload(s, P[s])
+    hp.move_absorb(-1)
+    mc, sg = hp.take_tallies()
+    mc_ref = np.zeros(geom.nmetal)
+    sg_ref = np.zeros((nz, nr))
+    for s in range(2):
+        q, m = SPECIES[s]
+        X = oracle.push(nz, nr, DX, DT, q, m, ez, er, P[s])
+        ref, m1, s1 = oracle.absorb(geom, q, X)
+        mc_ref += m1
+        sg_ref += s1
+        got = hp.download(s)
+        assert got.shape == ref.shape, f"species {s}: survivors {got.shape} vs {ref.shape}"
+        assert_close(sort_set(got), sort_set(ref), rtol=1e-12, atol=1e-300, what=f"species {s}")
+    assert (np.abs(sg_ref) > 0).sum() > 20, "dielectric edges were not exercised"
+    assert (mc_ref != 0).all(), "both metals take charge"
+    assert_close(mc, mc_ref, rtol=1e-12, what="metal charge")
+    assert_close(sg, sg_ref, rtol=1e-12, atol=1e-12 * np.abs(sg_ref).max(), what="sigma")
+
+
 def test_deposit_matches_oracle(oracle, rng):
     geom = small_geometry()
     nz, nr = geom.nz, geom.nr
