@@ -110,6 +110,7 @@ struct DevCounters {
     unsigned long long n[IPM_NS_MAX];      // slots in use at step start
     unsigned long long n_new[IPM_NS_MAX];  // products appended during the pass
     unsigned long long n_dead[IPM_NS_MAX]; // dead slots among n
+    unsigned long long q_total[IPM_NS_MAX]; // collision-queue entries of the pass (cleared by the append)
     unsigned long long seed;
     unsigned int step;
     unsigned int blocks_done;

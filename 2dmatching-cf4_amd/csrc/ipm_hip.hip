@@ -754,7 +754,9 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
             if (threadIdx.x == 0) {
                 const unsigned n = q_scount[qbuf];
                 const long long R = st.q_tile_cap[s];
-                st.q_tile_count[st.q_cbase[s] + (t - tbeg)] = (long long)n < R ? n : (unsigned)R;
+                const unsigned nq = (long long)n < R ? n : (unsigned)R;
+                st.q_tile_count[st.q_cbase[s] + (t - tbeg)] = nq;
+                if (nq) atomicAdd(&ctr->q_total[s], (unsigned long long)nq);
                 q_scount[qbuf] = 0;     // reused two tiles later, after a barrier
             }
             qbuf ^= 1;
@@ -814,7 +816,11 @@ collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
     const ProdSink sink = {prec, pmeta, &pcount, pspc};
     if (threadIdx.x < IPM_NS_MAX) pspc[threadIdx.x] = 0;
 
-    // species of this block and its tile range
+    // species of this block and its tile range: one block for every species
+    // with MCC tables and live tiles, the others in proportion to the
+    // entries the particle pass queued for each species (after the null
+    // screen the electrons keep a much smaller share of their selections
+    // than ions, whose selections are nearly all real)
     int sb = -1;
     long long t0 = 0, t1 = 0;
     {
@@ -822,7 +828,7 @@ collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
         int A = 0;
         for (int s = 0; s < ns; ++s) {
             const long long nt = ((long long)ctr->n[s] + kTile - 1) / kTile;
-            if (nt > 0 && st.q_weight[s] > 0.0) { W += (double)nt * st.q_weight[s]; ++A; }
+            if (nt > 0 && st.q_weight[s] > 0.0) { W += (double)ctr->q_total[s] + 1.0; ++A; }
         }
         // every species with work needs a block of its own: a grid smaller
         // than that would leave queued selections uncollided
@@ -834,7 +840,7 @@ collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
             for (int s = 0; s < ns; ++s) {
                 const long long nt = ((long long)ctr->n[s] + kTile - 1) / kTile;
                 if (!(nt > 0 && st.q_weight[s] > 0.0)) continue;
-                const double Wle = Wlt + (double)nt * st.q_weight[s];
+                const double Wle = Wlt + (double)ctr->q_total[s] + 1.0;
                 const long long nb = 1 + (long long)floor(Grem * Wle / W) - (long long)floor(Grem * Wlt / W);
                 if ((long long)blockIdx.x >= B && (long long)blockIdx.x < B + nb) {
                     const long long lb = (long long)blockIdx.x - B;
@@ -975,6 +981,8 @@ collide_append_kernel(const DevState *__restrict__ stp, DevCounters *ctr)
     __shared__ unsigned long long part[kBlock / 64][IPM_NS_MAX];
     const int ns = st.ns;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // the collision pass has read the pass's queue totals: clear them for the next
+    if (blockIdx.x == 0 && threadIdx.x < IPM_NS_MAX) ctr->q_total[threadIdx.x] = 0;
     unsigned long long acc[IPM_NS_MAX];
     for (int d = 0; d < IPM_NS_MAX; ++d) acc[d] = 0;
     for (unsigned b = threadIdx.x; b < blockIdx.x; b += kBlock)
@@ -2054,6 +2062,7 @@ int ipm_upload_species(ipm_ctx *c, int s, int64_t n, const double *po, int layou
     HIPCHK(hipMemcpy(&ctr, c->d_ctr, sizeof ctr, hipMemcpyDeviceToHost));
     ctr.n[s] = (unsigned long long)n; ctr.n_new[s] = 0; ctr.n_dead[s] = 0;
     ctr.overflow = 0;     // a fresh population: earlier overflow reports are void
+    for (int k = 0; k < IPM_NS_MAX; ++k) ctr.q_total[k] = 0;
     HIPCHK(hipMemcpy(c->d_ctr, &ctr, sizeof ctr, hipMemcpyHostToDevice));
     return IPM_OK;
 }
@@ -2076,6 +2085,7 @@ int ipm_upload_species_device(ipm_ctx *c, int s, int64_t offset, int64_t n, cons
     HIPCHK(hipMemcpy(&ctr, c->d_ctr, sizeof ctr, hipMemcpyDeviceToHost));
     ctr.n[s] = (unsigned long long)(offset + n); ctr.n_new[s] = 0; ctr.n_dead[s] = 0;
     ctr.overflow = 0;
+    for (int k = 0; k < IPM_NS_MAX; ++k) ctr.q_total[k] = 0;
     HIPCHK(hipMemcpy(c->d_ctr, &ctr, sizeof ctr, hipMemcpyHostToDevice));
     return IPM_OK;
 }
@@ -2452,6 +2462,7 @@ int ipm_set_rng(ipm_ctx *c, uint64_t seed, uint32_t step)
     ctr.seed = seed;
     ctr.step = step;
     ctr.overflow = 0;     // a restart of the stream clears earlier overflow reports
+    for (int k = 0; k < IPM_NS_MAX; ++k) ctr.q_total[k] = 0;
     HIPCHK(hipMemcpy(c->d_ctr, &ctr, sizeof ctr, hipMemcpyHostToDevice));
     return IPM_OK;
 }
