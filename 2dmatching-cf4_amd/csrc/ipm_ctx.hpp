@@ -115,9 +115,20 @@ inline int read_counters(ipm_ctx *c, DevCounters *out)
 // Species s's current buffer set becomes set `cur` of its two; f[] and alt[]
 // of the host copy of DevState follow.  The device copy is updated by
 // sync_state or, after a cell sort, by the sort's own swap kernel.
+// Elements per field array of a species of the given capacity: a multiple of
+// 8192 (64 KiB-aligned arrays: of the strides tried, 256 B-, 4 KiB- and
+// 64 KiB-aligned, the particle pass streams its fifteen arrays fastest at
+// this one) with at least one slot past the capacity, which the cell sort's
+// scatter writes for sorted positions that hold no particle (so its stores
+// need no branch).
+inline size_t field_stride(long long capacity)
+{
+    return ((size_t)capacity + 8192) & ~(size_t)8191;
+}
+
 inline void set_buffers(ipm_ctx *c, int s, int cur)
 {
-    const size_t cap = ((size_t)c->h.cap[s] + 31) & ~(size_t)31;
+    const size_t cap = field_stride(c->h.cap[s]);
     c->cur[s] = cur;
     for (int f = 0; f < 8; ++f) {
         c->h.f[s][f] = c->buf[cur][s] + (size_t)f * cap;

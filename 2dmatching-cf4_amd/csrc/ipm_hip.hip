@@ -322,8 +322,17 @@ __device__ __forceinline__ bool wave_segmented_sum(int key, double (&v)[4])
 // that budget leaves without spilling).  The fused pass is instantiated for
 // several (BLOCK, WINR) pairs, which the autotuner chooses among; every
 // other mode uses (256, 24).
+// Waves per SIMD that the particle pass's LDS (deposit window and queue
+// staging) leaves room for, at most the register budget's five.
+constexpr int pk_waves(int block, int winr)
+{
+    const int bytes = kWinZ * winr * 8 + block * kRecD * 8 + 512;
+    const int w = (160 * 1024 / bytes) * (block / 64) / 4;
+    return w < 5 ? w : 5;
+}
 template <int MODE, int BLOCK = kBlock, int WINR = kWinR>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu((MODE & M_REZONE) ? 4 : 5)))
+__global__ void __launch_bounds__(BLOCK)
+__attribute__((amdgpu_waves_per_eu((MODE & M_REZONE) ? 4 : pk_waves(BLOCK, WINR))))
 particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr, int sel)
 {
     constexpr int ITEMS = kTile / BLOCK;
@@ -1401,32 +1410,54 @@ struct SortTileLds {
 // population return at once, so a captured launch fits any population.
 template <bool SCATTER>
 __global__ void __launch_bounds__(kBlock)
-sort_tile_kernel(const DevState *__restrict__ stp, const DevCounters *__restrict__ ctr, int s, int end_bit,
+sort_tile_kernel(const DevState *__restrict__ stp, const DevCounters *__restrict__ ctr, int s,
                  unsigned long long *__restrict__ counts, const unsigned long long *__restrict__ offs)
 {
     __shared__ SortTileLds L;
     __shared__ double stage[SCATTER ? kSortTile : 1];   // one field of the tile (scatter pass)
+    __shared__ unsigned kmin_s, kmax_s;
     const DevState &st = *stp;
     const long long n = (long long)ctr->n[s];
     const long long tbase = (long long)blockIdx.x * kSortTile;
     if (tbase >= n) return;
     const unsigned S = (unsigned)((st.nz - 1) * (st.nr - 1));
     const int t = threadIdx.x;
+    if (t == 0) { kmin_s = S; kmax_s = 0; }
     unsigned key[kSortItems];
     unsigned short loc[kSortItems];
+    unsigned kmin = S, kmax = 0;
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) {
         const int l = t + j * kBlock;
         const long long i = tbase + l;
         unsigned k = S;
         if (i < n) {
-            const double z = st.f[s][0][i];
-            if (!is_dead(z)) k = (unsigned)cell_of(st, z, st.f[s][1][i]);
+            const double z = gp(st.f[s][0])[i];
+            if (!is_dead(z)) k = (unsigned)cell_of(st, z, gp(st.f[s][1])[i]);
         }
         key[j] = k;
         loc[j] = (unsigned short)l;
+        if (k < S) { kmin = min(kmin, k); kmax = max(kmax, k); }
     }
-    TileSort(L.tmp.sort).Sort(key, loc, 0, end_bit);      // blocked: thread t holds 8t .. 8t+7
+    // The tile's keys span a few hundred cells between sorts: sort them
+    // relative to the smallest, on only the bits that span needs (the
+    // sentinel becomes one past the largest), instead of on every bit of
+    // the cell index.
+    __syncthreads();                        // kmin_s / kmax_s initialised
+    atomicMin(&kmin_s, kmin);
+    atomicMax(&kmax_s, kmax);
+    __syncthreads();
+    kmin = kmin_s;
+    kmax = kmax_s;
+    if (kmin > kmax) return;                // no live particle in the tile (uniform)
+    const unsigned span = kmax - kmin + 1;  // relative key of the sentinel
+    int nbits = 0;
+    while (nbits < 32 && (span >> nbits) != 0) ++nbits;
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) key[j] = key[j] < S ? key[j] - kmin : span;
+    TileSort(L.tmp.sort).Sort(key, loc, 0, nbits);        // blocked: thread t holds 8t .. 8t+7
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) key[j] = key[j] < span ? key[j] + kmin : S;
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) L.key[t * kSortItems + j] = key[j];
     __syncthreads();
@@ -1443,7 +1474,8 @@ sort_tile_kernel(const DevState *__restrict__ stp, const DevCounters *__restrict
         const bool run_end = p == kSortTile - 1 || L.key[p + 1] != key[j];
         if (run_end && key[j] < S) {
             const unsigned long long len = (unsigned long long)(p - start[j] + 1);
-            const unsigned long long b = atomicAdd(&counts[key[j]], len);
+            const unsigned long long b =
+                __hip_atomic_fetch_add(gp(&counts[key[j]]), len, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (SCATTER) L.base[start[j]] = (unsigned)b;
         }
     }
@@ -1452,31 +1484,61 @@ sort_tile_kernel(const DevState *__restrict__ stp, const DevCounters *__restrict
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) {
         const int p = t * kSortItems + j;
-        L.key[p] = key[j] < S ? (unsigned)(offs[key[j]] + L.base[start[j]] + (unsigned)(p - start[j])) : 0xffffffffu;
+        L.key[p] = key[j] < S ? (unsigned)(gp(offs)[key[j]] + L.base[start[j]] + (unsigned)(p - start[j])) : 0xffffffffu;
         L.tmp.src[p] = loc[j];
     }
     // Field by field: the tile's values are read coalesced (non-temporal:
     // read once) into LDS, then written in sorted order, neighbouring lanes
     // to neighbouring slots of a run.  Reading them straight from HBM in
-    // sorted order costs a cache line per lane.  Slots past the population
-    // are not loaded; their sorted positions carry no destination.
-    const long long ntile = n - tbase;
-    for (int f = 0; f < 8; ++f) {
-        const double *sf = st.f[s][f] + tbase;
+    // sorted order costs a cache line per lane.  Each thread keeps the
+    // destinations and source slots of its sorted positions in registers,
+    // and loads the next field while the current one is written out (loads
+    // of slots past the population read the tile's first slot instead;
+    // sorted positions without a particle write to the spare slot past the
+    // capacity, so no load or store is conditional and the waits for the
+    // loads need not wait for the stores).  The barriers only order LDS: no
+    // memory operation is waited for there.
+    const int ntile = (int)min((long long)kSortTile, n - tbase);
+    __syncthreads();                        // key/src written
+    unsigned dd[kSortItems];
+    int ss[kSortItems];
+    const unsigned spare = (unsigned)st.cap[s];
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+        const unsigned d = L.key[t + j * kBlock];
+        dd[j] = d != 0xffffffffu ? d : spare;
+        ss[j] = L.tmp.src[t + j * kBlock];
+    }
+    double nx[kSortItems];
+    {
+        const auto *sf = gp(st.f[s][0]) + tbase;
 #pragma unroll
         for (int j = 0; j < kSortItems; ++j) {
             const int l = t + j * kBlock;
-            if (l < ntile) stage[l] = __builtin_nontemporal_load(sf + l);
+            nx[j] = __builtin_nontemporal_load(sf + (l < ntile ? l : 0));
         }
-        __syncthreads();                    // stage (and, for f = 0, key/src) written
-        double *df = st.alt[s][f];
+    }
+    for (int f = 0; f < 8; ++f) {
+        __builtin_amdgcn_s_waitcnt(0xC07F);       // lgkmcnt(0): this thread's LDS reads of stage are done
+        __builtin_amdgcn_s_barrier();            // ... and every other thread's: stage may be overwritten
 #pragma unroll
-        for (int j = 0; j < kSortItems; ++j) {
-            const int p = t + j * kBlock;
-            const unsigned d = L.key[p];
-            if (d != 0xffffffffu) df[d] = stage[L.tmp.src[p]];
+        for (int j = 0; j < kSortItems; ++j) stage[t + j * kBlock] = nx[j];
+        __builtin_amdgcn_s_waitcnt(0xC07F);
+        __builtin_amdgcn_s_barrier();            // stage written
+        if (f < 7) {
+            const auto *sf = gp(st.f[s][f + 1]) + tbase;
+#pragma unroll
+            for (int j = 0; j < kSortItems; ++j) {
+                const int l = t + j * kBlock;
+                nx[j] = __builtin_nontemporal_load(sf + (l < ntile ? l : 0));
+            }
         }
-        __syncthreads();                    // stage read before the next field overwrites it
+        double v[kSortItems];
+#pragma unroll
+        for (int j = 0; j < kSortItems; ++j) v[j] = stage[ss[j]];
+        auto *df = gp(st.alt[s][f]);
+#pragma unroll
+        for (int j = 0; j < kSortItems; ++j) df[dd[j]] = v[j];
     }
 }
 
@@ -1793,8 +1855,6 @@ static int sort_enqueue(ipm_ctx *c, int s0, int s1)
     const size_t ncell = (size_t)(c->nz - 1) * (c->nr - 1);
     const unsigned nblk = (unsigned)((ncell + kScanBlock - 1) / kScanBlock);
     if (nblk > kScanBlock) return fail(IPM_EINVAL, "grid too large for the two-level scan");
-    int end_bit = 0;                        // bits of the sentinel key ncell
-    while (end_bit < 32 && (ncell >> end_bit) != 0) ++end_bit;
     const unsigned cblk = (unsigned)((ncell + kBlock - 1) / kBlock);
     for (int sp = s0; sp < s1; ++sp) {
         if (!c->species_set[sp]) continue;
@@ -1802,13 +1862,13 @@ static int sort_enqueue(ipm_ctx *c, int s0, int s1)
         hipLaunchKernelGGL(clear_u64_kernel, dim3(cblk), dim3(kBlock), 0, c->stream, c->cursor, ncell);
         const unsigned pblk = (unsigned)(((size_t)c->h.cap[sp] + kSortTile - 1) / kSortTile);
         hipLaunchKernelGGL(sort_tile_kernel<false>, dim3(pblk), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr, sp,
-                           end_bit, c->hist, (const unsigned long long *)nullptr);
+                           c->hist, (const unsigned long long *)nullptr);
         hipLaunchKernelGGL(scan_block_kernel, dim3(nblk), dim3(kScanBlock), 0, c->stream, c->hist, c->offs, c->partial, ncell);
         hipLaunchKernelGGL(scan_block_kernel, dim3(1), dim3(kScanBlock), 0, c->stream, c->partial, c->partial2,
                            c->partial + nblk, (size_t)nblk);
         hipLaunchKernelGGL(scan_add_kernel, dim3(nblk), dim3(kScanBlock), 0, c->stream, c->offs, c->partial2, ncell);
         hipLaunchKernelGGL(sort_tile_kernel<true>, dim3(pblk), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr, sp,
-                           end_bit, c->cursor, c->offs);
+                           c->cursor, c->offs);
         hipLaunchKernelGGL(set_count_kernel, dim3(1), dim3(64), 0, c->stream, c->d_ctr, sp, c->offs, c->hist, ncell);
         hipLaunchKernelGGL(swap_buffers_kernel, dim3(1), dim3(64), 0, c->stream, c->d_state, sp);
         if (c->tanc) {
@@ -1954,7 +2014,7 @@ int ipm_set_species(ipm_ctx *c, int s, double charge, double mass, int64_t capac
         if (c->buf[b][s]) HIPCHK(hipFree(c->buf[b][s]));
         c->buf[b][s] = nullptr;
     }
-    const size_t cap = ((size_t)capacity + 31) & ~(size_t)31;   // 256-byte aligned arrays
+    const size_t cap = field_stride(capacity);
     for (int b = 0; b < 2; ++b) HIPCHK(hipMalloc(&c->buf[b][s], sizeof(double) * 8 * cap));
     c->h.cap[s] = (long long)capacity;
     set_buffers(c, s, 0);
