@@ -147,6 +147,21 @@ __device__ __forceinline__ void global_add(double *a, double v)
 {
     __hip_atomic_fetch_add(gp(a), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Device-scope atomics through global-address-space views (flat atomics
+// would also count as LDS operations, so that every later wait for LDS
+// waited for them too).
+template <class T>
+__device__ __forceinline__ T global_fetch_add(T *a, T v)
+{
+    return __hip_atomic_fetch_add(gp(a), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void global_or(int *a, int v)
+{
+    __hip_atomic_fetch_or(gp(a), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Pointers into LDS, typed so: accesses through them are ds_* instructions.
+template <class T>
+using lds_ptr = __attribute__((address_space(3))) T *;
 
 // ---------------------------------------------------------------- RNG
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
@@ -550,12 +565,13 @@ __device__ __forceinline__ int select_probability(const Mc &m, const DevTable &t
 {
     const int Nr = t.nreaction, Ns = t.nsigma;
     const double E = m.energy;
-    const double *__restrict__ P = t.prob;
+    const auto *__restrict__ P = gp(t.prob);
+    const auto *__restrict__ TH = gp(t.threshold);
     double lx = (log10(E) - t.emin) / t.einterval;
     int N = (E > 0.0) ? (int)ceil(lx) : 0;
     int mode;
     double S1 = 0.0, S2 = 0.0;
-    const double *a, *b = nullptr;
+    decltype(P) a, b = nullptr;
     if (N < 1) {
         mode = 0; a = P;
     } else if (N < Ns) {
@@ -572,14 +588,14 @@ __device__ __forceinline__ int select_probability(const Mc &m, const DevTable &t
         if (mode == 0) v = a[j];
         else if (mode == 1) v = a[j] * S1 + b[j] * S2;
         else v = a[j] * E / t.emax;
-        if (E < t.threshold[j]) v = 0.0;
+        if (E < TH[j]) v = 0.0;
         return v;
     };
     // A cumulative table (columns nondecreasing in j, as ProbilityNonReactive
     // builds them) interpolates to values nondecreasing in j (the weights
     // are nonnegative and rounding is monotone), so R >= the last value
     // misses every reaction: the null event is known from one lookup.
-    if (t.cumulative && !(E < t.threshold[Nr - 1]) && !(R < value(Nr - 1))) return 0;
+    if (t.cumulative && !(E < TH[Nr - 1]) && !(R < value(Nr - 1))) return 0;
     for (int j = 0; j < Nr; ++j)
         if (R < value(j)) return j + 1;
     return 0;
@@ -592,7 +608,7 @@ __device__ __forceinline__ int select_probability_reactive(const Mc &m, const De
 {
     double beta = t.beta_max * sqrt(R);
     if (beta <= 1.0) {
-        if (m.energy <= t.threshold[1]) return 1;
+        if (m.energy <= gp(t.threshold)[1]) return 1;
         return select_probability(m, t, rg.next());
     }
     return 1;
@@ -632,7 +648,7 @@ __device__ __forceinline__ bool surely_null(const DevTable &t, double vz, double
     if (ehi < 1e300) hi = (int)fmin(fmax(ceil(approx_lx(t, ehi) + t.nb_eps), 0.0), (double)Ns);
     if (hi < lo) return false;
     const int L = 31 - __clz(hi - lo + 1);
-    const double *row = t.nb_tab + (size_t)L * (size_t)(Ns + 1);
+    const auto *row = gp(t.nb_tab) + (size_t)L * (size_t)(Ns + 1);
     double bound = fmax(row[lo], row[hi - (1 << L) + 1]);
     if (hi == Ns) bound = bound * fmax(1.0, ehi * t.nb_inv_emax) * (1.0 + 1e-12);
     return R >= bound;

@@ -66,10 +66,10 @@ __device__ __forceinline__ bool is_dead(double z) { return z < -1.0e299; }
 constexpr int kProdStage = 256;
 constexpr int kCollideBlocksPerCuMax = 6;   // the product stages are sized for this many
 struct ProdSink {
-    double *rec;        // kProdStage x 8 doubles
-    unsigned *meta;     // species | rank << 8
-    unsigned *count;    // staged so far
-    unsigned *spc;      // staged per species (IPM_NS_MAX)
+    lds_ptr<double> rec;        // kProdStage x 8 doubles
+    lds_ptr<unsigned> meta;     // species | rank << 8
+    lds_ptr<unsigned> count;    // staged so far
+    lds_ptr<unsigned> spc;      // staged per species (IPM_NS_MAX)
 };
 
 // twin != 0: the product would be a bit-identical copy of another particle
@@ -81,7 +81,7 @@ __device__ __forceinline__ void emit_product(const DevState &st, DevCounters *ct
                                              const ProdSink *ps = nullptr, bool twin = false)
 {
     if (d < 0 || d >= st.ns) {
-        atomicOr(&ctr->overflow, 2);
+        global_or(&ctr->overflow, 2);
         return;
     }
     q.Vz = q.Vz * st.inv_vfactor;
@@ -89,25 +89,25 @@ __device__ __forceinline__ void emit_product(const DevState &st, DevCounters *ct
     q.Vt = q.Vt * st.inv_vfactor;
     if (twin) q.Vt = __longlong_as_double(__double_as_longlong(q.Vt) ^ 1ll);
     if (ps) {
-        const unsigned k = atomicAdd(ps->count, 1u);
+        const unsigned k = __hip_atomic_fetch_add(ps->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         if (k < (unsigned)kProdStage) {
-            const unsigned r = atomicAdd(&ps->spc[d], 1u);
-            double *o = ps->rec + (size_t)k * 8;
+            const unsigned r = __hip_atomic_fetch_add(ps->spc + d, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            lds_ptr<double> o = ps->rec + (size_t)k * 8;
             o[0] = q.Z; o[1] = q.R; o[2] = q.Vz; o[3] = q.Vr; o[4] = q.Vt; o[5] = q.Az; o[6] = q.Ar; o[7] = q.WQ;
             ps->meta[k] = (unsigned)d | (r << 8);
             return;
         }
     }
     // no sink, or the block's stage is full: reserve a slot directly
-    unsigned long long k = atomicAdd(&ctr->n_new[d], 1ull);
-    unsigned long long pos = ctr->n[d] + k;
+    unsigned long long k = global_fetch_add(&ctr->n_new[d], 1ull);
+    unsigned long long pos = gp(ctr->n)[d] + k;
     if (pos >= (unsigned long long)st.cap[d]) {
-        atomicOr(&ctr->overflow, 1);
+        global_or(&ctr->overflow, 1);
         return;
     }
     double *const *F = st.f[d];
-    F[0][pos] = q.Z; F[1][pos] = q.R; F[2][pos] = q.Vz; F[3][pos] = q.Vr;
-    F[4][pos] = q.Vt; F[5][pos] = q.Az; F[6][pos] = q.Ar; F[7][pos] = q.WQ;
+    gp(F[0])[pos] = q.Z; gp(F[1])[pos] = q.R; gp(F[2])[pos] = q.Vz; gp(F[3])[pos] = q.Vr;
+    gp(F[4])[pos] = q.Vt; gp(F[5])[pos] = q.Az; gp(F[6])[pos] = q.Ar; gp(F[7])[pos] = q.WQ;
 }
 
 // PostVelocityIonReactive, R/src/mc/MCCIon.F90:54-74 (Vy from Vz as written there)
@@ -149,9 +149,9 @@ __device__ __forceinline__ void mcc_finish(const DevState &st, DevCounters *ctr,
                                            Mc &m, Rng &rg, int idx, Part &p, const ProdSink *ps = nullptr)
 {
     if (idx > 0) {
-        const int ty = t.rtype[idx - 1];
-        const int res = t.resultant[idx - 1];
-        const double th = t.threshold[idx - 1];
+        const int ty = gp(t.rtype)[idx - 1];
+        const int res = gp(t.resultant)[idx - 1];
+        const double th = gp(t.threshold)[idx - 1];
         if (t.model == 1) {
             switch (ty) {
             case 11:
@@ -214,7 +214,7 @@ __device__ __forceinline__ void mcc_finish(const DevState &st, DevCounters *ctr,
         }
     }
     if (st.count_events)
-        atomicAdd(&st.mcc_counts[((size_t)s * st.ng + g) * 128 + (idx & 127)], 1ull);
+        global_fetch_add(&st.mcc_counts[((size_t)s * st.ng + g) * 128 + (idx & 127)], 1ull);
     m.poi.Vz = m.poi.Vz * st.inv_vfactor;
     m.poi.Vr = m.poi.Vr * st.inv_vfactor;
     m.poi.Vt = m.poi.Vt * st.inv_vfactor;
@@ -350,7 +350,7 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
     const int ns = st.ns;
     long long T = 0;
     for (int s = 0; s < ns; ++s)
-        if (sel < 0 || sel == s) T += ((long long)ctr->n[s] + kTile - 1) / kTile;
+        if (sel < 0 || sel == s) T += ((long long)gp(ctr->n)[s] + kTile - 1) / kTile;
 
     if (MODE & M_DEPOSIT)
         for (int q = threadIdx.x; q < kWinZ * WINR; q += BLOCK) win[q] = 0.0;
@@ -371,11 +371,11 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
         int s = 0;
         long long tbeg = 0;
         for (int q = 0; q < ns; ++q) {
-            const long long nt = (sel < 0 || sel == q) ? ((long long)ctr->n[q] + kTile - 1) / kTile : 0;
+            const long long nt = (sel < 0 || sel == q) ? ((long long)gp(ctr->n)[q] + kTile - 1) / kTile : 0;
             if (t < tbeg + nt) { s = q; break; }
             tbeg += nt;
         }
-        const long long n_s = (long long)ctr->n[s];
+        const long long n_s = (long long)gp(ctr->n)[s];
         const long long base = (t - tbeg) * kTile;
         double *const *F = st.f[s];
         const double charge = st.charge[s];
@@ -385,7 +385,7 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
         // Deposit window of the tile: placed by the tile's first particle as
         // the previous pass left it (any placement is correct: cells outside
         // the window go straight to HBM), so no wave waits for a load here.
-        int *const anc = st.tile_anchor + 2 * (st.q_cbase[s] + (t - tbeg));
+        const auto anc = gp(st.tile_anchor) + 2 * (st.q_cbase[s] + (t - tbeg));
         const int zw0 = (MODE & M_DEPOSIT) ? anc[0] : 0;
         const int rw0 = (MODE & M_DEPOSIT) ? anc[1] : 0;
         int na0 = zw0, na1 = rw0;   // next pass's window, from slot base after this pass
@@ -433,7 +433,7 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
                 const unsigned long long km = __ballot(keep);
                 const unsigned nkeep = (unsigned)__popcll(km);
                 if (st.count_events && lane == 0 && cw > nkeep)
-                    atomicAdd(&st.mcc_counts[(size_t)s * 128], (unsigned long long)(cw - nkeep));
+                    global_fetch_add(&st.mcc_counts[(size_t)s * 128], (unsigned long long)(cw - nkeep));
                 if (keep) {
                     const unsigned r = (unsigned)__popcll(km & ((1ull << lane) - 1ull));
 #pragma unroll
@@ -764,7 +764,7 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
                 const unsigned n = q_scount[qbuf];
                 const long long R = st.q_tile_cap[s];
                 const unsigned nq = (long long)n < R ? n : (unsigned)R;
-                st.q_tile_count[st.q_cbase[s] + (t - tbeg)] = nq;
+                gp(st.q_tile_count)[st.q_cbase[s] + (t - tbeg)] = nq;
                 if (nq) atomicAdd(&ctr->q_total[s], (unsigned long long)nq);
                 q_scount[qbuf] = 0;     // reused two tiles later, after a barrier
             }
@@ -822,7 +822,8 @@ collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
     const int ns = st.ns;
     const unsigned G = gridDim.x;
     if (threadIdx.x == 0) pcount = 0;
-    const ProdSink sink = {prec, pmeta, &pcount, pspc};
+    const ProdSink sink = {(lds_ptr<double>)prec, (lds_ptr<unsigned>)pmeta, (lds_ptr<unsigned>)&pcount,
+                           (lds_ptr<unsigned>)pspc};
     if (threadIdx.x < IPM_NS_MAX) pspc[threadIdx.x] = 0;
 
     // species of this block and its tile range: one block for every species
@@ -841,7 +842,7 @@ collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
         }
         // every species with work needs a block of its own: a grid smaller
         // than that would leave queued selections uncollided
-        if (A > (int)G && blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&ctr->overflow, 4);
+        if (A > (int)G && blockIdx.x == 0 && threadIdx.x == 0) global_or(&ctr->overflow, 4);
         if (A > 0 && G >= (unsigned)A) {
             const double Grem = (double)(G - A);
             double Wlt = 0.0;
@@ -878,7 +879,7 @@ collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
             unsigned tsum = 0;
             for (int j = 0; j < per; ++j) {
                 const int tt = threadIdx.x * per + j;
-                c[j] = tt < m ? cnt[tc + tt] : 0u;
+                c[j] = tt < m ? gp(cnt)[tc + tt] : 0u;
                 tsum += c[j];
             }
             unsigned x = tsum;
@@ -932,7 +933,7 @@ collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
                             surely_null(tb, p.Vz * st.vfactor, p.Vr * st.vfactor, p.Vt * st.vfactor, u)) {
                             // a null event: the count and VelRes both ways
                             if (st.count_events)
-                                atomicAdd(&st.mcc_counts[((size_t)s * st.ng + g) * 128], 1ull);
+                                global_fetch_add(&st.mcc_counts[((size_t)s * st.ng + g) * 128], 1ull);
                             p.Vz = p.Vz * st.vfactor * st.inv_vfactor;
                             p.Vr = p.Vr * st.vfactor * st.inv_vfactor;
                             p.Vt = p.Vt * st.vfactor * st.inv_vfactor;
@@ -957,14 +958,14 @@ collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
                 if (__double_as_longlong(p.Vz) != __double_as_longlong(vz0) ||
                     __double_as_longlong(p.Vr) != __double_as_longlong(vr0) ||
                     __double_as_longlong(p.Vt) != __double_as_longlong(vt0)) {
-                    __builtin_nontemporal_store(p.Vz, &F[2][i]);
-                    __builtin_nontemporal_store(p.Vr, &F[3][i]);
-                    __builtin_nontemporal_store(p.Vt, &F[4][i]);
+                    __builtin_nontemporal_store(p.Vz, gp(F[2]) + i);
+                    __builtin_nontemporal_store(p.Vr, gp(F[3]) + i);
+                    __builtin_nontemporal_store(p.Vt, gp(F[4]) + i);
                 }
                 if (__double_as_longlong(p.Az) != __double_as_longlong(p0.Az) ||
                     __double_as_longlong(p.Ar) != __double_as_longlong(p0.Ar)) {
-                    __builtin_nontemporal_store(p.Az, &F[5][i]);
-                    __builtin_nontemporal_store(p.Ar, &F[6][i]);
+                    __builtin_nontemporal_store(p.Az, gp(F[5]) + i);
+                    __builtin_nontemporal_store(p.Ar, gp(F[6]) + i);
                 }
             }
             __syncthreads();
@@ -974,9 +975,9 @@ collide_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr)
     __syncthreads();
     const unsigned np = min(pcount, (unsigned)kProdStage);
     double *gs = st.p_stage + (size_t)blockIdx.x * kProdStage * 8;
-    for (unsigned k = threadIdx.x; k < np * 8u; k += kBlock) gs[k] = prec[k];
-    for (unsigned k = threadIdx.x; k < np; k += kBlock) st.p_meta[(size_t)blockIdx.x * kProdStage + k] = pmeta[k];
-    if (threadIdx.x < IPM_NS_MAX) st.p_cnt[(size_t)blockIdx.x * IPM_NS_MAX + threadIdx.x] = pspc[threadIdx.x];
+    for (unsigned k = threadIdx.x; k < np * 8u; k += kBlock) gp(gs)[k] = prec[k];
+    for (unsigned k = threadIdx.x; k < np; k += kBlock) gp(st.p_meta)[(size_t)blockIdx.x * kProdStage + k] = pmeta[k];
+    if (threadIdx.x < IPM_NS_MAX) gp(st.p_cnt)[(size_t)blockIdx.x * IPM_NS_MAX + threadIdx.x] = pspc[threadIdx.x];
 }
 
 // Block b appends its staged products of species d after n[d], the
@@ -995,7 +996,7 @@ collide_append_kernel(const DevState *__restrict__ stp, DevCounters *ctr)
     unsigned long long acc[IPM_NS_MAX];
     for (int d = 0; d < IPM_NS_MAX; ++d) acc[d] = 0;
     for (unsigned b = threadIdx.x; b < blockIdx.x; b += kBlock)
-        for (int d = 0; d < ns; ++d) acc[d] += st.p_cnt[(size_t)b * IPM_NS_MAX + d];
+        for (int d = 0; d < ns; ++d) acc[d] += gp(st.p_cnt)[(size_t)b * IPM_NS_MAX + d];
     for (int d = 0; d < ns; ++d) {
         unsigned long long x = acc[d];
         for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o);
@@ -1005,23 +1006,23 @@ collide_append_kernel(const DevState *__restrict__ stp, DevCounters *ctr)
     if (threadIdx.x < (unsigned)ns) {
         unsigned long long t = 0;
         for (int k = 0; k < kBlock / 64; ++k) t += part[k][threadIdx.x];
-        off[threadIdx.x] = ctr->n[threadIdx.x] + ctr->n_new[threadIdx.x] + t;
+        off[threadIdx.x] = gp(ctr->n)[threadIdx.x] + gp(ctr->n_new)[threadIdx.x] + t;
     }
     __syncthreads();
     unsigned np = 0;
-    for (int d = 0; d < ns; ++d) np += st.p_cnt[(size_t)blockIdx.x * IPM_NS_MAX + d];
-    const double *gs = st.p_stage + (size_t)blockIdx.x * kProdStage * 8;
+    for (int d = 0; d < ns; ++d) np += gp(st.p_cnt)[(size_t)blockIdx.x * IPM_NS_MAX + d];
+    const auto *gs = gp(st.p_stage) + (size_t)blockIdx.x * kProdStage * 8;
     for (unsigned k = threadIdx.x; k < np; k += kBlock) {
-        const unsigned meta = st.p_meta[(size_t)blockIdx.x * kProdStage + k];
+        const unsigned meta = gp(st.p_meta)[(size_t)blockIdx.x * kProdStage + k];
         const int d = (int)(meta & 0xffu);
         const unsigned long long pos = off[d] + (meta >> 8);
         if (pos >= (unsigned long long)st.cap[d]) {
-            atomicOr(&ctr->overflow, 1);
+            global_or(&ctr->overflow, 1);
             continue;
         }
         double *const *F = st.f[d];
 #pragma unroll
-        for (int f = 0; f < 8; ++f) F[f][pos] = gs[(size_t)k * 8 + f];
+        for (int f = 0; f < 8; ++f) gp(F[f])[pos] = gs[(size_t)k * 8 + f];
     }
 }
 
