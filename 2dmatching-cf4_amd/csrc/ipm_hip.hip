@@ -348,9 +348,15 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
     __shared__ double q_wst[(MODE & M_MCC) ? (BLOCK / 64) * 64 * kRecD : 1];
 
     const int ns = st.ns;
+    // The populations, the step and the seed are read through the scalar
+    // cache (every wave needs the same values, and this pass never writes
+    // them: it adds to n_new, n_dead, q_total and overflow only), so a
+    // block's first loads of particles do not wait behind vector loads of
+    // counters.
+    const auto *cn = (const __attribute__((address_space(4))) unsigned long long *)ctr->n;
     long long T = 0;
     for (int s = 0; s < ns; ++s)
-        if (sel < 0 || sel == s) T += ((long long)gp(ctr->n)[s] + kTile - 1) / kTile;
+        if (sel < 0 || sel == s) T += ((long long)cn[s] + kTile - 1) / kTile;
 
     if (MODE & M_DEPOSIT)
         for (int q = threadIdx.x; q < kWinZ * WINR; q += BLOCK) win[q] = 0.0;
@@ -362,8 +368,8 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
 
     const long long t1 = T, tfirst = blockIdx.x, tstride = gridDim.x;
 
-    const uint32_t step = ctr->step;
-    const unsigned long long seed = ctr->seed;
+    const uint32_t step = *(const __attribute__((address_space(4))) uint32_t *)&ctr->step;
+    const unsigned long long seed = *(const __attribute__((address_space(4))) unsigned long long *)&ctr->seed;
     const int lane = threadIdx.x & 63;
 
     for (long long t = tfirst; t < t1; t += tstride) {
@@ -371,11 +377,11 @@ particle_kernel(const DevState *__restrict__ stp, DevCounters *__restrict__ ctr,
         int s = 0;
         long long tbeg = 0;
         for (int q = 0; q < ns; ++q) {
-            const long long nt = (sel < 0 || sel == q) ? ((long long)gp(ctr->n)[q] + kTile - 1) / kTile : 0;
+            const long long nt = (sel < 0 || sel == q) ? ((long long)cn[q] + kTile - 1) / kTile : 0;
             if (t < tbeg + nt) { s = q; break; }
             tbeg += nt;
         }
-        const long long n_s = (long long)gp(ctr->n)[s];
+        const long long n_s = (long long)cn[s];
         const long long base = (t - tbeg) * kTile;
         double *const *F = st.f[s];
         const double charge = st.charge[s];
