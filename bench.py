@@ -64,6 +64,34 @@ def stream_ceiling(n, reps=5):
                     f"non-temporal accesses (value: the faster); copy_GBps: one-array device copy (read + write)"}
 
 
+# Products one collision of each reaction type makes (the resultants of
+# ElectronIonization, ElectronAttachment, ElectronDissociation and the
+# reactive ion channels, R/src/mc/MCCElectron.F90:47-147, R/src/mc/MCCIon.F90:54-74).
+PRODUCTS_BY_TYPE = {13: 2, 14: 1, 15: 2, 112: 1, 113: 1}
+
+
+def collision_counts(hp, tables, steps=4):
+    """Selections (null events included) and created particles per step,
+    from the event counts of `steps` further steps.  The whole step's
+    algorithmic bytes add 48 B per selected particle (its velocity read and
+    written around the collision) and 64 B per created particle (8 fields
+    written) to the particle pass's 120 B per particle."""
+    hp.set_event_counting(True)
+    for _ in range(steps):
+        hp.one_step()
+    hp.synchronize()
+    c = hp.take_mcc_counts()
+    hp.set_event_counting(False)
+    selected = float(c.sum()) / steps
+    created = 0.0
+    for (s, g), t in tables.items():
+        if t is None or getattr(t, "rtype", None) is None:
+            continue
+        for j, ty in enumerate(t.rtype):
+            created += float(c[s, g, j + 1]) * PRODUCTS_BY_TYPE.get(int(ty), 0)
+    return selected, created / steps
+
+
 def live_total(hp, ns):
     return sum(hp.count(s) for s in range(ns))
 
@@ -505,6 +533,11 @@ def main():
         except Exception:
             traffic = None
 
+    # (not in the traffic probe, whose last launches the PMC figures average)
+    sel_step, made_step = collision_counts(hp, tables) if not args.traffic_probe else (0.0, 0.0)
+    step_bytes = bytes_per_launch + 48.0 * sel_step + 64.0 * made_step
+    log(f"[bench] per step: {sel_step:.4g} selected, {made_step:.4g} created particles")
+
     extra = {"eager_value": eager_value, "step_form": "eager launches" if el is el_e else "captured graph replay"}
     if not args.no_extra:
         p_steps = max(5, args.steps // 10)
@@ -574,11 +607,16 @@ def main():
             "stream_ceiling": ceiling,
         },
         "step_roofline": {
-            "achieved": bytes_per_launch / step_s / 1e9,
-            "frac": bytes_per_launch / step_s / 1e9 / HBM_PEAK_GBS,
+            "achieved": step_bytes / step_s / 1e9,
+            "frac": step_bytes / step_s / 1e9 / HBM_PEAK_GBS,
             "unit": "GB/s",
+            "algorithmic_bytes_per_step": step_bytes,
+            "selected_per_step": sel_step,
+            "created_per_step": made_step,
+            "frac_particle_pass_bytes_only": bytes_per_launch / step_s / 1e9 / HBM_PEAK_GBS,
             "what": "the whole step (particle pass, collision pass, grid epilogue, launch gaps and the amortised "
-                    "sort): the particle pass's algorithmic bytes over ms_per_step",
+                    "sort): its algorithmic bytes (120 B per particle-step plus the grids, 48 B per selected "
+                    "particle, 64 B per created particle) over ms_per_step",
         },
         "sort": {"interval": sort_interval, "ms_after_full_interval": t_sort * 1e3,
                  "ms_per_step_amortised": amort * 1e3,
