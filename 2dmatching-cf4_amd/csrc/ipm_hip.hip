@@ -323,12 +323,15 @@ __device__ __forceinline__ bool wave_segmented_sum(int key, double (&v)[4])
 // several (BLOCK, WINR) pairs, which the autotuner chooses among; every
 // other mode uses (256, 24).
 // Waves per SIMD that the particle pass's LDS (deposit window and queue
-// staging) leaves room for, at most the register budget's five.
+// staging) leaves room for, at most the register budget's five, and a
+// whole number of blocks (a 512-thread block puts two waves on each SIMD).
 constexpr int pk_waves(int block, int winr)
 {
     const int bytes = kWinZ * winr * 8 + block * kRecD * 8 + 512;
+    const int per = block >= 256 ? block / 256 : 1;       // waves per SIMD of one block
     const int w = (160 * 1024 / bytes) * (block / 64) / 4;
-    return w < 5 ? w : 5;
+    const int cap = 5 / per * per;
+    return (w < cap ? w : cap) / per * per;
 }
 template <int MODE, int BLOCK = kBlock, int WINR = kWinR>
 __global__ void __launch_bounds__(BLOCK)
