@@ -1425,7 +1425,11 @@ sort_tile_kernel(const DevState *__restrict__ stp, const DevCounters *__restrict
 {
     __shared__ SortTileLds L;
     __shared__ double stage[SCATTER ? kSortTile : 1];   // one field of the tile (scatter pass)
-    __shared__ unsigned kmin_s, kmax_s;
+    // The tile's smallest and largest key are reduced in the last two words
+    // of L.key, which nothing writes before the counting (or the radix
+    // sort's own barriers) has passed: a separate variable would take the
+    // LDS block past 40 KB and the scatter pass to three blocks per CU.
+    unsigned &kmin_s = L.key[kSortTile - 2], &kmax_s = L.key[kSortTile - 1];
     const DevState &st = *stp;
     const long long n = (long long)ctr->n[s];
     const long long tbase = (long long)blockIdx.x * kSortTile;
@@ -1433,18 +1437,37 @@ sort_tile_kernel(const DevState *__restrict__ stp, const DevCounters *__restrict
     const unsigned S = (unsigned)((st.nz - 1) * (st.nr - 1));
     const int t = threadIdx.x;
     if (t == 0) { kmin_s = S; kmax_s = 0; }
+    // Z and R of the thread's slots, read together (slots past the
+    // population read the tile's first slot and take the sentinel key); the
+    // scatter pass moves these values as its first two fields, and loads the
+    // third while the keys are counted.
+    const int ntile = (int)min((long long)kSortTile, n - tbase);
+    double zv[kSortItems], rv[kSortItems], nx[kSortItems];
+    {
+        const auto *fz = gp(st.f[s][0]) + tbase, *fr = gp(st.f[s][1]) + tbase;
+#pragma unroll
+        for (int j = 0; j < kSortItems; ++j) {
+            const int l = t + j * kBlock;
+            zv[j] = __builtin_nontemporal_load(fz + (l < ntile ? l : 0));
+            rv[j] = __builtin_nontemporal_load(fr + (l < ntile ? l : 0));
+        }
+        if (SCATTER) {
+            const auto *f2 = gp(st.f[s][2]) + tbase;
+#pragma unroll
+            for (int j = 0; j < kSortItems; ++j) {
+                const int l = t + j * kBlock;
+                nx[j] = __builtin_nontemporal_load(f2 + (l < ntile ? l : 0));
+            }
+        }
+    }
     unsigned key[kSortItems];
     unsigned short loc[kSortItems];
     unsigned kmin = S, kmax = 0;
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) {
         const int l = t + j * kBlock;
-        const long long i = tbase + l;
         unsigned k = S;
-        if (i < n) {
-            const double z = gp(st.f[s][0])[i];
-            if (!is_dead(z)) k = (unsigned)cell_of(st, z, gp(st.f[s][1])[i]);
-        }
+        if (l < ntile && !is_dead(zv[j])) k = (unsigned)cell_of(st, zv[j], rv[j]);
         key[j] = k;
         loc[j] = (unsigned short)l;
         if (k < S) { kmin = min(kmin, k); kmax = max(kmax, k); }
@@ -1461,54 +1484,129 @@ sort_tile_kernel(const DevState *__restrict__ stp, const DevCounters *__restrict
     kmax = kmax_s;
     if (kmin > kmax) return;                // no live particle in the tile (uniform)
     const unsigned span = kmax - kmin + 1;  // relative key of the sentinel
-    int nbits = 0;
-    while (nbits < 32 && (span >> nbits) != 0) ++nbits;
+    if (span <= (unsigned)kSortTile) {
+        // Counted in LDS, one bin per cell of the span: a wave's 64 slots
+        // (consecutive) fall into a few runs of equal cells, and each run
+        // takes its ranks with one LDS atomic; each nonzero bin then costs
+        // one atomic on its cell's counter.  The bins' prefix sums place
+        // the runs in the tile's sorted order.  (Tiles whose cells span
+        // more than kSortTile -- the products appended since the last sort
+        // -- are radix-sorted below.)
+        unsigned *bins = L.base;
+        for (int k = t; k < (int)span; k += kBlock) bins[k] = 0u;
+        __syncthreads();
+        const int lane = t & 63;
+        const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+        unsigned rank[kSortItems];
 #pragma unroll
-    for (int j = 0; j < kSortItems; ++j) key[j] = key[j] < S ? key[j] - kmin : span;
-    TileSort(L.tmp.sort).Sort(key, loc, 0, nbits);        // blocked: thread t holds 8t .. 8t+7
-#pragma unroll
-    for (int j = 0; j < kSortItems; ++j) key[j] = key[j] < span ? key[j] + kmin : S;
-#pragma unroll
-    for (int j = 0; j < kSortItems; ++j) L.key[t * kSortItems + j] = key[j];
-    __syncthreads();
-    int head[kSortItems], start[kSortItems];
-#pragma unroll
-    for (int j = 0; j < kSortItems; ++j) {
-        const int p = t * kSortItems + j;
-        head[j] = (p == 0 || L.key[p - 1] != key[j]) ? p : 0;
-    }
-    TileScan(L.tmp.scan).InclusiveScan(head, start, hipcub::Max());   // first position of each run
-#pragma unroll
-    for (int j = 0; j < kSortItems; ++j) {
-        const int p = t * kSortItems + j;
-        const bool run_end = p == kSortTile - 1 || L.key[p + 1] != key[j];
-        if (run_end && key[j] < S) {
-            const unsigned long long len = (unsigned long long)(p - start[j] + 1);
-            const unsigned long long b =
-                __hip_atomic_fetch_add(gp(&counts[key[j]]), len, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (SCATTER) L.base[start[j]] = (unsigned)b;
+        for (int j = 0; j < kSortItems; ++j) {
+            const unsigned rel = key[j] < S ? key[j] - kmin : span;
+            key[j] = rel;
+            const unsigned kp = __shfl_up(rel, 1);
+            const unsigned long long starts = __ballot(lane == 0 || kp != rel);
+            const int first = 63 - __clzll((long long)(starts & upto));
+            const unsigned long long above = starts & ~upto;
+            const int last = (above ? __ffsll((long long)above) - 1 : 64) - 1;
+            unsigned old = 0u;
+            if (lane == last && rel < span) old = atomicAdd(&bins[rel], (unsigned)(lane - first + 1));
+            old = __shfl(old, last);
+            rank[j] = old + (unsigned)(lane - first);
         }
-    }
-    if (!SCATTER) return;
-    __syncthreads();                        // base[] written; tmp.scan and key[] reads done
+        __syncthreads();                    // every bin counted
+        unsigned c8[kSortItems];
+        unsigned tsum = 0u;
 #pragma unroll
-    for (int j = 0; j < kSortItems; ++j) {
-        const int p = t * kSortItems + j;
-        L.key[p] = key[j] < S ? (unsigned)(gp(offs)[key[j]] + L.base[start[j]] + (unsigned)(p - start[j])) : 0xffffffffu;
-        L.tmp.src[p] = loc[j];
+        for (int q = 0; q < kSortItems; ++q) {
+            const int k = t * kSortItems + q;
+            c8[q] = k < (int)span ? bins[k] : 0u;
+            tsum += c8[q];
+        }
+        // the tile's first rank within each of its cells (scatter pass: kept
+        // in the staging buffer, which the fields use only later)
+        unsigned *gb = SCATTER ? reinterpret_cast<unsigned *>(stage) : nullptr;
+#pragma unroll
+        for (int q = 0; q < kSortItems; ++q) {
+            if (c8[q]) {
+                const int k = t * kSortItems + q;
+                const unsigned long long g = __hip_atomic_fetch_add(gp(&counts[kmin + (unsigned)k]),
+                                                                    (unsigned long long)c8[q], __ATOMIC_RELAXED,
+                                                                    __HIP_MEMORY_SCOPE_AGENT);
+                if (SCATTER) gb[k] = (unsigned)g;
+            }
+        }
+        if (!SCATTER) return;
+        int excl = 0, total = 0;
+        TileScan(L.tmp.scan).ExclusiveSum((int)tsum, excl, total);
+        __syncthreads();                    // scan storage free; bins read
+        unsigned run = (unsigned)excl;
+#pragma unroll
+        for (int q = 0; q < kSortItems; ++q) {
+            const int k = t * kSortItems + q;
+            if (k < (int)span) bins[k] = run;   // first sorted position of the bin
+            run += c8[q];
+        }
+        for (int p = total + t; p < kSortTile; p += kBlock) {   // positions past the tile's particles
+            L.key[p] = 0xffffffffu;
+            L.tmp.src[p] = 0;
+        }
+        __syncthreads();                    // starts written
+#pragma unroll
+        for (int j = 0; j < kSortItems; ++j) {
+            if (key[j] < span) {
+                const unsigned p = bins[key[j]] + rank[j];
+                L.tmp.src[p] = (unsigned short)(t + j * kBlock);
+                L.key[p] = (unsigned)(gp(offs)[kmin + key[j]] + gb[key[j]] + rank[j]);
+            }
+        }
+    } else {
+        int nbits = 0;
+        while (nbits < 32 && (span >> nbits) != 0) ++nbits;
+#pragma unroll
+        for (int j = 0; j < kSortItems; ++j) key[j] = key[j] < S ? key[j] - kmin : span;
+        TileSort(L.tmp.sort).Sort(key, loc, 0, nbits);        // blocked: thread t holds 8t .. 8t+7
+#pragma unroll
+        for (int j = 0; j < kSortItems; ++j) key[j] = key[j] < span ? key[j] + kmin : S;
+#pragma unroll
+        for (int j = 0; j < kSortItems; ++j) L.key[t * kSortItems + j] = key[j];
+        __syncthreads();
+        int head[kSortItems], start[kSortItems];
+#pragma unroll
+        for (int j = 0; j < kSortItems; ++j) {
+            const int p = t * kSortItems + j;
+            head[j] = (p == 0 || L.key[p - 1] != key[j]) ? p : 0;
+        }
+        TileScan(L.tmp.scan).InclusiveScan(head, start, hipcub::Max());   // first position of each run
+#pragma unroll
+        for (int j = 0; j < kSortItems; ++j) {
+            const int p = t * kSortItems + j;
+            const bool run_end = p == kSortTile - 1 || L.key[p + 1] != key[j];
+            if (run_end && key[j] < S) {
+                const unsigned long long len = (unsigned long long)(p - start[j] + 1);
+                const unsigned long long b =
+                    __hip_atomic_fetch_add(gp(&counts[key[j]]), len, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (SCATTER) L.base[start[j]] = (unsigned)b;
+            }
+        }
+        if (!SCATTER) return;
+        __syncthreads();                        // base[] written; tmp.scan and key[] reads done
+#pragma unroll
+        for (int j = 0; j < kSortItems; ++j) {
+            const int p = t * kSortItems + j;
+            L.key[p] = key[j] < S ? (unsigned)(gp(offs)[key[j]] + L.base[start[j]] + (unsigned)(p - start[j])) : 0xffffffffu;
+            L.tmp.src[p] = loc[j];
+        }
     }
     // Field by field: the tile's values are read coalesced (non-temporal:
     // read once) into LDS, then written in sorted order, neighbouring lanes
     // to neighbouring slots of a run.  Reading them straight from HBM in
     // sorted order costs a cache line per lane.  Each thread keeps the
     // destinations and source slots of its sorted positions in registers,
-    // and loads the next field while the current one is written out (loads
-    // of slots past the population read the tile's first slot instead;
+    // and loads the next field while the current one is written out (Z and
+    // R are the key phase's values, the third field was loaded then;
     // sorted positions without a particle write to the spare slot past the
     // capacity, so no load or store is conditional and the waits for the
     // loads need not wait for the stores).  The barriers only order LDS: no
     // memory operation is waited for there.
-    const int ntile = (int)min((long long)kSortTile, n - tbase);
     __syncthreads();                        // key/src written
     unsigned dd[kSortItems];
     int ss[kSortItems];
@@ -1519,23 +1617,15 @@ sort_tile_kernel(const DevState *__restrict__ stp, const DevCounters *__restrict
         dd[j] = d != 0xffffffffu ? d : spare;
         ss[j] = L.tmp.src[t + j * kBlock];
     }
-    double nx[kSortItems];
-    {
-        const auto *sf = gp(st.f[s][0]) + tbase;
 #pragma unroll
-        for (int j = 0; j < kSortItems; ++j) {
-            const int l = t + j * kBlock;
-            nx[j] = __builtin_nontemporal_load(sf + (l < ntile ? l : 0));
-        }
-    }
     for (int f = 0; f < 8; ++f) {
         __builtin_amdgcn_s_waitcnt(0xC07F);       // lgkmcnt(0): this thread's LDS reads of stage are done
         __builtin_amdgcn_s_barrier();            // ... and every other thread's: stage may be overwritten
 #pragma unroll
-        for (int j = 0; j < kSortItems; ++j) stage[t + j * kBlock] = nx[j];
+        for (int j = 0; j < kSortItems; ++j) stage[t + j * kBlock] = f == 0 ? zv[j] : f == 1 ? rv[j] : nx[j];
         __builtin_amdgcn_s_waitcnt(0xC07F);
         __builtin_amdgcn_s_barrier();            // stage written
-        if (f < 7) {
+        if (f >= 2 && f < 7) {
             const auto *sf = gp(st.f[s][f + 1]) + tbase;
 #pragma unroll
             for (int j = 0; j < kSortItems; ++j) {
