@@ -80,6 +80,7 @@ struct ipm_ctx {
     int block = 256, blocks_per_cu = 0;
     int pk_block = 256, pk_winr = 24;      // fused particle pass: threads per block, deposit window rows
     int collide_bpc = 6;                   // collision-pass blocks per compute unit
+    int sort_block = 256;                  // cell sort: threads per block (8 slots each)
     // optional HIP-event timing of the kernels ipm_step enqueues
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;

@@ -1399,18 +1399,22 @@ __global__ void rezone_fold_kernel(DevCounters *ctr, int s, long long cap)
 // which tiles reach the counters; nothing downstream depends on it (the
 // collision draws are keyed by particle state, not by slot).
 constexpr int kSortItems = 8;
-constexpr int kSortTile = kBlock * kSortItems;
-using TileSort = hipcub::BlockRadixSort<unsigned, kBlock, kSortItems, unsigned short>;
-using TileScan = hipcub::BlockScan<int, kBlock>;
+constexpr int kSortTile = kBlock * kSortItems;          // default tile: 2048 slots, 256 threads
+constexpr int kSortTileMax = 2 * kSortTile;             // 4096 slots, 512 threads
 
-struct SortTileLds {
+template <int SB> struct SortTypes {
+    using Sort = hipcub::BlockRadixSort<unsigned, SB, kSortItems, unsigned short>;
+    using Scan = hipcub::BlockScan<int, SB>;
+};
+
+template <int SB> struct SortTileLds {
+    static constexpr int TILE = SB * kSortItems;
     union {
-        typename TileSort::TempStorage sort;
-        typename TileScan::TempStorage scan;
-        unsigned short src[kSortTile];     // tile slot of each sorted position
+        typename SortTypes<SB>::Sort::TempStorage sort;
+        typename SortTypes<SB>::Scan::TempStorage scan;
+        unsigned short src[TILE];          // tile slot of each sorted position
     } tmp;
-    unsigned key[kSortTile];               // sorted keys, then destination slots
-    unsigned base[kSortTile];              // at a run's first position: its first rank in the cell
+    unsigned key[TILE];                    // sorted keys, then destination slots
 };
 
 // SCATTER false: counts[cell] += particles of the cell (the histogram).
@@ -1418,36 +1422,48 @@ struct SortTileLds {
 // and each particle's eight fields go to the species' other buffer set at
 // offs[cell] + rank.  The grid covers the capacity; blocks past the
 // population return at once, so a captured launch fits any population.
-template <bool SCATTER>
-__global__ void __launch_bounds__(kBlock)
+// SB threads sort a tile of SB * kSortItems slots: a larger tile gives
+// longer runs of equal cells, so the scatter writes fewer partial lines.
+template <bool SCATTER, int SB>
+__global__ void __launch_bounds__(SB)
 sort_tile_kernel(const DevState *__restrict__ stp, const DevCounters *__restrict__ ctr, int s,
                  unsigned long long *__restrict__ counts, const unsigned long long *__restrict__ offs)
 {
-    __shared__ SortTileLds L;
-    __shared__ double stage[SCATTER ? kSortTile : 1];   // one field of the tile (scatter pass)
-    // The tile's smallest and largest key are reduced in the last two words
-    // of L.key, which nothing writes before the counting (or the radix
-    // sort's own barriers) has passed: a separate variable would take the
-    // LDS block past 40 KB and the scatter pass to three blocks per CU.
-    unsigned &kmin_s = L.key[kSortTile - 2], &kmax_s = L.key[kSortTile - 1];
+    constexpr int TILE = SB * kSortItems;
+    using TileSort = typename SortTypes<SB>::Sort;
+    using TileScan = typename SortTypes<SB>::Scan;
+    __shared__ SortTileLds<SB> L;
+    // Scatter pass: one field of the tile at a time.  Before the fields, its
+    // first TILE words hold the counting bins (or, after a radix sort, each
+    // run's first rank in its cell) and the next TILE words the tile's first
+    // rank in each cell of its span; the histogram pass needs the bins only.
+    __shared__ double stage[SCATTER ? TILE : TILE / 2];
+    unsigned *const base = reinterpret_cast<unsigned *>(stage);
+    // The bounds of the tile's cells in z and r are reduced in the last four
+    // words of L.key, which nothing writes before the counting (or the radix
+    // sort's own barriers) has passed: separate variables would enlarge the
+    // LDS block past what two 4096-slot (four 2048-slot) blocks share on a CU.
+    unsigned &zlo_s = L.key[TILE - 4], &zhi_s = L.key[TILE - 3];
+    unsigned &rlo_s = L.key[TILE - 2], &rhi_s = L.key[TILE - 1];
     const DevState &st = *stp;
     const long long n = (long long)ctr->n[s];
-    const long long tbase = (long long)blockIdx.x * kSortTile;
+    const long long tbase = (long long)blockIdx.x * TILE;
     if (tbase >= n) return;
     const unsigned S = (unsigned)((st.nz - 1) * (st.nr - 1));
     const int t = threadIdx.x;
-    if (t == 0) { kmin_s = S; kmax_s = 0; }
+    const unsigned nzc = (unsigned)(st.nz - 1);     // cells per row (z fastest in the cell index)
+    if (t == 0) { zlo_s = ~0u; zhi_s = 0u; rlo_s = ~0u; rhi_s = 0u; }
     // Z and R of the thread's slots, read together (slots past the
     // population read the tile's first slot and take the sentinel key); the
     // scatter pass moves these values as its first two fields, and loads the
     // third while the keys are counted.
-    const int ntile = (int)min((long long)kSortTile, n - tbase);
+    const int ntile = (int)min((long long)TILE, n - tbase);
     double zv[kSortItems], rv[kSortItems], nx[kSortItems];
     {
         const auto *fz = gp(st.f[s][0]) + tbase, *fr = gp(st.f[s][1]) + tbase;
 #pragma unroll
         for (int j = 0; j < kSortItems; ++j) {
-            const int l = t + j * kBlock;
+            const int l = t + j * SB;
             zv[j] = __builtin_nontemporal_load(fz + (l < ntile ? l : 0));
             rv[j] = __builtin_nontemporal_load(fr + (l < ntile ? l : 0));
         }
@@ -1455,53 +1471,67 @@ sort_tile_kernel(const DevState *__restrict__ stp, const DevCounters *__restrict
             const auto *f2 = gp(st.f[s][2]) + tbase;
 #pragma unroll
             for (int j = 0; j < kSortItems; ++j) {
-                const int l = t + j * kBlock;
+                const int l = t + j * SB;
                 nx[j] = __builtin_nontemporal_load(f2 + (l < ntile ? l : 0));
             }
         }
     }
-    unsigned key[kSortItems];
+    // Cell coordinates (0-based) of the thread's slots; cr = ~0u marks a
+    // slot without a live particle.
+    unsigned cz[kSortItems], cr[kSortItems];
     unsigned short loc[kSortItems];
-    unsigned kmin = S, kmax = 0;
+    unsigned zlo = ~0u, zhi = 0u, rlo = ~0u, rhi = 0u;
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) {
-        const int l = t + j * kBlock;
-        unsigned k = S;
-        if (l < ntile && !is_dead(zv[j])) k = (unsigned)cell_of(st, zv[j], rv[j]);
-        key[j] = k;
+        const int l = t + j * SB;
+        cz[j] = 0u;
+        cr[j] = ~0u;
+        if (l < ntile && !is_dead(zv[j])) {
+            cz[j] = (unsigned)(min(max((int)ceil(zv[j]), 1), st.nz - 1) - 1);
+            cr[j] = (unsigned)(min(max((int)ceil(rv[j]), 1), st.nr - 1) - 1);
+            zlo = min(zlo, cz[j]); zhi = max(zhi, cz[j]);
+            rlo = min(rlo, cr[j]); rhi = max(rhi, cr[j]);
+        }
         loc[j] = (unsigned short)l;
-        if (k < S) { kmin = min(kmin, k); kmax = max(kmax, k); }
     }
-    // The tile's keys span a few hundred cells between sorts: sort them
-    // relative to the smallest, on only the bits that span needs (the
-    // sentinel becomes one past the largest), instead of on every bit of
-    // the cell index.
-    __syncthreads();                        // kmin_s / kmax_s initialised
-    atomicMin(&kmin_s, kmin);
-    atomicMax(&kmax_s, kmax);
+    const int lane = t & 63;
+    for (int o = 32; o > 0; o >>= 1) {
+        zlo = min(zlo, (unsigned)__shfl_xor((int)zlo, o)); zhi = max(zhi, (unsigned)__shfl_xor((int)zhi, o));
+        rlo = min(rlo, (unsigned)__shfl_xor((int)rlo, o)); rhi = max(rhi, (unsigned)__shfl_xor((int)rhi, o));
+    }
+    __syncthreads();                        // bounds initialised
+    if (lane == 0) {
+        atomicMin(&zlo_s, zlo); atomicMax(&zhi_s, zhi);
+        atomicMin(&rlo_s, rlo); atomicMax(&rhi_s, rhi);
+    }
     __syncthreads();
-    kmin = kmin_s;
-    kmax = kmax_s;
-    if (kmin > kmax) return;                // no live particle in the tile (uniform)
-    const unsigned span = kmax - kmin + 1;  // relative key of the sentinel
-    if (span <= (unsigned)kSortTile) {
-        // Counted in LDS, one bin per cell of the span: a wave's 64 slots
+    zlo = zlo_s; zhi = zhi_s; rlo = rlo_s; rhi = rhi_s;
+    if (rlo > rhi) return;                  // no live particle in the tile (uniform)
+    // The tile's cells lie in the box [zlo, zhi] x [rlo, rhi]: between sorts
+    // a few dozen cells in z by a few rows in r, a few hundred cells in all
+    // though their cell indices span several rows of the grid.  Numbered
+    // row by row within the box, they keep the order of the cell index.
+    const unsigned W = zhi - zlo + 1u, H = rhi - rlo + 1u;
+    unsigned key[kSortItems];
+    if ((unsigned long long)W * H <= (unsigned long long)TILE) {
+        const unsigned span = W * H;        // box index of the sentinel
+#pragma unroll
+        for (int j = 0; j < kSortItems; ++j) key[j] = cr[j] != ~0u ? (cr[j] - rlo) * W + (cz[j] - zlo) : span;
+        // Counted in LDS, one bin per cell of the box: a wave's 64 slots
         // (consecutive) fall into a few runs of equal cells, and each run
         // takes its ranks with one LDS atomic; each nonzero bin then costs
         // one atomic on its cell's counter.  The bins' prefix sums place
-        // the runs in the tile's sorted order.  (Tiles whose cells span
-        // more than kSortTile -- the products appended since the last sort
-        // -- are radix-sorted below.)
-        unsigned *bins = L.base;
-        for (int k = t; k < (int)span; k += kBlock) bins[k] = 0u;
+        // the runs in the tile's sorted order.  (Tiles whose box holds
+        // more than TILE cells -- the products appended since the last
+        // sort -- are radix-sorted below.)
+        unsigned *bins = base;
+        for (int k = t; k < (int)span; k += SB) bins[k] = 0u;
         __syncthreads();
-        const int lane = t & 63;
         const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
         unsigned rank[kSortItems];
 #pragma unroll
         for (int j = 0; j < kSortItems; ++j) {
-            const unsigned rel = key[j] < S ? key[j] - kmin : span;
-            key[j] = rel;
+            const unsigned rel = key[j];
             const unsigned kp = __shfl_up(rel, 1);
             const unsigned long long starts = __ballot(lane == 0 || kp != rel);
             const int first = 63 - __clzll((long long)(starts & upto));
@@ -1521,18 +1551,21 @@ sort_tile_kernel(const DevState *__restrict__ stp, const DevCounters *__restrict
             c8[q] = k < (int)span ? bins[k] : 0u;
             tsum += c8[q];
         }
-        // the tile's first rank within each of its cells (scatter pass: kept
-        // in the staging buffer, which the fields use only later)
-        unsigned *gb = SCATTER ? reinterpret_cast<unsigned *>(stage) : nullptr;
+        // the destination of each bin's first particle: its cell's offset
+        // plus the tile's first rank within the cell (scatter pass: kept in
+        // the staging buffer, which the fields use only later)
+        unsigned *gb = SCATTER ? base + TILE : nullptr;
+        unsigned bz = (unsigned)(t * kSortItems) % W, br = (unsigned)(t * kSortItems) / W;
 #pragma unroll
         for (int q = 0; q < kSortItems; ++q) {
             if (c8[q]) {
                 const int k = t * kSortItems + q;
-                const unsigned long long g = __hip_atomic_fetch_add(gp(&counts[kmin + (unsigned)k]),
-                                                                    (unsigned long long)c8[q], __ATOMIC_RELAXED,
-                                                                    __HIP_MEMORY_SCOPE_AGENT);
-                if (SCATTER) gb[k] = (unsigned)g;
+                const unsigned cell = (zlo + bz) + (rlo + br) * nzc;
+                const unsigned long long g = __hip_atomic_fetch_add(gp(&counts[cell]), (unsigned long long)c8[q],
+                                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (SCATTER) gb[k] = (unsigned)(gp(offs)[cell] + g);
             }
+            if (++bz == W) { bz = 0u; ++br; }
         }
         if (!SCATTER) return;
         int excl = 0, total = 0;
@@ -1545,7 +1578,7 @@ sort_tile_kernel(const DevState *__restrict__ stp, const DevCounters *__restrict
             if (k < (int)span) bins[k] = run;   // first sorted position of the bin
             run += c8[q];
         }
-        for (int p = total + t; p < kSortTile; p += kBlock) {   // positions past the tile's particles
+        for (int p = total + t; p < TILE; p += SB) {   // positions past the tile's particles
             L.key[p] = 0xffffffffu;
             L.tmp.src[p] = 0;
         }
@@ -1554,15 +1587,20 @@ sort_tile_kernel(const DevState *__restrict__ stp, const DevCounters *__restrict
         for (int j = 0; j < kSortItems; ++j) {
             if (key[j] < span) {
                 const unsigned p = bins[key[j]] + rank[j];
-                L.tmp.src[p] = (unsigned short)(t + j * kBlock);
-                L.key[p] = (unsigned)(gp(offs)[kmin + key[j]] + gb[key[j]] + rank[j]);
+                L.tmp.src[p] = (unsigned short)(t + j * SB);
+                L.key[p] = gb[key[j]] + rank[j];
             }
         }
     } else {
+        // radix-sorted on the cell index relative to the box's first cell,
+        // on only the bits up to the box's last cell (the sentinel, one past
+        // it, takes the last place)
+        const unsigned kmin = zlo + rlo * nzc;
+        const unsigned span = zhi + rhi * nzc - kmin + 1u;
         int nbits = 0;
         while (nbits < 32 && (span >> nbits) != 0) ++nbits;
 #pragma unroll
-        for (int j = 0; j < kSortItems; ++j) key[j] = key[j] < S ? key[j] - kmin : span;
+        for (int j = 0; j < kSortItems; ++j) key[j] = cr[j] != ~0u ? cz[j] + cr[j] * nzc - kmin : span;
         TileSort(L.tmp.sort).Sort(key, loc, 0, nbits);        // blocked: thread t holds 8t .. 8t+7
 #pragma unroll
         for (int j = 0; j < kSortItems; ++j) key[j] = key[j] < span ? key[j] + kmin : S;
@@ -1579,12 +1617,12 @@ sort_tile_kernel(const DevState *__restrict__ stp, const DevCounters *__restrict
 #pragma unroll
         for (int j = 0; j < kSortItems; ++j) {
             const int p = t * kSortItems + j;
-            const bool run_end = p == kSortTile - 1 || L.key[p + 1] != key[j];
+            const bool run_end = p == TILE - 1 || L.key[p + 1] != key[j];
             if (run_end && key[j] < S) {
                 const unsigned long long len = (unsigned long long)(p - start[j] + 1);
                 const unsigned long long b =
                     __hip_atomic_fetch_add(gp(&counts[key[j]]), len, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (SCATTER) L.base[start[j]] = (unsigned)b;
+                if (SCATTER) base[start[j]] = (unsigned)b;
             }
         }
         if (!SCATTER) return;
@@ -1592,7 +1630,7 @@ sort_tile_kernel(const DevState *__restrict__ stp, const DevCounters *__restrict
 #pragma unroll
         for (int j = 0; j < kSortItems; ++j) {
             const int p = t * kSortItems + j;
-            L.key[p] = key[j] < S ? (unsigned)(gp(offs)[key[j]] + L.base[start[j]] + (unsigned)(p - start[j])) : 0xffffffffu;
+            L.key[p] = key[j] < S ? (unsigned)(gp(offs)[key[j]] + base[start[j]] + (unsigned)(p - start[j])) : 0xffffffffu;
             L.tmp.src[p] = loc[j];
         }
     }
@@ -1613,23 +1651,23 @@ sort_tile_kernel(const DevState *__restrict__ stp, const DevCounters *__restrict
     const unsigned spare = (unsigned)st.cap[s];
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) {
-        const unsigned d = L.key[t + j * kBlock];
+        const unsigned d = L.key[t + j * SB];
         dd[j] = d != 0xffffffffu ? d : spare;
-        ss[j] = L.tmp.src[t + j * kBlock];
+        ss[j] = L.tmp.src[t + j * SB];
     }
 #pragma unroll
     for (int f = 0; f < 8; ++f) {
         __builtin_amdgcn_s_waitcnt(0xC07F);       // lgkmcnt(0): this thread's LDS reads of stage are done
         __builtin_amdgcn_s_barrier();            // ... and every other thread's: stage may be overwritten
 #pragma unroll
-        for (int j = 0; j < kSortItems; ++j) stage[t + j * kBlock] = f == 0 ? zv[j] : f == 1 ? rv[j] : nx[j];
+        for (int j = 0; j < kSortItems; ++j) stage[t + j * SB] = f == 0 ? zv[j] : f == 1 ? rv[j] : nx[j];
         __builtin_amdgcn_s_waitcnt(0xC07F);
         __builtin_amdgcn_s_barrier();            // stage written
         if (f >= 2 && f < 7) {
             const auto *sf = gp(st.f[s][f + 1]) + tbase;
 #pragma unroll
             for (int j = 0; j < kSortItems; ++j) {
-                const int l = t + j * kBlock;
+                const int l = t + j * SB;
                 nx[j] = __builtin_nontemporal_load(sf + (l < ntile ? l : 0));
             }
         }
@@ -1960,15 +1998,27 @@ static int sort_enqueue(ipm_ctx *c, int s0, int s1)
         if (!c->species_set[sp]) continue;
         hipLaunchKernelGGL(clear_u64_kernel, dim3(cblk), dim3(kBlock), 0, c->stream, c->hist, ncell);
         hipLaunchKernelGGL(clear_u64_kernel, dim3(cblk), dim3(kBlock), 0, c->stream, c->cursor, ncell);
-        const unsigned pblk = (unsigned)(((size_t)c->h.cap[sp] + kSortTile - 1) / kSortTile);
-        hipLaunchKernelGGL(sort_tile_kernel<false>, dim3(pblk), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr, sp,
-                           c->hist, (const unsigned long long *)nullptr);
+        const bool wide = c->sort_block == 2 * kBlock;
+        const size_t tile = wide ? kSortTileMax : kSortTile;
+        const unsigned pblk = (unsigned)(((size_t)c->h.cap[sp] + tile - 1) / tile);
+        const dim3 sblk(wide ? 2 * kBlock : kBlock);
+        const unsigned long long *no_offs = nullptr;
+        if (wide)
+            hipLaunchKernelGGL((sort_tile_kernel<false, 2 * kBlock>), dim3(pblk), sblk, 0, c->stream, c->d_state,
+                               c->d_ctr, sp, c->hist, no_offs);
+        else
+            hipLaunchKernelGGL((sort_tile_kernel<false, kBlock>), dim3(pblk), sblk, 0, c->stream, c->d_state, c->d_ctr,
+                               sp, c->hist, no_offs);
         hipLaunchKernelGGL(scan_block_kernel, dim3(nblk), dim3(kScanBlock), 0, c->stream, c->hist, c->offs, c->partial, ncell);
         hipLaunchKernelGGL(scan_block_kernel, dim3(1), dim3(kScanBlock), 0, c->stream, c->partial, c->partial2,
                            c->partial + nblk, (size_t)nblk);
         hipLaunchKernelGGL(scan_add_kernel, dim3(nblk), dim3(kScanBlock), 0, c->stream, c->offs, c->partial2, ncell);
-        hipLaunchKernelGGL(sort_tile_kernel<true>, dim3(pblk), dim3(kBlock), 0, c->stream, c->d_state, c->d_ctr, sp,
-                           c->cursor, c->offs);
+        if (wide)
+            hipLaunchKernelGGL((sort_tile_kernel<true, 2 * kBlock>), dim3(pblk), sblk, 0, c->stream, c->d_state,
+                               c->d_ctr, sp, c->cursor, c->offs);
+        else
+            hipLaunchKernelGGL((sort_tile_kernel<true, kBlock>), dim3(pblk), sblk, 0, c->stream, c->d_state, c->d_ctr,
+                               sp, c->cursor, c->offs);
         hipLaunchKernelGGL(set_count_kernel, dim3(1), dim3(64), 0, c->stream, c->d_ctr, sp, c->offs, c->hist, ncell);
         hipLaunchKernelGGL(swap_buffers_kernel, dim3(1), dim3(64), 0, c->stream, c->d_state, sp);
         if (c->tanc) {
@@ -2795,6 +2845,16 @@ int ipm_set_sort_interval(ipm_ctx *c, int steps)
     if (steps < 0) return fail(IPM_EINVAL, "sort interval must be >= 0, got %d", steps);
     c->sort_every = steps;
     c->since_sort = 0;
+    return IPM_OK;
+}
+
+int ipm_set_sort_tile(ipm_ctx *c, int slots)
+{
+    if (!c) return fail(IPM_EINVAL, "null context");
+    if (slots != kSortTile && slots != kSortTileMax)
+        return fail(IPM_EINVAL, "sort tile must be %d or %d slots, got %d", kSortTile, kSortTileMax, slots);
+    c->sort_block = slots / kSortItems;
+    drop_graphs(c);
     return IPM_OK;
 }
 

@@ -179,6 +179,11 @@ module ipm_hip
             type(c_ptr), value :: ctx
             integer(c_int), value :: rows
         end function
+        integer(c_int) function ipm_set_sort_tile(ctx, slots) bind(C, name="ipm_set_sort_tile")
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: slots
+        end function
         integer(c_int) function ipm_set_sort_interval(ctx, steps) bind(C, name="ipm_set_sort_interval")
             import :: c_ptr, c_int
             type(c_ptr), value :: ctx

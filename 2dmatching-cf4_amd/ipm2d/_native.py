@@ -72,6 +72,7 @@ SIGNATURES = {
     "ipm_set_sort_interval": [_vp, C.c_int],
     "ipm_set_launch": [_vp, C.c_int, C.c_int],
     "ipm_set_deposit_window": [_vp, C.c_int],
+    "ipm_set_sort_tile": [_vp, C.c_int],
     "ipm_set_collide_grid": [_vp, C.c_int],
     "ipm_synchronize": [_vp],
     "ipm_species_device_ptrs": [_vp, C.c_int, C.POINTER(_dp)],

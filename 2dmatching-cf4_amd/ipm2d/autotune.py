@@ -17,6 +17,8 @@ live simulation: every candidate advances the real state by a few steps.
   particle pass but keeps the deposit's LDS window and wave reduction
   effective; the interval minimising the mean step time over a full
   interval (sort included) is kept.
+* ``sort_tile``: slots per tile of the cell sort (2048 or 4096), chosen by
+  timing one sort after a full interval of drift with each.
 
 Results can be cached in a JSON file keyed by grid, species count and
 particle count (rounded), so a production run tunes once.
@@ -102,13 +104,35 @@ def tune_sort_interval(hp, candidates=(5, 10, 20, 40), timer=None):
     return best, res
 
 
+def tune_sort_tile(hp, interval, candidates=(2048, 4096), timer=None):
+    """Time of one cell sort after `interval` steps of drift from a sorted
+    state, per sort tile size in slots; returns (best, {slots: seconds})."""
+    res = {}
+    for tile in candidates:
+        hp.set_sort_tile(tile)
+        hp.sort(-1)
+        if timer is not None:
+            res[tile] = timer(tile)
+        else:
+            for _ in range(interval):
+                hp.one_step()
+            hp.synchronize()
+            t0 = time.perf_counter()
+            hp.sort(-1)
+            hp.synchronize()
+            res[tile] = time.perf_counter() - t0
+    best = min(res, key=res.get)
+    hp.set_sort_tile(best)
+    return best, res
+
+
 def cache_key(hp, nparticles):
     return f"{hp.nz}x{hp.nr}_ns{hp.ns}_n{int(round(nparticles, -5))}"
 
 
 def autotune(hp, nparticles, cache_path=None, bpc_candidates=(0, 2, 4, 6, 8),
              sort_candidates=(5, 10, 20, 40, 60, 80), steps=6, collide_candidates=(2, 4, 6),
-             tile_candidates=TILE_SHAPES, tile_lead=10):
+             tile_candidates=TILE_SHAPES, tile_lead=10, sort_tile_candidates=(2048, 4096)):
     """Tune (or load from cache) and apply; returns the chosen settings."""
     key = cache_key(hp, nparticles)
     if cache_path and os.path.exists(cache_path):
@@ -118,12 +142,15 @@ def autotune(hp, nparticles, cache_path=None, bpc_candidates=(0, 2, 4, 6, 8),
             hp.set_launch(cache[key]["blocks_per_cu"])
             hp.set_tile_shape(*cache[key].get("tile_shape", (256, 24)))
             hp.set_collide_grid(cache[key].get("collide_blocks_per_cu", 6))
+            hp.set_sort_tile(cache[key].get("sort_tile", 2048))
             return cache[key]
     bpc, tb = tune_blocks_per_cu(hp, bpc_candidates, steps)
     shape, tt = tune_tile_shape(hp, tile_candidates, steps, tile_lead) if tile_candidates else ((256, 24), {})
     cbpc, tc = tune_collide_grid(hp, collide_candidates, steps) if collide_candidates else (6, {})
     si, ts = tune_sort_interval(hp, sort_candidates)
+    stile, tst = tune_sort_tile(hp, si, sort_tile_candidates) if (sort_tile_candidates and si) else (2048, {})
     out = {"blocks_per_cu": bpc, "tile_shape": list(shape), "collide_blocks_per_cu": cbpc, "sort_interval": si,
+           "sort_tile": stile, "sort_seconds_by_sort_tile": {str(k): v for k, v in tst.items()},
            "step_seconds_by_tile_shape": {f"{k[0]}x{k[1]}": v for k, v in tt.items()},
            "step_seconds_by_blocks_per_cu": {str(k): v for k, v in tb.items()},
            "step_seconds_by_collide_blocks_per_cu": {str(k): v for k, v in tc.items()},

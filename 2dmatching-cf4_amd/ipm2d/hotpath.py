@@ -164,6 +164,15 @@ class HotPath:
     def tile_shape(self):
         return self._block, self._winr
 
+    def set_sort_tile(self, slots=2048):
+        """Slots per tile of the cell sort (2048 or 4096; results unchanged)."""
+        N.call("ipm_set_sort_tile", self._h, int(slots))
+        self._sort_tile = int(slots)
+
+    @property
+    def sort_tile(self):
+        return getattr(self, "_sort_tile", 2048)
+
     def set_collide_grid(self, blocks_per_cu=6):
         N.call("ipm_set_collide_grid", self._h, int(blocks_per_cu))
 

@@ -591,8 +591,9 @@ def main():
                 else f"{tuned['blocks_per_cu'] if tuned else args.blocks_per_cu} persistent blocks per CU") + (
                 f", {tuned['tile_shape'][0]} threads per tile ({2048 // tuned['tile_shape'][0]} particles per "
                 f"thread), {tuned['tile_shape'][1]}-row deposit window, collision pass "
-                f"{tuned['collide_blocks_per_cu']} blocks per CU" if tuned else ""),
-            "autotune": ({k: v for k, v in tuned.items() if k.startswith("step_seconds")} if tuned else None),
+                f"{tuned['collide_blocks_per_cu']} blocks per CU, {tuned['sort_tile']}-slot sort tiles" if tuned else ""),
+            "autotune": ({k: v for k, v in tuned.items() if k.startswith(("step_seconds", "sort_seconds"))}
+                         if tuned else None),
         },
         "roofline": {
             "bound": "hbm",

@@ -176,6 +176,12 @@ int ipm_sort(ipm_ctx *ctx, int s);
  * synchronization at all.  A scheduling choice: results do not depend on it
  * (the sort changes storage order only). */
 int ipm_set_sort_interval(ipm_ctx *ctx, int steps);
+/* Slots per tile of the cell sort: 2048 (256 threads, the default) or 4096
+ * (512 threads).  A tile's particles of one cell go to consecutive slots, so
+ * a larger tile writes longer runs.  A scheduling choice with no counterpart
+ * in the reference (which does not sort): the sorted set, its cell order and
+ * every later result do not depend on it. */
+int ipm_set_sort_tile(ipm_ctx *ctx, int slots);
 
 /* Launch shape of the fused particle pass (ipm_step, ipm_step_graph):
  * block = threads per block of a tile of 2048 slots, 128, 256 or 512 (16, 8

@@ -28,6 +28,10 @@ class FakeHP:
         self.cbpc = b
         self.calls.append(("collide", b))
 
+    def set_sort_tile(self, slots):
+        self.sort_tile = slots
+        self.calls.append(("sort_tile", slots))
+
     def sort(self, s=-1):
         self.calls.append(("sort", s))
 
@@ -70,6 +74,16 @@ def test_picks_fastest_sort_interval():
     assert best == min(res, key=res.get) == 20
 
 
+def test_picks_fastest_sort_tile():
+    hp = FakeHP()
+    cost = {2048: 2.3e-3, 4096: 2.4e-3}
+    best, res = autotune.tune_sort_tile(hp, 60, candidates=list(cost), timer=lambda tile: cost[tile])
+    assert best == 2048 and hp.sort_tile == 2048 and res == cost
+    # each candidate is timed from a sorted state; the winner is applied last
+    assert hp.calls[-1] == ("sort_tile", 2048)
+    assert hp.calls.count(("sort", -1)) == len(cost)
+
+
 def test_cache_round_trip(tmp_path, monkeypatch):
     hp = FakeHP()
     path = str(tmp_path / "tune.json")
@@ -77,12 +91,14 @@ def test_cache_round_trip(tmp_path, monkeypatch):
     monkeypatch.setattr(autotune, "tune_sort_interval", lambda h, c: (10, {10: 1.0}))
     monkeypatch.setattr(autotune, "tune_collide_grid", lambda h, c, s: (2, {2: 1.0}))
     monkeypatch.setattr(autotune, "tune_tile_shape", lambda h, c, s, l: ((128, 24), {(128, 24): 1.0}))
+    monkeypatch.setattr(autotune, "tune_sort_tile", lambda h, i, c: (4096, {4096: 1.0}))
     out = autotune.autotune(hp, 2.46e6, cache_path=path)
     assert out["blocks_per_cu"] == 5 and out["sort_interval"] == 10 and out["collide_blocks_per_cu"] == 2
     assert out["tile_shape"] == [128, 24]
     monkeypatch.setattr(autotune, "tune_blocks_per_cu", lambda *a: pytest.fail("should use the cache"))
     hp2 = FakeHP()
     out2 = autotune.autotune(hp2, 2.46e6, cache_path=path)
+    assert out2["sort_tile"] == 4096 and hp2.sort_tile == 4096
     assert out2["blocks_per_cu"] == 5 and hp2.bpc == 5 and hp2.cbpc == 2 and hp2.shape == (128, 24)
     assert autotune.cache_key(hp, 2.46e6) in json.load(open(path))
 
@@ -103,9 +119,10 @@ def test_tuning_does_not_change_results(rng):
         hp.set_rng(3, 0)
     out = autotune.autotune(a, 1.2e5, bpc_candidates=(2, 6), sort_candidates=(3, 5), steps=2,
                             collide_candidates=(1, 6), tile_lead=1)
-    # a advanced 4 x (1 untimed + 2 timed) + 5 x (1 + 2) + 3 + 5 = 35 steps (plus sorts);
-    # b takes 35 plain steps
-    for _ in range(35):
+    # a advanced 4 x (1 untimed + 2 timed) + 5 x (1 + 2) + 3 + 5 = 35 steps (plus sorts),
+    # then one interval per sort tile size; b takes as many plain steps
+    assert out["sort_tile"] in (2048, 4096)
+    for _ in range(35 + 2 * out["sort_interval"]):
         b.one_step()
     for s in range(cfg.ns):
         ga, gb = sort_set(a.download(s)), sort_set(b.download(s))

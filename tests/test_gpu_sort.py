@@ -35,7 +35,8 @@ def _hashes(P):
     return np.sort(h)
 
 
-def test_sort_after_drift():
+@pytest.mark.parametrize("tile", [2048, 4096])
+def test_sort_after_drift(tile):
     cfg = W.make_config("c1")
     geom = W.make_geometry(cfg)
     tables = W.build_tables(cfg)
@@ -43,6 +44,7 @@ def test_sort_after_drift():
     parts = [W.load_species(cfg, geom, s, rng) for s in range(cfg.ns)]
     a, _, _, _ = ipm2d.from_config(cfg, geom=geom, tables=tables, particles=parts, capacity_factor=1.5)
     b, _, _, _ = ipm2d.from_config(cfg, geom=geom, tables=tables, particles=parts, capacity_factor=1.5)
+    a.set_sort_tile(tile)
     for _ in range(30):
         a.one_step()
         b.one_step()
@@ -67,6 +69,29 @@ def test_sort_after_drift():
                                       err_msg=f"species {s} after the sort")
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("tile", [2048, 4096])
+def test_sort_of_shuffled_population(tile):
+    """Slots in random order: every tile spans nearly the whole grid, so the
+    tiles are radix-sorted rather than counted in LDS bins.  The sorted set
+    and its cell order must not depend on the tile size."""
+    cfg = W.make_config("c1")
+    geom = W.make_geometry(cfg)
+    tables = W.build_tables(cfg)
+    rng = np.random.default_rng(7)
+    parts = [W.load_species(cfg, geom, s, rng) for s in range(cfg.ns)]
+    parts = [P[:, rng.permutation(P.shape[1])] for P in parts]
+    a, _, _, _ = ipm2d.from_config(cfg, geom=geom, tables=tables, particles=parts, capacity_factor=1.5)
+    a.set_sort_tile(tile)
+    a.sort(-1)
+    for s in range(cfg.ns):
+        P = a.download(s)
+        assert P.shape[1] == parts[s].shape[1] == a.count(s)
+        np.testing.assert_array_equal(_hashes(P), _hashes(parts[s]), err_msg=f"species {s}")
+        cell = (np.ceil(P[0]) - 1) + (np.maximum(np.ceil(P[1]), 1) - 1) * (cfg.nz - 1)
+        assert np.all(np.diff(cell) >= 0), f"species {s} not in cell order"
+    a.close()
 
 
 def test_sort_interval_in_graph_replays():
