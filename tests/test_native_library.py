@@ -35,6 +35,15 @@ def test_python_binding_covers_header():
     assert declared <= set(_native.SIGNATURES), sorted(declared - set(_native.SIGNATURES))
 
 
+def test_fortran_module_binds_header():
+    """The iso_c_binding module the Fortran host uses names every entry
+    point of the header (the reference's side of the boundary is Fortran)."""
+    mod = open(os.path.join(ROOT, "2dmatching-cf4_amd", "fortran", "ipm_hip_mod.f90")).read()
+    bound = set(re.findall(r'bind\(C,\s*name="(ipm_[a-z_0-9]+)"\)', mod))
+    missing = sorted(set(declared_symbols()) - bound)
+    assert not missing, f"not bound in ipm_hip_mod.f90: {missing}"
+
+
 def test_create_without_gpu_fails_cleanly():
     """Without a device the library reports IPM_ENODEV instead of crashing."""
     from conftest import gpu_available
