@@ -94,6 +94,43 @@ def test_sort_of_shuffled_population(tile):
     a.close()
 
 
+@pytest.mark.parametrize("tile", [2048, 4096])
+def test_sort_of_degenerate_boxes(tile):
+    """Tiles whose cells form degenerate boxes: one cell (a box of one bin),
+    one full row of cells in z, and the grid's four corner cells (a box of
+    the whole grid: radix-sorted on every bit of the cell index), with
+    particles on the outermost cells' faces.  Each species keeps its
+    particles and ends in cell order."""
+    cfg = W.make_config("c1")
+    geom = W.make_geometry(cfg)
+    tables = W.build_tables(cfg)
+    rng = np.random.default_rng(11)
+    parts = [W.load_species(cfg, geom, s, rng)[:, :6000] for s in range(cfg.ns)]
+    n = parts[0].shape[1]
+    parts[0][0] = 10.0 + rng.uniform(0.01, 0.99, n)              # all in cell (11, 21)
+    parts[0][1] = 20.0 + rng.uniform(0.01, 0.99, n)
+    n = parts[1].shape[1]
+    parts[1][0] = rng.uniform(0.0, cfg.nz - 1.0, n)               # one row, every z
+    parts[1][1] = 5.0 + rng.uniform(0.01, 0.99, n)
+    n = parts[2].shape[1]
+    corner = rng.integers(0, 4, n)
+    parts[2][0] = np.where(corner & 1, cfg.nz - 1.0 - rng.uniform(0.0, 0.5, n), rng.uniform(0.0, 0.5, n))
+    parts[2][1] = np.where(corner & 2, cfg.nr - 1.0 - rng.uniform(0.0, 0.5, n), rng.uniform(0.0, 0.5, n))
+    parts = [P[:, rng.permutation(P.shape[1])] for P in parts]
+    a, _, _, _ = ipm2d.from_config(cfg, geom=geom, tables=tables, particles=parts, capacity_factor=1.5)
+    a.set_sort_tile(tile)
+    a.sort(-1)
+    for s in range(cfg.ns):
+        P = a.download(s)
+        assert P.shape[1] == parts[s].shape[1] == a.count(s)
+        np.testing.assert_array_equal(_hashes(P), _hashes(parts[s]), err_msg=f"species {s}")
+        iz = np.clip(np.ceil(P[0]), 1, cfg.nz - 1) - 1
+        ir = np.clip(np.ceil(P[1]), 1, cfg.nr - 1) - 1
+        cell = iz + ir * (cfg.nz - 1)
+        assert np.all(np.diff(cell) >= 0), f"species {s} not in cell order"
+    a.close()
+
+
 def test_sort_interval_in_graph_replays():
     """ipm_set_sort_interval: the sort every 7 steps replayed as a graph of
     its own inside ipm_step_graph equals explicit eager sorts at the same
